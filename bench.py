@@ -1,0 +1,155 @@
+"""MDDF benchmark on one MI355X: frames/s on the 5k-atom solute / 40k-water system.
+
+Workload (one step = one full mddf job over the trajectory, inputs resident in
+HBM): cubic 110 A box, 5,000-atom random-walk globule (radius ~25 A), 40,000
+rigid 3-site waters outside it, 200 jittered frames, FP32 coordinates,
+Options(): cutoff = dbulk = 10 A, binstep 0.02 A, n_random_samples 10, all
+5,000 per-atom solute groups.
+
+Prints one JSON line.  roofline: FP32 VALU work of the minimum-distance
+kernels (12 FP32 ops per useful pair, W pairs per frame) over their measured
+time (HIP events on the job's stream).  cpu_baseline: the CPU FP64
+restatement of the reference (oracle/, cell lists, the reference's chunked
+parallel setup) timed on a bounded sample of the same frames.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+import __graft_entry__ as G  # noqa: E402
+
+METRIC = "frames/s (5k-atom solute, 40k waters) GPU vs CPU; served jobs/s + p99 latency"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def work_per_frame(n_u, n_v_atoms, volume, n_rand, cutoff=10.0):
+    """Useful (within-cutoff at uniform density) pair evaluations per frame:
+    W = N_u rho_v V_c + n_rand n_u rho_v V_c (single-molecule solute)."""
+    rho = n_v_atoms / volume
+    vc = 4.0 / 3.0 * math.pi * cutoff ** 3
+    return n_u * rho * vc * (1 + n_rand)
+
+
+def bytes_per_frame(n_u, n_v_atoms, m_v, n_rand, coord_bytes=4):
+    """Algorithmic HBM bytes per frame: coordinates in, placed solvent written
+    and read per random pass, one 16-byte record per solvent molecule per pass."""
+    return 3 * coord_bytes * (n_u + n_v_atoms) + n_rand * 24 * n_v_atoms + 16 * m_v * (1 + n_rand)
+
+
+def cpu_baseline(pkg, O, s, opts, tm, nframes_sample, threads):
+    from complexmixtures_amd.trajectory import convert_unitcell
+    job = O.Job(1, s["solute"].natomspermol, s["solvent"].nmols, s["solvent"].natomspermol,
+                irefatom=tm.irefatom - 1, cutoff=opts.cutoff, dbulk=opts.dbulk,
+                n_random_samples=opts.n_random_samples, seed=opts.seed, nthreads=threads)
+    f = nframes_sample
+    xu = s["x_solute"][:f].astype(np.float64)
+    xv = s["x_solvent"][:f].astype(np.float64)
+    cells = np.stack([convert_unitcell(c, True) for c in s["cells"][:f]])
+    t = time.perf_counter()
+    O.run(job, list(range(1, f + 1)), xu, xv, cells, np.ones(f))
+    dt = time.perf_counter() - t
+    return f / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=200)
+    ap.add_argument("--cpu-frames", type=int, default=0, help="CPU baseline sample (0: one frame per thread)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; the path shards by frames across GPUs "
+                                   "(see DESIGN.md) but multi-GPU runs are not part of it"}))
+        sys.exit(2)
+    pkg = G.load_package()
+    import complexmixtures_amd.synthetic as S
+    from complexmixtures_amd import _lib
+    from complexmixtures_amd.mddf import TrajectoryMetaData, _job_for
+    from complexmixtures_amd.results import make_result
+    from complexmixtures_amd.trajectory import convert_unitcell
+
+    t0 = time.perf_counter()
+    s = S.config2(nframes=args.frames)
+    gen_s = time.perf_counter() - t0
+    opts = pkg.Options(dbulk=10.0, binstep=0.02, n_random_samples=10, seed=20261204, silent=True)
+    traj = pkg.MemoryTrajectory(s["x_solute"], s["x_solvent"], s["cells"], s["solute"], s["solvent"])
+    tm = TrajectoryMetaData(traj, opts)
+    R = make_result(traj, opts, tm)
+    nf = args.frames
+    du = _lib.DeviceArray(s["x_solute"])
+    dv = _lib.DeviceArray(s["x_solvent"])
+    cells = np.stack([convert_unitcell(c, tm.orthorhombic) for c in s["cells"]])
+    frames = np.arange(1, nf + 1)
+    weights = np.ones(nf)
+
+    def one_job(profile=False):
+        job = _job_for(R, opts, tm, False)
+        if profile:
+            job.set_profiling(True)
+        job.submit_device(frames, du, dv, 0, cells, weights)
+        raw = job.finish()
+        st = job.stats() if profile else None
+        job.close()
+        return raw, st
+
+    for _ in range(args.warmup):
+        one_job()
+    _lib.check(_lib.lib().cm_device_synchronize())
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        raw, _ = one_job()
+    _lib.check(_lib.lib().cm_device_synchronize())
+    el = time.perf_counter() - t
+    fps = nf * args.steps / el
+    _, st = one_job(profile=True)
+
+    n_u = s["solute"].natomspermol
+    n_v = s["solvent"].nmols * s["solvent"].natomspermol
+    vol = float(np.linalg.det(cells[0]))
+    W = work_per_frame(n_u, n_v, vol, opts.n_random_samples, opts.dbulk)
+    ops = 12.0 * W * nf
+    eval_s = st["eval_ms"] / 1e3
+    achieved = ops / eval_s / 1e12
+    bpf = bytes_per_frame(n_u, n_v, s["solvent"].nmols, opts.n_random_samples)
+    out = {
+        "metric": METRIC, "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "mddf: 110 A cubic box, 5,000-atom globule solute, 40,000 3-site waters, "
+                               f"{nf} frames, FP32 coordinates, cutoff 10 A, binstep 0.02 A, n_random_samples 10",
+                   "frames_per_step": nf, "solute_atoms": n_u, "solvent_molecules": s["solvent"].nmols},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": None,
+                     "kernel": "k_eval_atoms (real + random passes)", "pairs_per_frame": round(W),
+                     "eval_ms_per_frame": round(st["eval_ms"] / nf, 5),
+                     "hbm_GBps_algorithmic": round(bpf * nf / eval_s / 1e9, 2),
+                     "fp64_exhaustive_resolves": st["fp64_rechecks"]},
+        "gen_s": round(gen_s, 2),
+    }
+    if not args.no_cpu:
+        O = G.load_oracle()
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        f = args.cpu_frames or threads
+        v, dt = cpu_baseline(pkg, O, s, opts, tm, f, threads)
+        out["cpu_baseline"] = {"value": round(v, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+                               "sample": f"first {f} frames of the same trajectory, {dt:.1f} s",
+                               "gpu_over_cpu": round(fps / v, 1)}
+    print(json.dumps(out), flush=True)
+    du.free()
+    dv.free()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,210 @@
+"""ctypes binding of libcmgpu.so (include/cmgpu.h).
+
+The device library is the only compute path: if it is not built, or no GPU
+is visible, calls raise -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .options import ArgumentError
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libcmgpu.so")
+
+CM_OK, CM_EINVAL, CM_EHIP, CM_ENOMEM, CM_EUNSUPPORTED, CM_ESTATE = 0, -1, -2, -3, -4, -5
+CM_F64, CM_F32 = 0, 1
+
+_P = ctypes.POINTER
+_i32, _i64, _u64, _d = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [("nmols_u", _i32), ("natoms_per_mol_u", _i32), ("nmols_v", _i32), ("natoms_per_mol_v", _i32),
+                ("autocorrelation", _i32), ("irefatom", _i32), ("nbins", _i32), ("usecutoff", _i32),
+                ("binstep", _d), ("cutoff", _d), ("dbulk", _d),
+                ("n_random_samples", _i32), ("coordination_number_only", _i32), ("triclinic", _i32), ("lcell", _i32),
+                ("seed", _u64), ("n_groups_u", _i32), ("n_groups_v", _i32),
+                ("groups_u_ptr", _P(_i32)), ("groups_u_idx", _P(_i32)),
+                ("groups_v_ptr", _P(_i32)), ("groups_v_idx", _P(_i32)),
+                ("max_batch_frames", _i64), ("memory_budget", _i64)]
+
+
+class RawOut(ctypes.Structure):
+    _fields_ = [("md_count", _P(_d)), ("md_count_random", _P(_d)), ("rdf_count", _P(_d)),
+                ("rdf_count_random", _P(_d)), ("solute_group_count", _P(_d)), ("solvent_group_count", _P(_d)),
+                ("solute_group_count_random", _P(_d)), ("solvent_group_count_random", _P(_d)),
+                ("volume_total", _d),
+                ("hits_md", _P(_u64)), ("hits_md_random", _P(_u64)), ("hits_rdf", _P(_u64)),
+                ("hits_rdf_random", _P(_u64))]
+
+
+class KernelStats(ctypes.Structure):
+    _fields_ = [("eval_ms", _d), ("eval_launches", _i64), ("total_ms", _d), ("total_launches", _i64),
+                ("frames", _i64), ("candidate_pairs", _i64), ("fp64_rechecks", _i64)]
+
+
+EXPORTS = ["cm_job_create", "cm_job_submit", "cm_job_submit_device", "cm_job_finish", "cm_job_set_profiling",
+           "cm_job_kernel_stats", "cm_job_destroy", "cm_last_error", "cm_device_alloc", "cm_device_upload",
+           "cm_device_free", "cm_device_synchronize"]
+
+_lib = None
+
+
+class GPUError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GPUError(f"{LIB_PATH} is not built: run __graft_entry__.build() (the HIP library is the only "
+                           "compute path)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.cm_job_create.argtypes = [_P(JobDesc), _P(ctypes.c_void_p)]
+        L.cm_job_submit.argtypes = [ctypes.c_void_p, _i32, _P(_i64), ctypes.c_void_p, ctypes.c_void_p, _i32, _P(_d),
+                                    _P(_d)]
+        L.cm_job_submit_device.argtypes = L.cm_job_submit.argtypes
+        L.cm_job_finish.argtypes = [ctypes.c_void_p, _P(RawOut)]
+        L.cm_job_set_profiling.argtypes = [ctypes.c_void_p, _i32]
+        L.cm_job_kernel_stats.argtypes = [ctypes.c_void_p, _P(KernelStats)]
+        L.cm_job_destroy.argtypes = [ctypes.c_void_p]
+        L.cm_job_destroy.restype = None
+        L.cm_last_error.restype = ctypes.c_char_p
+        L.cm_device_alloc.argtypes = [ctypes.c_size_t, _P(ctypes.c_void_p)]
+        L.cm_device_upload.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.cm_device_free.argtypes = [ctypes.c_void_p]
+        for f in ("cm_job_create", "cm_job_submit", "cm_job_submit_device", "cm_job_finish", "cm_job_set_profiling",
+                  "cm_job_kernel_stats", "cm_device_alloc", "cm_device_upload", "cm_device_free",
+                  "cm_device_synchronize"):
+            getattr(L, f).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code == CM_OK:
+        return
+    msg = lib().cm_last_error().decode()
+    if code == CM_EINVAL:
+        raise ArgumentError(msg)
+    raise GPUError(f"libcmgpu error {code}: {msg}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(_P(_d))
+
+
+class DeviceArray:
+    """A host array copied once into device memory (HBM-resident input)."""
+
+    def __init__(self, host):
+        host = np.ascontiguousarray(host)
+        self.dtype = host.dtype
+        self.shape = host.shape
+        self.nbytes = host.nbytes
+        self.ptr = ctypes.c_void_p()
+        check(lib().cm_device_alloc(self.nbytes, ctypes.byref(self.ptr)))
+        check(lib().cm_device_upload(self.ptr, host.ctypes.data_as(ctypes.c_void_p), self.nbytes))
+
+    def offset(self, nbytes):
+        return ctypes.c_void_p(self.ptr.value + nbytes)
+
+    def free(self):
+        if self.ptr:
+            check(lib().cm_device_free(self.ptr))
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Job:
+    """One accumulation job on the device (cm_job_*)."""
+
+    def __init__(self, *, nmols_u, natoms_per_mol_u, nmols_v, natoms_per_mol_v, autocorrelation, irefatom, nbins,
+                 binstep, cutoff, dbulk, usecutoff, n_random_samples, coordination_number_only, triclinic, seed,
+                 n_groups_u, n_groups_v, groups_u=(None, None), groups_v=(None, None), lcell=1, max_batch_frames=0,
+                 memory_budget=0):
+        self._keep = [g for g in (*groups_u, *groups_v) if g is not None]
+        nul = _P(_i32)()
+
+        def ip(a):
+            return a.ctypes.data_as(_P(_i32)) if a is not None else nul
+
+        self.nbins, self.ngu, self.ngv = int(nbins), int(n_groups_u), int(n_groups_v)
+        d = JobDesc(int(nmols_u), int(natoms_per_mol_u), int(nmols_v), int(natoms_per_mol_v), int(bool(autocorrelation)),
+                    int(irefatom), int(nbins), int(bool(usecutoff)), float(binstep), float(cutoff), float(dbulk),
+                    int(n_random_samples), int(bool(coordination_number_only)), int(bool(triclinic)), int(lcell),
+                    int(seed) & 0xFFFFFFFFFFFFFFFF, int(n_groups_u), int(n_groups_v),
+                    ip(groups_u[0]), ip(groups_u[1]), ip(groups_v[0]), ip(groups_v[1]),
+                    int(max_batch_frames), int(memory_budget))
+        self.h = ctypes.c_void_p()
+        check(lib().cm_job_create(ctypes.byref(d), ctypes.byref(self.h)))
+
+    def submit(self, frame_index, x_solute, x_solvent, cells, weights):
+        """Host arrays: frame_index [F] (1-based), x_* [F][N][3] float64/float32,
+        cells [F][3][3] (lattice vectors as columns), weights [F]."""
+        xu = np.ascontiguousarray(x_solute)
+        xv = np.ascontiguousarray(x_solvent)
+        if xu.dtype != xv.dtype or xu.dtype not in (np.float64, np.float32):
+            raise ArgumentError("coordinates must be both float64 or both float32")
+        dt = CM_F32 if xu.dtype == np.float32 else CM_F64
+        fi = np.ascontiguousarray(frame_index, dtype=np.int64)
+        c9 = np.ascontiguousarray(np.transpose(np.asarray(cells, dtype=np.float64), (0, 2, 1)).reshape(-1, 9))
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        check(lib().cm_job_submit(self.h, len(fi), fi.ctypes.data_as(_P(_i64)), xu.ctypes.data_as(ctypes.c_void_p),
+                                  xv.ctypes.data_as(ctypes.c_void_p), dt, _dp(c9), _dp(w)))
+
+    def submit_device(self, frame_index, d_solute, d_solvent, frame0, cells, weights):
+        """Device-resident coordinates (DeviceArray of [F][N][3]); frames
+        frame0 .. frame0+len(frame_index)-1 of those arrays."""
+        dt = CM_F32 if d_solute.dtype == np.float32 else CM_F64
+        fi = np.ascontiguousarray(frame_index, dtype=np.int64)
+        c9 = np.ascontiguousarray(np.transpose(np.asarray(cells, dtype=np.float64), (0, 2, 1)).reshape(-1, 9))
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        fu = d_solute.nbytes // d_solute.shape[0]
+        fv = d_solvent.nbytes // d_solvent.shape[0]
+        check(lib().cm_job_submit_device(self.h, len(fi), fi.ctypes.data_as(_P(_i64)), d_solute.offset(frame0 * fu),
+                                         d_solvent.offset(frame0 * fv), dt, _dp(c9), _dp(w)))
+
+    def finish(self):
+        nb = self.nbins
+        out = {k: np.zeros(nb) for k in ("md_count", "md_count_random", "rdf_count", "rdf_count_random")}
+        for k, g in (("solute_group_count", self.ngu), ("solvent_group_count", self.ngv),
+                     ("solute_group_count_random", self.ngu), ("solvent_group_count_random", self.ngv)):
+            out[k] = np.zeros((g, nb))
+        hits = {k: np.zeros(nb, dtype=np.uint64) for k in ("hits_md", "hits_md_random", "hits_rdf", "hits_rdf_random")}
+        r = RawOut(*[_dp(out[k]) for k in ("md_count", "md_count_random", "rdf_count", "rdf_count_random",
+                                           "solute_group_count", "solvent_group_count", "solute_group_count_random",
+                                           "solvent_group_count_random")], 0.0,
+                   *[hits[k].ctypes.data_as(_P(_u64)) for k in ("hits_md", "hits_md_random", "hits_rdf",
+                                                               "hits_rdf_random")])
+        check(lib().cm_job_finish(self.h, ctypes.byref(r)))
+        out["volume_total"] = r.volume_total
+        out.update(hits)
+        return out
+
+    def set_profiling(self, on=True):
+        check(lib().cm_job_set_profiling(self.h, int(on)))
+
+    def stats(self):
+        s = KernelStats()
+        check(lib().cm_job_kernel_stats(self.h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in KernelStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().cm_job_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,1483 @@
+// cmgpu.hip -- MI355X (gfx950) implementation of the MDDF accumulation of
+// ComplexMixtures.jl behind the C ABI of include/cmgpu.h.
+//
+// Per batch of frames, on the job's stream:
+//   k_radii      molecule radii about their centres (cell sizing, guard band)
+//   k_grid       per-frame cell grid and FP32 thresholds
+//   k_wrap_atoms / k_prep_solute_mols, k_scan_cells, k_scatter
+//                device cell list: counting sort by cell + exclusive scan
+//   k_prep_solvent  solvent molecules made whole, in the frame of their cell
+//   k_eval_atoms / k_eval_mols  (real pass) minimum distance per solvent
+//                molecule in FP32, exact FP64 resolve, LDS histograms
+//   k_bulk_compact  ordered list of bulk molecules (inbulk, src/mddf.jl:55-57)
+//   k_place      Philox random rigid-body placement (src/rigid_body.jl:107-137)
+//   k_eval_*     (random passes) the same kernels on the placed molecules
+//   k_flush      integer hits -> FP64 accumulators times the frame weight
+//
+// Two kernel families: "atoms" for a single solute molecule of any size (its
+// atoms in a cell list, one thread per solvent molecule scanning the 27
+// neighbour cells of the molecule's cell) and "mols" for many solute
+// molecules (molecule centres in a cell list, all atom pairs of close
+// molecule pairs).
+//
+// Exactness.  Candidate pairs are evaluated in FP32 from coordinates in the
+// frame of each molecule's cell.  Per solvent molecule the kernel keeps the
+// best and second-best squared distance; when the gap exceeds a guard band
+// that bounds the FP32 error, the FP32 argmin is the FP64 argmin and only that
+// pair is recomputed in FP64 (the recorded distance).  Otherwise every pair
+// of the molecule is recomputed in FP64 (exhaustive resolve).  Records,
+// indices and bins therefore equal those of an FP64 evaluation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cm_math.h"
+#include "cmgpu.h"
+
+using ull = unsigned long long;
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int MAX_NBINS = 8192;
+
+// ---------------------------------------------------------------------------
+// device-side structures
+// ---------------------------------------------------------------------------
+struct Geo {                 // one frame of a batch
+    cm::Cell64 cell;
+    float H32[9];
+    double sp[3];            // plane spacings
+    int nc[3];
+    int ncell;
+    uint32_t frame;          // 1-based trajectory frame (keys the random streams)
+    float cut2_hi;           // FP32 squared distances above this are beyond the cutoff
+    float amb;               // best/second gap below which the FP64 resolve is exhaustive
+    float rpre2;             // molecule-centre pre-check radius^2 (mols family)
+    unsigned ru_bits, rv_bits;   // molecule radii (float bits, atomicMax)
+    int err;                 // geometry the kernels cannot handle
+};
+
+struct KP {                  // job parameters, by value
+    int Mu, nu, Mv, nv;
+    int self_, iref, nbins, usecutoff;
+    double binstep, cutsys, cut2, dbulk;
+    int K;                   // random passes per frame (n_random_samples, 0 for coordination numbers)
+    int kb;                  // bulk lists per frame (1 in the atoms family, K in the mols family)
+    uint32_t key0, key1;
+    const int *gu_ptr, *gu_idx, *gv_ptr, *gv_idx;
+    int cap;                 // cell slots per frame
+    int F;                   // frames in the batch
+    int64_t Nu, Nv;
+};
+
+struct Bufs {
+    Geo* geo;
+    // cell list
+    int* cnt;                // [F][cap+1]
+    int* start;              // [F][cap+1]
+    float4* tmp4;            // [F][nitems] unsorted entries
+    int* tmpcell;            // [F][nitems]
+    float4* sorted;          // [F][nitems]
+    int nitems;              // atoms (atoms family) or solute molecules (mols family)
+    // solute molecules (mols family)
+    float4* solw;            // [F][Nu] atoms in the frame of their molecule's cell
+    float4* ucen;            // [F][Mu]
+    // solvent, real pass
+    float4* vw;              // [F][Nv]
+    int* vcell;              // [F][Mv]
+    float4* vcen;            // [F][Mv]
+    // random passes
+    double* P;               // [F][K][Nv][3]
+    float4* vw_r;            // [F][K][Nv]
+    int* vcell_r;            // [F][K][Mv]
+    float4* vcen_r;          // [F][K][Mv]
+    unsigned char* bflag;    // [F][kb][Mv]
+    int* blist;              // [F][kb][Mv]
+    int* nbulk;              // [F][kb]
+    int* uk;                 // [F][K] drawn solute molecule of each random pass
+    int* drawn_ptr;          // [F][Mu+1] CSR: solute molecule -> random passes
+    int* drawn_idx;          // [F][K]
+    // accumulators
+    ull* cnt_md; ull* cnt_rdf; ull* cnt_md_r; ull* cnt_rdf_r;
+    ull* cnt_gu; ull* cnt_gv; ull* cnt_gu_r; ull* cnt_gv_r;
+    ull* stats;              // [0] exhaustive resolves
+};
+
+struct Rec {
+    int within;
+    int i;
+    int64_t j;
+    double d;
+    int ref;
+    double dref;
+};
+
+__device__ __forceinline__ Rec rec_zero()
+{
+    Rec r;
+    r.within = 0; r.i = -1; r.j = -1; r.d = INFINITY; r.ref = 0; r.dref = INFINITY;
+    return r;
+}
+
+// offer a within-cutoff pair to an exact record: ties on d go to the
+// smallest (i, j) (update_md, src/minimum_distances.jl:30-39)
+__device__ __forceinline__ void rec_offer(Rec& r, int i, int64_t j, double d, bool isref)
+{
+    if (d < r.d || (d == r.d && (i < r.i || (i == r.i && j < r.j)))) {
+        r.within = 1; r.i = i; r.j = j; r.d = d;
+    }
+    if (isref) { r.ref = 1; r.dref = fmin(r.dref, d); }
+}
+
+template <class T>
+__device__ __forceinline__ void load3(const T* p, int64_t atom, double& x, double& y, double& z)
+{
+    x = (double)p[3 * atom]; y = (double)p[3 * atom + 1]; z = (double)p[3 * atom + 2];
+}
+
+__device__ __forceinline__ float4 mk4(float x, float y, float z, float w) { return make_float4(x, y, z, w); }
+
+__device__ __forceinline__ void cell_of_frac(const Geo& g, double f0, double f1, double f2, int& cell)
+{
+    int c0 = (int)(f0 * g.nc[0]); if (c0 >= g.nc[0]) c0 = g.nc[0] - 1; if (c0 < 0) c0 = 0;
+    int c1 = (int)(f1 * g.nc[1]); if (c1 >= g.nc[1]) c1 = g.nc[1] - 1; if (c1 < 0) c1 = 0;
+    int c2 = (int)(f2 * g.nc[2]); if (c2 >= g.nc[2]) c2 = g.nc[2] - 1; if (c2 < 0) c2 = 0;
+    cell = (c2 * g.nc[1] + c1) * g.nc[0] + c0;
+}
+
+// fractional coordinates of a point, the lattice translation n = floor(f)
+// that brings it into the cell, and the wrapped fractional part
+__device__ __forceinline__ void wrap_point(const Geo& g, double x, double y, double z, double fr[3], double n[3])
+{
+    const double* Hi = g.cell.Hi;
+    double f0 = Hi[0] * x + Hi[3] * y + Hi[6] * z;
+    double f1 = Hi[1] * x + Hi[4] * y + Hi[7] * z;
+    double f2 = Hi[2] * x + Hi[5] * y + Hi[8] * z;
+    n[0] = floor(f0); n[1] = floor(f1); n[2] = floor(f2);
+    fr[0] = f0 - n[0]; fr[1] = f1 - n[1]; fr[2] = f2 - n[2];
+    for (int k = 0; k < 3; k++) if (fr[k] >= 1.0) { fr[k] = 0.0; n[k] += 1.0; }
+}
+
+// Molecule made whole around its first atom (minimum image of every atom
+// relative to atom 0); returns the centre of coordinates.
+template <class T>
+__device__ void mol_centre(const Geo& g, const T* x, int64_t a0, int n, double c[3])
+{
+    double x0, y0, z0;
+    load3(x, a0, x0, y0, z0);
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int a = 0; a < n; a++) {
+        double xa, ya, za, ox, oy, oz;
+        load3(x, a0 + a, xa, ya, za);
+        cm::minimage(g.cell, xa - x0, ya - y0, za - z0, &ox, &oy, &oz);
+        sx += ox; sy += oy; sz += oz;
+    }
+    c[0] = x0 + sx / n; c[1] = y0 + sy / n; c[2] = z0 + sz / n;
+}
+
+// ---------------------------------------------------------------------------
+// radii and grid
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_radii(KP P, Bufs B, const T* xu, const T* xv, int do_solute)
+{
+    int64_t nmol = (int64_t)P.Mv + (do_solute ? P.Mu : 0);
+    int64_t total = (int64_t)P.F * nmol;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int f = (int)(it / nmol);
+        int64_t m = it - (int64_t)f * nmol;
+        Geo& g = B.geo[f];
+        bool solv = m < P.Mv;
+        const T* x = solv ? xv + (int64_t)f * P.Nv * 3 : xu + (int64_t)f * P.Nu * 3;
+        int n = solv ? P.nv : P.nu;
+        int64_t a0 = solv ? m * P.nv : (m - P.Mv) * P.nu;
+        double c[3];
+        mol_centre(g, x, a0, n, c);
+        double x0, y0, z0;
+        load3(x, a0, x0, y0, z0);
+        double r2 = 0.0;
+        for (int a = 0; a < n; a++) {
+            double xa, ya, za, ox, oy, oz;
+            load3(x, a0 + a, xa, ya, za);
+            cm::minimage(g.cell, xa - x0, ya - y0, za - z0, &ox, &oy, &oz);
+            double dx = x0 + ox - c[0], dy = y0 + oy - c[1], dz = z0 + oz - c[2];
+            r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+        }
+        float r = (float)(sqrt(r2) * (1.0 + 1e-6) + 1e-4);
+        atomicMax(solv ? &g.rv_bits : &g.ru_bits, __float_as_uint(r));
+    }
+}
+
+__device__ __forceinline__ double halfulp32(double x)
+{
+    if (!(x > 1e-30)) return 0x1p-150;
+    return ldexp(1.0, ilogb(x) - 24);
+}
+
+__global__ void k_grid(KP P, Bufs B, int mols_family)
+{
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= P.F) return;
+    Geo& g = B.geo[f];
+    double ru = mols_family ? (double)__uint_as_float(g.ru_bits) : 0.0;
+    double rv = (double)__uint_as_float(g.rv_bits);
+    double side = P.cutsys + ru + rv + 1e-3 + 1e-7 * (g.sp[0] + g.sp[1] + g.sp[2]);
+    int ncell = 1;
+    for (int k = 0; k < 3; k++) {
+        int n = (int)floor(g.sp[k] / side);
+        g.nc[k] = n < 1 ? 1 : n;
+        ncell *= g.nc[k];
+        if (mols_family && !(side < g.sp[k])) g.err = 2;   // image enumeration assumes ru+rv+cutoff < spacing
+    }
+    if (ncell > P.cap) g.err = 3;
+    g.ncell = ncell;
+    const double* H = g.cell.H;
+    double xm = 0.0;
+    for (int r = 0; r < 3; r++) xm = fmax(xm, fabs(H[r]) + fabs(H[3 + r]) + fabs(H[6 + r]));
+    xm += fmax(ru, rv) + P.cutsys + 1.0;
+    double E = 6.0 * halfulp32(2.0 * xm) + 1e-7;
+    double dmax = P.cutsys + 0.5;
+    double G = E * (2.0 * 1.7320508075688772 * dmax + 3.0 * E) + 8.0 * 0x1p-24 * dmax * dmax;
+    float hi = (float)(P.cut2 + G);
+    g.cut2_hi = nextafterf(hi, INFINITY);
+    float ulpc = (float)(halfulp32((double)g.cut2_hi) * 2.0);
+    g.amb = nextafterf((float)(2.0 * G) + 8.0f * ulpc, INFINITY);
+    float rp = (float)(P.cutsys + ru + rv + 1e-2);
+    g.rpre2 = rp * rp;
+}
+
+// ---------------------------------------------------------------------------
+// cell list: counting sort by cell
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_wrap_atoms(KP P, Bufs B, const T* xu)
+{
+    int64_t total = (int64_t)P.F * P.nu;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int f = (int)(it / P.nu);
+        int i = (int)(it - (int64_t)f * P.nu);
+        const Geo& g = B.geo[f];
+        double x, y, z;
+        load3(xu + (int64_t)f * P.Nu * 3, i, x, y, z);
+        double fr[3], n[3];
+        wrap_point(g, x, y, z, fr, n);
+        int cell;
+        cell_of_frac(g, fr[0], fr[1], fr[2], cell);
+        const double* H = g.cell.H;
+        double wx = H[0] * fr[0] + H[3] * fr[1] + H[6] * fr[2];
+        double wy = H[1] * fr[0] + H[4] * fr[1] + H[7] * fr[2];
+        double wz = H[2] * fr[0] + H[5] * fr[1] + H[8] * fr[2];
+        int64_t o = (int64_t)f * B.nitems + i;
+        B.tmp4[o] = mk4((float)wx, (float)wy, (float)wz, __int_as_float(i));
+        B.tmpcell[o] = cell;
+        atomicAdd(&B.cnt[(int64_t)f * (P.cap + 1) + cell], 1);
+    }
+}
+
+// Solute molecules (mols family): made whole, centre wrapped into the cell,
+// atoms written in the frame of that cell.
+template <class T>
+__global__ void k_prep_solute_mols(KP P, Bufs B, const T* xu)
+{
+    int64_t total = (int64_t)P.F * P.Mu;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int f = (int)(it / P.Mu);
+        int u = (int)(it - (int64_t)f * P.Mu);
+        const Geo& g = B.geo[f];
+        const T* x = xu + (int64_t)f * P.Nu * 3;
+        int64_t a0 = (int64_t)u * P.nu;
+        double c[3];
+        mol_centre(g, x, a0, P.nu, c);
+        double fr[3], n[3];
+        wrap_point(g, c[0], c[1], c[2], fr, n);
+        int cell;
+        cell_of_frac(g, fr[0], fr[1], fr[2], cell);
+        const double* H = g.cell.H;
+        double tx = H[0] * n[0] + H[3] * n[1] + H[6] * n[2];
+        double ty = H[1] * n[0] + H[4] * n[1] + H[7] * n[2];
+        double tz = H[2] * n[0] + H[5] * n[1] + H[8] * n[2];
+        double x0, y0, z0;
+        load3(x, a0, x0, y0, z0);
+        for (int a = 0; a < P.nu; a++) {
+            double xa, ya, za, ox, oy, oz;
+            load3(x, a0 + a, xa, ya, za);
+            cm::minimage(g.cell, xa - x0, ya - y0, za - z0, &ox, &oy, &oz);
+            B.solw[(int64_t)f * P.Nu + a0 + a] = mk4((float)(x0 + ox - tx), (float)(y0 + oy - ty), (float)(z0 + oz - tz), 0.f);
+        }
+        float4 cw = mk4((float)(c[0] - tx), (float)(c[1] - ty), (float)(c[2] - tz), __int_as_float(u));
+        B.ucen[(int64_t)f * P.Mu + u] = cw;
+        int64_t o = (int64_t)f * B.nitems + u;
+        B.tmp4[o] = cw;
+        B.tmpcell[o] = cell;
+        atomicAdd(&B.cnt[(int64_t)f * (P.cap + 1) + cell], 1);
+    }
+}
+
+// exclusive scan of the cell counts, one block per frame; leaves cnt zeroed
+__global__ void __launch_bounds__(1024) k_scan_cells(KP P, Bufs B)
+{
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    int f = blockIdx.x;
+    int ncell = B.geo[f].ncell;
+    int* cnt = B.cnt + (int64_t)f * (P.cap + 1);
+    int* st = B.start + (int64_t)f * (P.cap + 1);
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int base = 0; base < ncell; base += blockDim.x) {
+        int idx = base + threadIdx.x;
+        int v = idx < ncell ? cnt[idx] : 0;
+        int incl = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        int woff = 0;
+        for (int w = 0; w < wid; w++) woff += wsum[w];
+        int tot = 0;
+        for (int w = 0; w < nw; w++) tot += wsum[w];
+        if (idx < ncell) { st[idx] = carry + woff + incl - v; cnt[idx] = 0; }
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) st[ncell] = carry;
+}
+
+__global__ void k_scatter(KP P, Bufs B, int nper)
+{
+    int64_t total = (int64_t)P.F * nper;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int f = (int)(it / nper);
+        int e = (int)(it - (int64_t)f * nper);
+        int64_t o = (int64_t)f * B.nitems + e;
+        int cell = B.tmpcell[o];
+        int64_t cb = (int64_t)f * (P.cap + 1);
+        int pos = B.start[cb + cell] + atomicAdd(&B.cnt[cb + cell], 1);
+        B.sorted[(int64_t)f * B.nitems + pos] = B.tmp4[o];
+    }
+}
+
+// solvent molecules of the real pass, in the frame of their cell
+template <class T>
+__global__ void k_prep_solvent(KP P, Bufs B, const T* xv)
+{
+    int64_t total = (int64_t)P.F * P.Mv;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int f = (int)(it / P.Mv);
+        int v = (int)(it - (int64_t)f * P.Mv);
+        const Geo& g = B.geo[f];
+        const T* x = xv + (int64_t)f * P.Nv * 3;
+        int64_t a0 = (int64_t)v * P.nv;
+        double c[3];
+        mol_centre(g, x, a0, P.nv, c);
+        double fr[3], n[3];
+        wrap_point(g, c[0], c[1], c[2], fr, n);
+        int cell;
+        cell_of_frac(g, fr[0], fr[1], fr[2], cell);
+        const double* H = g.cell.H;
+        double tx = H[0] * n[0] + H[3] * n[1] + H[6] * n[2];
+        double ty = H[1] * n[0] + H[4] * n[1] + H[7] * n[2];
+        double tz = H[2] * n[0] + H[5] * n[1] + H[8] * n[2];
+        double x0, y0, z0;
+        load3(x, a0, x0, y0, z0);
+        for (int a = 0; a < P.nv; a++) {
+            double xa, ya, za, ox, oy, oz;
+            load3(x, a0 + a, xa, ya, za);
+            cm::minimage(g.cell, xa - x0, ya - y0, za - z0, &ox, &oy, &oz);
+            B.vw[(int64_t)f * P.Nv + a0 + a] = mk4((float)(x0 + ox - tx), (float)(y0 + oy - ty), (float)(z0 + oz - tz), 0.f);
+        }
+        B.vcell[(int64_t)f * P.Mv + v] = cell;
+        B.vcen[(int64_t)f * P.Mv + v] = mk4((float)(c[0] - tx), (float)(c[1] - ty), (float)(c[2] - tz), 0.f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// histogram updates (update_counters!, src/update_counters.jl:43-88)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void group_hit(ull* gc, int nbins, int b, int64_t atom, const int* ptr, const int* idx, int natomspermol)
+{
+    if (!ptr) {
+        int64_t t = atom % natomspermol;
+        atomicAdd(&gc[t * nbins + b], 1ull);
+    } else {
+        for (int k = ptr[atom]; k < ptr[atom + 1]; k++) atomicAdd(&gc[(int64_t)idx[k] * nbins + b], 1ull);
+    }
+}
+
+__device__ __forceinline__ void hist_record(const KP& P, unsigned* sh, ull* gu, ull* gv, const Rec& r)
+{
+    if (!r.within) return;
+    int b = (int)cm::setbin(r.d, P.binstep) - 1;
+    if (b >= P.nbins) return;   // unreachable: d <= cutoff (validated at job creation)
+    atomicAdd(&sh[b], 1u);
+    group_hit(gu, P.nbins, b, r.i, P.gu_ptr, P.gu_idx, P.nu);
+    if (P.self_) group_hit(gu, P.nbins, b, r.j, P.gv_ptr, P.gv_idx, P.nu);
+    else group_hit(gv, P.nbins, b, r.j, P.gv_ptr, P.gv_idx, P.nv);
+    if (r.ref) {
+        int rb = (int)cm::setbin(r.dref, P.binstep) - 1;
+        if (rb < P.nbins) atomicAdd(&sh[P.nbins + rb], 1u);
+    }
+}
+
+__device__ __forceinline__ void hist_clear(unsigned* sh, int n)
+{
+    for (int k = threadIdx.x; k < n; k += blockDim.x) sh[k] = 0u;
+    __syncthreads();
+}
+
+__device__ __forceinline__ void hist_flush(unsigned* sh, int nbins, ull* md, ull* rdf)
+{
+    __syncthreads();
+    for (int k = threadIdx.x; k < nbins; k += blockDim.x) {
+        if (sh[k]) atomicAdd(&md[k], (ull)sh[k]);
+        if (sh[nbins + k]) atomicAdd(&rdf[k], (ull)sh[nbins + k]);
+    }
+}
+
+__device__ __forceinline__ bool inbulk(const KP& P, const Rec& r)
+{
+    return P.usecutoff ? (r.within && r.d > P.dbulk) : !r.within;
+}
+
+// FP32 running minimum with second best
+struct Best {
+    float best, second;
+    int bi, ba;
+    __device__ __forceinline__ void init() { best = INFINITY; second = INFINITY; bi = -1; ba = -1; }
+    __device__ __forceinline__ void offer(float d2, int i, int a)
+    {
+        second = fminf(second, fmaxf(best, d2));
+        if (d2 < best) { best = d2; bi = i; ba = a; }
+    }
+};
+
+// neighbour cell (with its lattice shift) of cell c at offset o in one dimension
+__device__ __forceinline__ int nb_cell(int c, int o, int n, int& m)
+{
+    int t = c + o;
+    m = t < 0 ? -1 : (t >= n ? 1 : 0);
+    return t - m * n;
+}
+
+// ---------------------------------------------------------------------------
+// minimum distances, atoms family (one solute molecule, any size)
+// ---------------------------------------------------------------------------
+template <class TU, class TV, bool RANDOM>
+__global__ void __launch_bounds__(BLOCK) k_eval_atoms(KP P, Bufs B, const TU* xu, const TV* xv)
+{
+    extern __shared__ unsigned sh[];
+    hist_clear(sh, 2 * P.nbins);
+    const int K = RANDOM ? P.K : 1;
+    int64_t total = (int64_t)P.F * K * P.Mv;
+    ull* gu = RANDOM ? B.cnt_gu_r : B.cnt_gu;
+    ull* gv = RANDOM ? B.cnt_gv_r : B.cnt_gv;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int64_t fk = it / P.Mv;
+        int v = (int)(it - fk * P.Mv);
+        int f = (int)(fk / K);
+        const Geo& g = B.geo[f];
+        const float4* vw = RANDOM ? B.vw_r + fk * P.Nv : B.vw + (int64_t)f * P.Nv;
+        int vcell = RANDOM ? B.vcell_r[fk * P.Mv + v] : B.vcell[(int64_t)f * P.Mv + v];
+        const float4* srt = B.sorted + (int64_t)f * B.nitems;
+        const int* st = B.start + (int64_t)f * (P.cap + 1);
+        int nc0 = g.nc[0], nc1 = g.nc[1], nc2 = g.nc[2];
+        int c0 = vcell % nc0, c1 = (vcell / nc0) % nc1, c2 = vcell / (nc0 * nc1);
+        Best bm, br;
+        bm.init(); br.init();
+        for (int a = 0; a < P.nv; a++) {
+            float4 y = vw[(int64_t)v * P.nv + a];
+            bool isref = (a == P.iref);
+            for (int oz = -1; oz <= 1; oz++) {
+                int mz; int cz = nb_cell(c2, oz, nc2, mz);
+                for (int oy = -1; oy <= 1; oy++) {
+                    int my; int cy = nb_cell(c1, oy, nc1, my);
+                    for (int ox = -1; ox <= 1; ox++) {
+                        int mx; int cx = nb_cell(c0, ox, nc0, mx);
+                        int cell = (cz * nc1 + cy) * nc0 + cx;
+                        float fx = (float)mx, fy = (float)my, fz = (float)mz;
+                        float tx = g.H32[0] * fx + g.H32[3] * fy + g.H32[6] * fz;
+                        float ty = g.H32[1] * fx + g.H32[4] * fy + g.H32[7] * fz;
+                        float tz = g.H32[2] * fx + g.H32[5] * fy + g.H32[8] * fz;
+                        float yx = y.x - tx, yy = y.y - ty, yz = y.z - tz;
+                        int p1 = st[cell + 1];
+                        for (int p = st[cell]; p < p1; p++) {
+                            float4 s = srt[p];
+                            float dx = yx - s.x, dy = yy - s.y, dz = yz - s.z;
+                            float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                            int i = __float_as_int(s.w);
+                            bm.offer(d2, i, a);
+                            if (isref) br.offer(d2, i, a);
+                        }
+                    }
+                }
+            }
+        }
+        // exact resolve in FP64
+        cm::Cell64 cl = g.cell;
+        const TU* xuf = xu + (int64_t)f * P.Nu * 3;
+        Rec r = rec_zero();
+        bool exhaustive = false;
+        int64_t jbase = (int64_t)v * P.nv;
+        auto ysrc = [&](int a, double& y0, double& y1, double& y2) {
+            if (RANDOM) { const double* pp = B.P + (fk * P.Nv + jbase + a) * 3; y0 = pp[0]; y1 = pp[1]; y2 = pp[2]; }
+            else load3(xv + (int64_t)f * P.Nv * 3, jbase + a, y0, y1, y2);
+        };
+        if (bm.best <= g.cut2_hi) {
+            if (bm.second - bm.best > g.amb) {
+                double x0, x1, x2, y0, y1, y2;
+                load3(xuf, bm.bi, x0, x1, x2);
+                ysrc(bm.ba, y0, y1, y2);
+                double d2 = cm::canon_d2(cl, x0, x1, x2, y0, y1, y2);
+                if (d2 <= P.cut2) { r.within = 1; r.i = bm.bi; r.j = jbase + bm.ba; r.d = sqrt(d2); }
+            } else exhaustive = true;
+        }
+        if (br.best <= g.cut2_hi) {
+            if (br.second - br.best > g.amb) {
+                double x0, x1, x2, y0, y1, y2;
+                load3(xuf, br.bi, x0, x1, x2);
+                ysrc(P.iref, y0, y1, y2);
+                double d2 = cm::canon_d2(cl, x0, x1, x2, y0, y1, y2);
+                if (d2 <= P.cut2) { r.ref = 1; r.dref = sqrt(d2); }
+            } else exhaustive = true;
+        }
+        if (exhaustive) {
+            atomicAdd(&B.stats[0], 1ull);
+            r = rec_zero();
+            for (int a = 0; a < P.nv; a++) {
+                double y0, y1, y2;
+                ysrc(a, y0, y1, y2);
+                for (int oz = -1; oz <= 1; oz++) {
+                    int mz; int cz = nb_cell(c2, oz, nc2, mz);
+                    for (int oy = -1; oy <= 1; oy++) {
+                        int my; int cy = nb_cell(c1, oy, nc1, my);
+                        for (int ox = -1; ox <= 1; ox++) {
+                            int mx; int cx = nb_cell(c0, ox, nc0, mx);
+                            int cell = (cz * nc1 + cy) * nc0 + cx;
+                            for (int p = st[cell]; p < st[cell + 1]; p++) {
+                                int i = __float_as_int(srt[p].w);
+                                double x0, x1, x2;
+                                load3(xuf, i, x0, x1, x2);
+                                double d2 = cm::canon_d2(cl, x0, x1, x2, y0, y1, y2);
+                                if (d2 <= P.cut2) rec_offer(r, i, jbase + a, sqrt(d2), a == P.iref);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        hist_record(P, sh, gu, gv, r);
+        if (!RANDOM && P.K > 0) B.bflag[(int64_t)f * P.kb * P.Mv + v] = inbulk(P, r) ? 1 : 0;
+    }
+    if (RANDOM) hist_flush(sh, P.nbins, B.cnt_md_r, B.cnt_rdf_r);
+    else hist_flush(sh, P.nbins, B.cnt_md, B.cnt_rdf);
+}
+
+// ---------------------------------------------------------------------------
+// minimum distances, mols family (many solute molecules)
+// ---------------------------------------------------------------------------
+// FP32 evaluation of all atom pairs of solute molecule u (atoms xs) and the
+// solvent molecule whose atoms are yw, the solute shifted by t.
+__device__ __forceinline__ void pairs_mol(const KP& P, const float4* xs, const float4* yw, float tx, float ty, float tz,
+                                          Best& bm, Best& br)
+{
+    for (int a = 0; a < P.nv; a++) {
+        float4 y = yw[a];
+        float yx = y.x - tx, yy = y.y - ty, yz = y.z - tz;
+        bool isref = (a == P.iref);
+        for (int i = 0; i < P.nu; i++) {
+            float4 s = xs[i];
+            float dx = yx - s.x, dy = yy - s.y, dz = yz - s.z;
+            float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            bm.offer(d2, i, a);
+            if (isref) br.offer(d2, i, a);
+        }
+    }
+}
+
+// exact record of the molecule pair (u, v) from the FP32 bests
+template <class TU, class YS>
+__device__ Rec resolve_mols(const KP& P, const Geo& g, const TU* xuf, int u, int v, YS ysrc, const Best& bm, const Best& br,
+                            ull* stats)
+{
+    Rec r = rec_zero();
+    bool exhaustive = false;
+    int64_t ibase = (int64_t)u * P.nu, jbase = (int64_t)v * P.nv;
+    if (bm.best <= g.cut2_hi) {
+        if (bm.second - bm.best > g.amb) {
+            double x0, x1, x2, y0, y1, y2;
+            load3(xuf, ibase + bm.bi, x0, x1, x2);
+            ysrc(bm.ba, y0, y1, y2);
+            double d2 = cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
+            if (d2 <= P.cut2) { r.within = 1; r.i = bm.bi; r.j = jbase + bm.ba; r.d = sqrt(d2); }
+        } else exhaustive = true;
+    }
+    if (br.best <= g.cut2_hi) {
+        if (br.second - br.best > g.amb) {
+            double x0, x1, x2, y0, y1, y2;
+            load3(xuf, ibase + br.bi, x0, x1, x2);
+            ysrc(P.iref, y0, y1, y2);
+            double d2 = cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
+            if (d2 <= P.cut2) { r.ref = 1; r.dref = sqrt(d2); }
+        } else exhaustive = true;
+    }
+    if (exhaustive) {
+        atomicAdd(stats, 1ull);
+        r = rec_zero();
+        for (int a = 0; a < P.nv; a++) {
+            double y0, y1, y2;
+            ysrc(a, y0, y1, y2);
+            for (int i = 0; i < P.nu; i++) {
+                double x0, x1, x2;
+                load3(xuf, ibase + i, x0, x1, x2);
+                double d2 = cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
+                if (d2 <= P.cut2) rec_offer(r, i, jbase + a, sqrt(d2), a == P.iref);
+            }
+        }
+    }
+    return r;
+}
+
+template <class TU, class TV>
+__global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU* xu, const TV* xv)
+{
+    extern __shared__ unsigned sh[];
+    hist_clear(sh, 2 * P.nbins);
+    int64_t total = (int64_t)P.F * P.Mv;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int f = (int)(it / P.Mv);
+        int v = (int)(it - (int64_t)f * P.Mv);
+        const Geo& g = B.geo[f];
+        const float4* yw = B.vw + (int64_t)f * P.Nv + (int64_t)v * P.nv;
+        float4 cv = B.vcen[(int64_t)f * P.Mv + v];
+        int vcell = B.vcell[(int64_t)f * P.Mv + v];
+        const float4* srt = B.sorted + (int64_t)f * B.nitems;
+        const int* st = B.start + (int64_t)f * (P.cap + 1);
+        const float4* solw = B.solw + (int64_t)f * P.Nu;
+        const TU* xuf = xu + (int64_t)f * P.Nu * 3;
+        const TV* xvf = xv + (int64_t)f * P.Nv * 3;
+        int64_t jbase = (int64_t)v * P.nv;
+        auto ysrc = [&](int a, double& y0, double& y1, double& y2) { load3(xvf, jbase + a, y0, y1, y2); };
+        int nc[3] = {g.nc[0], g.nc[1], g.nc[2]};
+        int cc[3] = {vcell % nc[0], (vcell / nc[0]) % nc[1], vcell / (nc[0] * nc[1])};
+        // distinct neighbour cells per dimension with all the shifts that map onto them
+        int dcell[3][3], dns[3], dsh[3][3][3];
+        for (int k = 0; k < 3; k++) {
+            dns[k] = 0;
+            int cnts[3] = {0, 0, 0};
+            for (int o = -1; o <= 1; o++) {
+                int m;
+                int c = nb_cell(cc[k], o, nc[k], m);
+                int q = -1;
+                for (int e = 0; e < dns[k]; e++) if (dcell[k][e] == c) q = e;
+                if (q < 0) { q = dns[k]++; dcell[k][q] = c; }
+                dsh[k][q][cnts[q]++] = m;
+            }
+            for (int e = 0; e < dns[k]; e++) dsh[k][e][2] = cnts[e] > 2 ? dsh[k][e][2] : 99;
+            for (int e = 0; e < dns[k]; e++) if (cnts[e] < 2) dsh[k][e][1] = 99;
+        }
+        for (int ez = 0; ez < dns[2]; ez++)
+        for (int ey = 0; ey < dns[1]; ey++)
+        for (int ex = 0; ex < dns[0]; ex++) {
+            int cell = (dcell[2][ez] * nc[1] + dcell[1][ey]) * nc[0] + dcell[0][ex];
+            for (int p = st[cell]; p < st[cell + 1]; p++) {
+                float4 cu = srt[p];
+                int u = __float_as_int(cu.w);
+                if (P.self_ && u == v) continue;
+                Best bm, br;
+                bm.init(); br.init();
+                bool any = false;
+                for (int sz = 0; sz < 3; sz++) {
+                    int mz = dsh[2][ez][sz]; if (mz == 99) break;
+                    for (int sy = 0; sy < 3; sy++) {
+                        int my = dsh[1][ey][sy]; if (my == 99) break;
+                        for (int sx = 0; sx < 3; sx++) {
+                            int mx = dsh[0][ex][sx]; if (mx == 99) break;
+                            float fx = (float)mx, fy = (float)my, fz = (float)mz;
+                            float tx = g.H32[0] * fx + g.H32[3] * fy + g.H32[6] * fz;
+                            float ty = g.H32[1] * fx + g.H32[4] * fy + g.H32[7] * fz;
+                            float tz = g.H32[2] * fx + g.H32[5] * fy + g.H32[8] * fz;
+                            float dx = cv.x - (cu.x + tx), dy = cv.y - (cu.y + ty), dz = cv.z - (cu.z + tz);
+                            if (dx * dx + dy * dy + dz * dz > g.rpre2) continue;
+                            any = true;
+                            pairs_mol(P, solw + (int64_t)u * P.nu, yw, tx, ty, tz, bm, br);
+                        }
+                    }
+                }
+                if (!any) continue;
+                Rec r = resolve_mols(P, g, xuf, u, v, ysrc, bm, br, &B.stats[0]);
+                hist_record(P, sh, B.cnt_gu, B.cnt_gv, r);
+                if (P.K > 0 && r.within) {
+                    const int* dp = B.drawn_ptr + (int64_t)f * (P.Mu + 1);
+                    const int* di = B.drawn_idx + (int64_t)f * P.K;
+                    for (int q = dp[u]; q < dp[u + 1]; q++) {
+                        int k = di[q];
+                        B.bflag[((int64_t)f * P.kb + k) * P.Mv + v] = inbulk(P, r) ? 1 : 0;
+                    }
+                }
+            }
+        }
+    }
+    hist_flush(sh, P.nbins, B.cnt_md, B.cnt_rdf);
+}
+
+template <class TU>
+__global__ void __launch_bounds__(BLOCK) k_eval_mols_rand(KP P, Bufs B, const TU* xu)
+{
+    extern __shared__ unsigned sh[];
+    hist_clear(sh, 2 * P.nbins);
+    int64_t total = (int64_t)P.F * P.K * P.Mv;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int64_t fk = it / P.Mv;
+        int v = (int)(it - fk * P.Mv);
+        int f = (int)(fk / P.K);
+        int u = B.uk[fk];
+        if (P.self_ && u == v) continue;
+        const Geo& g = B.geo[f];
+        const float4* yw = B.vw_r + fk * P.Nv + (int64_t)v * P.nv;
+        float4 cv = B.vcen_r[fk * P.Mv + v];
+        float4 cu = B.ucen[(int64_t)f * P.Mu + u];
+        const float4* xs = B.solw + (int64_t)f * P.Nu + (int64_t)u * P.nu;
+        Best bm, br;
+        bm.init(); br.init();
+        bool any = false;
+        for (int mz = -1; mz <= 1; mz++)
+        for (int my = -1; my <= 1; my++)
+        for (int mx = -1; mx <= 1; mx++) {
+            float fx = (float)mx, fy = (float)my, fz = (float)mz;
+            float tx = g.H32[0] * fx + g.H32[3] * fy + g.H32[6] * fz;
+            float ty = g.H32[1] * fx + g.H32[4] * fy + g.H32[7] * fz;
+            float tz = g.H32[2] * fx + g.H32[5] * fy + g.H32[8] * fz;
+            float dx = cv.x - (cu.x + tx), dy = cv.y - (cu.y + ty), dz = cv.z - (cu.z + tz);
+            if (dx * dx + dy * dy + dz * dz > g.rpre2) continue;
+            any = true;
+            pairs_mol(P, xs, yw, tx, ty, tz, bm, br);
+        }
+        if (!any) continue;
+        const TU* xuf = xu + (int64_t)f * P.Nu * 3;
+        int64_t jbase = (int64_t)v * P.nv;
+        auto ysrc = [&](int a, double& y0, double& y1, double& y2) {
+            const double* pp = B.P + (fk * P.Nv + jbase + a) * 3; y0 = pp[0]; y1 = pp[1]; y2 = pp[2];
+        };
+        Rec r = resolve_mols(P, g, xuf, u, v, ysrc, bm, br, &B.stats[0]);
+        hist_record(P, sh, B.cnt_gu_r, B.cnt_gv_r, r);
+    }
+    hist_flush(sh, P.nbins, B.cnt_md_r, B.cnt_rdf_r);
+}
+
+// ---------------------------------------------------------------------------
+// bulk lists and random placement
+// ---------------------------------------------------------------------------
+__global__ void k_bulk_init(KP P, Bufs B)
+{
+    int64_t total = (int64_t)P.F * P.kb * P.Mv;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int64_t fk = it / P.Mv;
+        int v = (int)(it - fk * P.Mv);
+        unsigned char fl = P.usecutoff ? 0 : 1;
+        if (P.self_ && B.uk[fk] == v) fl = 0;   // the bulk pool excludes isolute, src/mddf.jl:410
+        B.bflag[it] = fl;
+    }
+}
+
+// ordered compaction of the bulk flags, one block per (frame, list)
+__global__ void __launch_bounds__(1024) k_bulk_compact(KP P, Bufs B)
+{
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    int64_t fk = blockIdx.x;
+    const unsigned char* fl = B.bflag + fk * P.Mv;
+    int* out = B.blist + fk * P.Mv;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int base = 0; base < P.Mv; base += blockDim.x) {
+        int v = base + threadIdx.x;
+        int flag = (v < P.Mv) ? fl[v] : 0;
+        ull mask = __ballot(flag);
+        int pre = __popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wid] = __popcll(mask);
+        __syncthreads();
+        int woff = 0, tot = 0;
+        for (int w = 0; w < nw; w++) { if (w < wid) woff += wsum[w]; tot += wsum[w]; }
+        if (flag) out[carry + woff + pre] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) B.nbulk[fk] = carry;
+}
+
+template <class TV>
+struct MolAcc {   // one molecule of the placement buffer, xyz interleaved
+    double* p;
+    __device__ double operator()(int a, int k) const { return p[3 * a + k]; }
+    __device__ void set(int a, double x, double y, double z) { p[3 * a] = x; p[3 * a + 1] = y; p[3 * a + 2] = z; }
+};
+
+// randomize_solvent!, src/mddf.jl:65-88, for every slot of every random pass
+template <class TV>
+__global__ void k_place(KP P, Bufs B, const TV* xv)
+{
+    int64_t total = (int64_t)P.F * P.K * P.Mv;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int64_t fk = it / P.Mv;
+        int v = (int)(it - fk * P.Mv);
+        int f = (int)(fk / P.K);
+        int k = (int)(fk - (int64_t)f * P.K);
+        const Geo& g = B.geo[f];
+        cm::u32x4 r = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT}, P.key0, P.key1);
+        cm::u32x4 r2 = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT + 1u}, P.key0, P.key1);
+        int64_t lk = (int64_t)f * P.kb + (P.kb == 1 ? 0 : k);
+        int nb = B.nbulk[lk];
+        int64_t jmol = nb > 0 ? (int64_t)B.blist[lk * P.Mv + cm::draw_index(r.x, nb)] : cm::draw_index(r.x, P.Mv);
+        const TV* src = xv + (int64_t)f * P.Nv * 3 + jmol * P.nv * 3;
+        double* dst = B.P + (fk * P.Nv + (int64_t)v * P.nv) * 3;
+        for (int q = 0; q < 3 * P.nv; q++) dst[q] = (double)src[q];
+        MolAcc<TV> acc{dst};
+        double uc[3] = {cm::u01(r.y), cm::u01(r.z), cm::u01(r.w)};
+        double ua[3] = {cm::u01(r2.x), cm::u01(r2.y), cm::u01(r2.z)};
+        cm::random_move(acc, P.nv, P.iref, g.cell, uc, ua);
+        // the new centre H*uc lies inside the cell: its cell follows from uc
+        int cell;
+        cell_of_frac(g, uc[0], uc[1], uc[2], cell);
+        B.vcell_r[fk * P.Mv + v] = cell;
+        const double* H = g.cell.H;
+        double c0 = H[0] * uc[0] + H[3] * uc[1] + H[6] * uc[2];
+        double c1 = H[1] * uc[0] + H[4] * uc[1] + H[7] * uc[2];
+        double c2 = H[2] * uc[0] + H[5] * uc[1] + H[8] * uc[2];
+        B.vcen_r[fk * P.Mv + v] = mk4((float)c0, (float)c1, (float)c2, 0.f);
+        for (int a = 0; a < P.nv; a++)
+            B.vw_r[fk * P.Nv + (int64_t)v * P.nv + a] = mk4((float)dst[3 * a], (float)dst[3 * a + 1], (float)dst[3 * a + 2], 0.f);
+    }
+}
+
+// integer hits -> FP64 accumulators
+__global__ void k_flush(ull* cnt, double* acc, ull* hits, int64_t n, int64_t nhist, int64_t half_lo, int64_t half_hi,
+                        double w)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        ull c = cnt[k];
+        if (!c) continue;
+        double mult = (k >= half_lo && k < half_hi) ? w / 2 : w;
+        acc[k] += (double)c * mult;
+        if (k < nhist) hits[k] += c;
+        cnt[k] = 0ull;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) return fail(CM_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int grid_for(int64_t n)
+{
+    int64_t b = (n + BLOCK - 1) / BLOCK;
+    if (b < 1) b = 1;
+    if (b > 4096) b = 4096;
+    return (int)b;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+int ensure(DevBuf& b, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    if (b.bytes >= bytes) return CM_OK;
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) return fail(CM_ENOMEM, std::string("hipMalloc failed for ") + std::to_string(bytes) + " bytes");
+    b.bytes = bytes;
+    return CM_OK;
+}
+
+void plane_spacings(const double* H, double sp[3])
+{
+    double V = std::fabs(cm::cell_volume(H));
+    for (int k = 0; k < 3; k++) {
+        const double *p = H + 3 * ((k + 1) % 3), *q = H + 3 * ((k + 2) % 3);
+        double x = p[1] * q[2] - p[2] * q[1], y = p[2] * q[0] - p[0] * q[2], z = p[0] * q[1] - p[1] * q[0];
+        sp[k] = V / std::sqrt(x * x + y * y + z * z);
+    }
+}
+
+}  // namespace
+
+struct cm_job {
+    cm_job_desc d;
+    KP kp;
+    bool atoms_family = true;
+    hipStream_t stream = nullptr;
+    int64_t Nu = 0, Nv = 0;
+    int K = 0, kb = 0;
+    // accumulators: [md | md_r | rdf | rdf_r | gu | gv | gu_r | gv_r]
+    int64_t acc_len = 0;
+    DevBuf cnt, acc, hits, stats;
+    double pending_w = 0.0;
+    bool pending = false;
+    double volume_total = 0.0;
+    DevBuf gmap;   // group maps
+    // batch buffers
+    DevBuf geo, cells_cnt, cells_start, tmp4, tmpcell, sorted, solw, ucen, vw, vcell, vcen;
+    DevBuf P, vw_r, vcell_r, vcen_r, bflag, blist, nbulk, uk, drawn_ptr, drawn_idx;
+    DevBuf xin_u, xin_v;
+    int Fcap = 0;
+    int cap_cells = 0;
+    bool finished = false;
+    bool profiling = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    double eval_ms = 0.0, total_ms = 0.0;
+    int64_t eval_launches = 0, total_launches = 0, frames = 0;
+    std::vector<Geo> hgeo, hgeo_out;
+    std::vector<int> huk, hdp, hdi;
+};
+
+namespace {
+
+int64_t per_frame_bytes(const cm_job* J, int dtype_size, bool host_input)
+{
+    int64_t Mu = J->d.nmols_u, Mv = J->d.nmols_v;
+    int64_t b = sizeof(Geo) + 64;
+    if (host_input) b += (J->Nu + J->Nv) * 3 * dtype_size;
+    int64_t items = J->atoms_family ? J->d.natoms_per_mol_u : Mu;
+    b += items * (16 + 16 + 4);
+    if (!J->atoms_family) b += J->Nu * 16 + Mu * 16 + (Mu + 1) * 4 + J->K * 8;
+    b += J->Nv * 16 + Mv * 20;
+    b += (int64_t)J->K * (J->Nv * (24 + 16) + Mv * 20);
+    b += (int64_t)J->kb * Mv * 5 + J->kb * 4;
+    return b;
+}
+
+int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, const void* xv_dev, int dtype,
+              const double* cells, int64_t cap)
+{
+    KP P = J->kp;
+    P.F = F;
+    P.cap = (int)cap;
+    // geometry on the host: cells, inverse, spacings; the device adds radii and grid
+    J->hgeo.assign(F, Geo());
+    double cutsys = P.cutsys;
+    for (int f = 0; f < F; f++) {
+        Geo& g = J->hgeo[f];
+        memset(&g, 0, sizeof(Geo));
+        cm::cell_setup(cells + 9 * f, J->d.triclinic, &g.cell);
+        for (int k = 0; k < 9; k++) g.H32[k] = (float)g.cell.H[k];
+        plane_spacings(g.cell.H, g.sp);
+        double spmin = std::min(g.sp[0], std::min(g.sp[1], g.sp[2]));
+        if (!(cutsys < 0.5 * spmin * (1.0 - 1e-9)))
+            return fail(CM_EUNSUPPORTED, "cutoff must be smaller than half of the smallest plane spacing of the unit cell");
+        g.frame = (uint32_t)frame_index[f];
+    }
+    HIPCHK(hipMemcpyAsync(J->geo.p, J->hgeo.data(), sizeof(Geo) * F, hipMemcpyHostToDevice, J->stream));
+    Bufs B;
+    memset(&B, 0, sizeof(B));
+    B.geo = (Geo*)J->geo.p;
+    B.cnt = (int*)J->cells_cnt.p;
+    B.start = (int*)J->cells_start.p;
+    B.tmp4 = (float4*)J->tmp4.p;
+    B.tmpcell = (int*)J->tmpcell.p;
+    B.sorted = (float4*)J->sorted.p;
+    B.nitems = J->atoms_family ? P.nu : P.Mu;
+    B.solw = (float4*)J->solw.p;
+    B.ucen = (float4*)J->ucen.p;
+    B.vw = (float4*)J->vw.p;
+    B.vcell = (int*)J->vcell.p;
+    B.vcen = (float4*)J->vcen.p;
+    B.P = (double*)J->P.p;
+    B.vw_r = (float4*)J->vw_r.p;
+    B.vcell_r = (int*)J->vcell_r.p;
+    B.vcen_r = (float4*)J->vcen_r.p;
+    B.bflag = (unsigned char*)J->bflag.p;
+    B.blist = (int*)J->blist.p;
+    B.nbulk = (int*)J->nbulk.p;
+    B.uk = (int*)J->uk.p;
+    B.drawn_ptr = (int*)J->drawn_ptr.p;
+    B.drawn_idx = (int*)J->drawn_idx.p;
+    ull* c = (ull*)J->cnt.p;
+    int64_t nb = P.nbins, ngu = J->d.n_groups_u, ngv = J->d.n_groups_v;
+    B.cnt_md = c; B.cnt_md_r = c + nb; B.cnt_rdf = c + 2 * nb; B.cnt_rdf_r = c + 3 * nb;
+    B.cnt_gu = c + 4 * nb; B.cnt_gv = B.cnt_gu + ngu * nb; B.cnt_gu_r = B.cnt_gv + ngv * nb; B.cnt_gv_r = B.cnt_gu_r + ngu * nb;
+    B.stats = (ull*)J->stats.p;
+
+    // random draws of the reference solute molecules (mddf_frame!, src/mddf.jl:374-376)
+    if (!J->atoms_family && J->K > 0) {
+        J->huk.assign((size_t)F * J->K, 0);
+        J->hdp.assign((size_t)F * (P.Mu + 1), 0);
+        J->hdi.assign((size_t)F * J->K, 0);
+        for (int f = 0; f < F; f++) {
+            int* dp = &J->hdp[(size_t)f * (P.Mu + 1)];
+            for (int k = 0; k < J->K; k++) {
+                cm::u32x4 r = cm::philox4x32_10({(uint32_t)k, 0u, (uint32_t)frame_index[f], cm::TAG_REF}, P.key0, P.key1);
+                int u = (int)cm::draw_index(r.x, P.Mu);
+                J->huk[(size_t)f * J->K + k] = u;
+                dp[u + 1]++;
+            }
+            for (int u = 0; u < P.Mu; u++) dp[u + 1] += dp[u];
+            std::vector<int> fill(P.Mu, 0);
+            for (int k = 0; k < J->K; k++) {
+                int u = J->huk[(size_t)f * J->K + k];
+                J->hdi[(size_t)f * J->K + dp[u] + fill[u]++] = k;
+            }
+        }
+        HIPCHK(hipMemcpyAsync(J->uk.p, J->huk.data(), sizeof(int) * J->huk.size(), hipMemcpyHostToDevice, J->stream));
+        HIPCHK(hipMemcpyAsync(J->drawn_ptr.p, J->hdp.data(), sizeof(int) * J->hdp.size(), hipMemcpyHostToDevice, J->stream));
+        HIPCHK(hipMemcpyAsync(J->drawn_idx.p, J->hdi.data(), sizeof(int) * J->hdi.size(), hipMemcpyHostToDevice, J->stream));
+    } else if (J->K > 0) {
+        HIPCHK(hipMemsetAsync(J->uk.p, 0, sizeof(int) * (size_t)F * J->K, J->stream));
+    }
+    HIPCHK(hipMemsetAsync(J->cells_cnt.p, 0, sizeof(int) * (size_t)F * (cap + 1), J->stream));
+
+    bool f32 = dtype == CM_F32;
+    hipStream_t s = J->stream;
+    if (J->profiling) HIPCHK(hipEventRecord(J->e0, s));
+    int mols = J->atoms_family ? 0 : 1;
+    int64_t nrad = (int64_t)F * (P.Mv + (mols ? P.Mu : 0));
+    if (f32) hipLaunchKernelGGL(k_radii<float>, dim3(grid_for(nrad)), dim3(BLOCK), 0, s, P, B, (const float*)xu_dev, (const float*)xv_dev, mols);
+    else hipLaunchKernelGGL(k_radii<double>, dim3(grid_for(nrad)), dim3(BLOCK), 0, s, P, B, (const double*)xu_dev, (const double*)xv_dev, mols);
+    hipLaunchKernelGGL(k_grid, dim3(1), dim3(64), 0, s, P, B, mols);
+    if (J->atoms_family) {
+        int64_t n = (int64_t)F * P.nu;
+        if (f32) hipLaunchKernelGGL(k_wrap_atoms<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const float*)xu_dev);
+        else hipLaunchKernelGGL(k_wrap_atoms<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const double*)xu_dev);
+    } else {
+        int64_t n = (int64_t)F * P.Mu;
+        if (f32) hipLaunchKernelGGL(k_prep_solute_mols<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const float*)xu_dev);
+        else hipLaunchKernelGGL(k_prep_solute_mols<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const double*)xu_dev);
+    }
+    hipLaunchKernelGGL(k_scan_cells, dim3(F), dim3(1024), 0, s, P, B);
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)F * B.nitems)), dim3(BLOCK), 0, s, P, B, B.nitems);
+    {
+        int64_t n = (int64_t)F * P.Mv;
+        if (f32) hipLaunchKernelGGL(k_prep_solvent<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const float*)xv_dev);
+        else hipLaunchKernelGGL(k_prep_solvent<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const double*)xv_dev);
+    }
+    if (P.K > 0) hipLaunchKernelGGL(k_bulk_init, dim3(grid_for((int64_t)F * P.kb * P.Mv)), dim3(BLOCK), 0, s, P, B);
+    size_t shm = sizeof(unsigned) * 2 * P.nbins;
+    if (J->profiling) HIPCHK(hipEventRecord(J->e1, s));
+    {
+        int64_t n = (int64_t)F * P.Mv;
+        if (J->atoms_family) {
+            if (f32) hipLaunchKernelGGL((k_eval_atoms<float, float, false>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const float*)xu_dev, (const float*)xv_dev);
+            else hipLaunchKernelGGL((k_eval_atoms<double, double, false>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const double*)xu_dev, (const double*)xv_dev);
+        } else {
+            if (f32) hipLaunchKernelGGL((k_eval_mols_real<float, float>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const float*)xu_dev, (const float*)xv_dev);
+            else hipLaunchKernelGGL((k_eval_mols_real<double, double>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const double*)xu_dev, (const double*)xv_dev);
+        }
+    }
+    if (J->profiling) HIPCHK(hipEventRecord(J->e2, s));
+    if (P.K > 0) {
+        hipLaunchKernelGGL(k_bulk_compact, dim3(F * P.kb), dim3(1024), 0, s, P, B);
+        int64_t n = (int64_t)F * P.K * P.Mv;
+        if (f32) hipLaunchKernelGGL(k_place<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const float*)xv_dev);
+        else hipLaunchKernelGGL(k_place<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const double*)xv_dev);
+    }
+    HIPCHK(hipGetLastError());
+    hipEvent_t er0 = nullptr, er1 = nullptr;
+    if (P.K > 0) {
+        if (J->profiling) { HIPCHK(hipEventCreate(&er0)); HIPCHK(hipEventCreate(&er1)); HIPCHK(hipEventRecord(er0, s)); }
+        int64_t n = (int64_t)F * P.K * P.Mv;
+        if (J->atoms_family) {
+            if (f32) hipLaunchKernelGGL((k_eval_atoms<float, double, true>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const float*)xu_dev, (const double*)nullptr);
+            else hipLaunchKernelGGL((k_eval_atoms<double, double, true>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const double*)xu_dev, (const double*)nullptr);
+        } else {
+            if (f32) hipLaunchKernelGGL(k_eval_mols_rand<float>, dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const float*)xu_dev);
+            else hipLaunchKernelGGL(k_eval_mols_rand<double>, dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const double*)xu_dev);
+        }
+        if (J->profiling) HIPCHK(hipEventRecord(er1, s));
+    }
+    HIPCHK(hipGetLastError());
+    // geometry errors detected on the device
+    J->hgeo_out.assign(F, Geo());
+    HIPCHK(hipMemcpyAsync(J->hgeo_out.data(), J->geo.p, sizeof(Geo) * F, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int f = 0; f < F; f++) {
+        if (J->hgeo_out[f].err == 2)
+            return fail(CM_EUNSUPPORTED, "molecules too large for the unit cell: cutoff plus molecule radii exceed a plane spacing");
+        if (J->hgeo_out[f].err)
+            return fail(CM_EHIP, "internal error: cell grid larger than its allocation");
+    }
+    if (J->profiling) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, J->e1, J->e2));
+        J->eval_ms += ms;
+        J->eval_launches += 1;
+        float tot = 0.f;
+        if (P.K > 0) {
+            HIPCHK(hipEventElapsedTime(&ms, er0, er1));
+            J->eval_ms += ms;
+            J->eval_launches += 1;
+            HIPCHK(hipEventElapsedTime(&tot, J->e0, er1));
+        } else {
+            HIPCHK(hipEventElapsedTime(&tot, J->e0, J->e2));
+        }
+        J->total_ms += tot;
+        J->total_launches += 1;
+        hipEventDestroy(er0);
+        hipEventDestroy(er1);
+    }
+    J->frames += F;
+    return CM_OK;
+}
+
+int alloc_batch(cm_job* J, int F, int64_t cap, int dtype_size, bool host_input)
+{
+    int r;
+    int64_t Mu = J->d.nmols_u, Mv = J->d.nmols_v;
+    int64_t items = J->atoms_family ? J->d.natoms_per_mol_u : Mu;
+    if ((r = ensure(J->geo, sizeof(Geo) * F))) return r;
+    if ((r = ensure(J->cells_cnt, sizeof(int) * F * (cap + 1)))) return r;
+    if ((r = ensure(J->cells_start, sizeof(int) * F * (cap + 1)))) return r;
+    if ((r = ensure(J->tmp4, 16 * F * items))) return r;
+    if ((r = ensure(J->tmpcell, 4 * F * items))) return r;
+    if ((r = ensure(J->sorted, 16 * F * items))) return r;
+    if (!J->atoms_family) {
+        if ((r = ensure(J->solw, 16 * F * J->Nu))) return r;
+        if ((r = ensure(J->ucen, 16 * F * Mu))) return r;
+        if ((r = ensure(J->drawn_ptr, 4 * F * (Mu + 1)))) return r;
+        if ((r = ensure(J->drawn_idx, 4 * F * std::max(J->K, 1)))) return r;
+    }
+    if ((r = ensure(J->vw, 16 * F * J->Nv))) return r;
+    if ((r = ensure(J->vcell, 4 * F * Mv))) return r;
+    if ((r = ensure(J->vcen, 16 * F * Mv))) return r;
+    if (J->K > 0) {
+        if ((r = ensure(J->P, (size_t)24 * F * J->K * J->Nv))) return r;
+        if ((r = ensure(J->vw_r, (size_t)16 * F * J->K * J->Nv))) return r;
+        if ((r = ensure(J->vcell_r, (size_t)4 * F * J->K * Mv))) return r;
+        if ((r = ensure(J->vcen_r, (size_t)16 * F * J->K * Mv))) return r;
+        if ((r = ensure(J->bflag, (size_t)F * J->kb * Mv))) return r;
+        if ((r = ensure(J->blist, (size_t)4 * F * J->kb * Mv))) return r;
+        if ((r = ensure(J->nbulk, (size_t)4 * F * J->kb))) return r;
+        if ((r = ensure(J->uk, (size_t)4 * F * J->K))) return r;
+    }
+    if (host_input) {
+        if ((r = ensure(J->xin_u, (size_t)dtype_size * F * J->Nu * 3))) return r;
+        if ((r = ensure(J->xin_v, (size_t)dtype_size * F * J->Nv * 3))) return r;
+    }
+    return CM_OK;
+}
+
+int flush(cm_job* J, double w)
+{
+    int64_t nb = J->d.nbins;
+    int64_t half_lo = 0, half_hi = 0;
+    if (J->d.autocorrelation) {   // self: both group hits of a record are w/2 (update_counters.jl:48-53)
+        half_lo = 4 * nb;
+        half_hi = J->acc_len;
+    }
+    hipLaunchKernelGGL(k_flush, dim3(grid_for(J->acc_len)), dim3(BLOCK), 0, J->stream, (ull*)J->cnt.p, (double*)J->acc.p,
+                       (ull*)J->hits.p, J->acc_len, 4 * nb, half_lo, half_hi, w);
+    HIPCHK(hipGetLastError());
+    return CM_OK;
+}
+
+int submit_common(cm_job* J, int32_t nframes, const int64_t* frame_index, const void* xu, const void* xv, int32_t dtype,
+                  const double* cells, const double* weights, bool host_input)
+{
+    if (!J) return fail(CM_EINVAL, "null job");
+    if (J->finished) return fail(CM_ESTATE, "job already finished");
+    if (nframes < 0 || (nframes > 0 && (!frame_index || !xu || !xv || !cells || !weights)))
+        return fail(CM_EINVAL, "invalid frame arrays");
+    if (dtype != CM_F64 && dtype != CM_F32) return fail(CM_EINVAL, "dtype must be CM_F64 or CM_F32");
+    int ds = dtype == CM_F32 ? 4 : 8;
+    for (int f = 0; f < nframes; f++) {
+        if (!(weights[f] > 0.0) || !std::isfinite(weights[f])) return fail(CM_EINVAL, "frame weights must be positive and finite");
+        if (frame_index[f] < 1 || frame_index[f] > 0xFFFFFFFFll) return fail(CM_EINVAL, "frame_index must be in [1, 2^32)");
+    }
+    int64_t pfb = per_frame_bytes(J, ds, host_input);
+    int64_t budget = J->d.memory_budget > 0 ? J->d.memory_budget : (int64_t)4 << 30;
+    int64_t Fmax = std::max<int64_t>(1, budget / std::max<int64_t>(pfb, 1));
+    if (J->d.max_batch_frames > 0) Fmax = std::min<int64_t>(Fmax, J->d.max_batch_frames);
+    Fmax = std::min<int64_t>(Fmax, 64);
+    double cutsys = J->kp.cutsys;
+    int f0 = 0;
+    while (f0 < nframes) {
+        // a batch holds frames of one weight (the weight multiplies the integer hits at flush)
+        double w = weights[f0];
+        int F = 1;
+        while (f0 + F < nframes && F < Fmax && weights[f0 + F] == w) F++;
+        if (J->pending && J->pending_w != w) {
+            int r = flush(J, J->pending_w);
+            if (r) return r;
+        }
+        J->pending = true;
+        J->pending_w = w;
+        int64_t cap = 1;
+        for (int f = 0; f < F; f++) {
+            cm::Cell64 c;
+            cm::cell_setup(cells + 9 * (f0 + f), J->d.triclinic, &c);
+            double sp[3];
+            plane_spacings(c.H, sp);
+            int64_t n = 1;
+            for (int k = 0; k < 3; k++) n *= std::max<int64_t>(1, (int64_t)std::floor(sp[k] / cutsys));
+            if (n > (int64_t)1 << 26) return fail(CM_EUNSUPPORTED, "unit cell too large for the cell grid");
+            cap = std::max(cap, n);
+            J->volume_total += w * cm::cell_volume(c.H);
+        }
+        int r = alloc_batch(J, F, cap, ds, host_input);
+        if (r) return r;
+        const void* du;
+        const void* dv;
+        if (host_input) {
+            HIPCHK(hipMemcpyAsync(J->xin_u.p, (const char*)xu + (size_t)f0 * J->Nu * 3 * ds, (size_t)F * J->Nu * 3 * ds,
+                                  hipMemcpyHostToDevice, J->stream));
+            HIPCHK(hipMemcpyAsync(J->xin_v.p, (const char*)xv + (size_t)f0 * J->Nv * 3 * ds, (size_t)F * J->Nv * 3 * ds,
+                                  hipMemcpyHostToDevice, J->stream));
+            du = J->xin_u.p;
+            dv = J->xin_v.p;
+        } else {
+            du = (const char*)xu + (size_t)f0 * J->Nu * 3 * ds;
+            dv = (const char*)xv + (size_t)f0 * J->Nv * 3 * ds;
+        }
+        r = run_batch(J, F, frame_index + f0, du, dv, dtype, cells + 9 * f0, cap);
+        if (r) return r;
+        f0 += F;
+    }
+    return CM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cm_last_error(void) { return g_err.c_str(); }
+
+int cm_job_create(const cm_job_desc* desc, cm_job** out)
+{
+    if (!desc || !out) return fail(CM_EINVAL, "null argument");
+    *out = nullptr;
+    const cm_job_desc& d = *desc;
+    if (d.nmols_u < 1 || d.natoms_per_mol_u < 1 || d.nmols_v < 1 || d.natoms_per_mol_v < 1)
+        return fail(CM_EINVAL, "selections must hold at least one molecule of at least one atom");
+    if (d.irefatom < 1 || d.irefatom > d.natoms_per_mol_v)
+        return fail(CM_EINVAL, "irefatom must be between 1 and natoms_per_mol_v");
+    if (d.nbins < 1 || d.nbins > MAX_NBINS) return fail(CM_EINVAL, "nbins out of range [1, 8192]");
+    if (!(d.binstep > 0.0) || !(d.cutoff > 0.0) || !(d.dbulk > 0.0)) return fail(CM_EINVAL, "binstep, cutoff and dbulk must be positive");
+    if ((int64_t)cm::setbin(d.cutoff, d.binstep) != d.nbins) return fail(CM_EINVAL, "nbins must equal setbin(cutoff, binstep)");
+    if (d.usecutoff ? !(d.dbulk < d.cutoff) : !(d.dbulk <= d.cutoff))
+        return fail(CM_EINVAL, "dbulk must not exceed cutoff (the histogram range)");
+    if (d.n_random_samples < 1 && !d.coordination_number_only) return fail(CM_EINVAL, "n_random_samples must be greater than 0");
+    if (d.n_groups_u < 1 || d.n_groups_v < 1) return fail(CM_EINVAL, "group counts must be positive");
+    if (!d.groups_u_ptr && d.n_groups_u != d.natoms_per_mol_u) return fail(CM_EINVAL, "n_groups_u must equal natoms_per_mol_u without custom groups");
+    if (!d.groups_v_ptr && d.n_groups_v != (d.autocorrelation ? d.natoms_per_mol_u : d.natoms_per_mol_v))
+        return fail(CM_EINVAL, "n_groups_v must equal the atoms per molecule without custom groups");
+    if (d.autocorrelation && (d.nmols_u != d.nmols_v || d.natoms_per_mol_u != d.natoms_per_mol_v))
+        return fail(CM_EINVAL, "autocorrelation needs identical solute and solvent selections");
+    if (d.seed == 0) return fail(CM_EINVAL, "seed must be positive");
+    int64_t Nu = (int64_t)d.nmols_u * d.natoms_per_mol_u, Nv = (int64_t)d.nmols_v * d.natoms_per_mol_v;
+    if (Nu > INT32_MAX / 4 || Nv > INT32_MAX / 4) return fail(CM_EINVAL, "selections too large");
+    cm_job* J = new cm_job();
+    J->d = d;
+    J->Nu = Nu;
+    J->Nv = Nv;
+    J->atoms_family = (d.nmols_u == 1 && !d.autocorrelation);
+    J->K = d.coordination_number_only ? 0 : d.n_random_samples;
+    J->kb = J->K > 0 ? (J->atoms_family ? 1 : J->K) : 0;
+    KP& P = J->kp;
+    memset(&P, 0, sizeof(P));
+    P.Mu = d.nmols_u; P.nu = d.natoms_per_mol_u; P.Mv = d.nmols_v; P.nv = d.natoms_per_mol_v;
+    P.self_ = d.autocorrelation ? 1 : 0;
+    P.iref = d.irefatom - 1;
+    P.nbins = d.nbins;
+    P.usecutoff = d.usecutoff ? 1 : 0;
+    P.binstep = d.binstep;
+    P.cutsys = d.usecutoff ? d.cutoff : d.dbulk;
+    P.cut2 = P.cutsys * P.cutsys;
+    P.dbulk = d.dbulk;
+    P.K = J->K;
+    P.kb = J->kb;
+    P.key0 = (uint32_t)d.seed;
+    P.key1 = (uint32_t)(d.seed >> 32);
+    P.Nu = Nu;
+    P.Nv = Nv;
+    int r;
+    auto bail = [&](int code) { cm_job_destroy(J); return code; };
+    if (hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(CM_EHIP, "hipStreamCreate failed"));
+    int64_t nb = d.nbins;
+    J->acc_len = 4 * nb + 2 * (int64_t)d.n_groups_u * nb + 2 * (int64_t)d.n_groups_v * nb;
+    if ((r = ensure(J->cnt, 8 * J->acc_len))) return bail(r);
+    if ((r = ensure(J->acc, 8 * J->acc_len))) return bail(r);
+    if ((r = ensure(J->hits, 8 * 4 * nb))) return bail(r);
+    if ((r = ensure(J->stats, 64))) return bail(r);
+    if (hipMemset(J->cnt.p, 0, 8 * J->acc_len) != hipSuccess || hipMemset(J->acc.p, 0, 8 * J->acc_len) != hipSuccess ||
+        hipMemset(J->hits.p, 0, 8 * 4 * nb) != hipSuccess || hipMemset(J->stats.p, 0, 64) != hipSuccess)
+        return bail(fail(CM_EHIP, "hipMemset failed"));
+    // group maps
+    std::vector<int32_t> gm;
+    int64_t ou_p = -1, ou_i = -1, ov_p = -1, ov_i = -1;
+    if (d.groups_u_ptr) {
+        if (!d.groups_u_idx) return bail(fail(CM_EINVAL, "groups_u_idx missing"));
+        int64_t n = d.natoms_per_mol_u;
+        int64_t m = d.groups_u_ptr[n];
+        for (int64_t k = 0; k < m; k++)
+            if (d.groups_u_idx[k] < 0 || d.groups_u_idx[k] >= d.n_groups_u) return bail(fail(CM_EINVAL, "solute group id out of range"));
+        ou_p = gm.size(); gm.insert(gm.end(), d.groups_u_ptr, d.groups_u_ptr + n + 1);
+        ou_i = gm.size(); gm.insert(gm.end(), d.groups_u_idx, d.groups_u_idx + m); gm.push_back(0);
+    }
+    if (d.groups_v_ptr) {
+        if (!d.groups_v_idx) return bail(fail(CM_EINVAL, "groups_v_idx missing"));
+        int64_t n = Nv;
+        int64_t m = d.groups_v_ptr[n];
+        int64_t lim = d.autocorrelation ? d.n_groups_u : d.n_groups_v;
+        for (int64_t k = 0; k < m; k++)
+            if (d.groups_v_idx[k] < 0 || d.groups_v_idx[k] >= lim) return bail(fail(CM_EINVAL, "solvent group id out of range"));
+        ov_p = gm.size(); gm.insert(gm.end(), d.groups_v_ptr, d.groups_v_ptr + n + 1);
+        ov_i = gm.size(); gm.insert(gm.end(), d.groups_v_idx, d.groups_v_idx + m); gm.push_back(0);
+    }
+    if (!gm.empty()) {
+        if ((r = ensure(J->gmap, 4 * gm.size()))) return bail(r);
+        if (hipMemcpy(J->gmap.p, gm.data(), 4 * gm.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(CM_EHIP, "hipMemcpy of group maps failed"));
+        int32_t* base = (int32_t*)J->gmap.p;
+        if (ou_p >= 0) { P.gu_ptr = base + ou_p; P.gu_idx = base + ou_i; }
+        if (ov_p >= 0) { P.gv_ptr = base + ov_p; P.gv_idx = base + ov_i; }
+    }
+    J->d.groups_u_ptr = J->d.groups_u_idx = J->d.groups_v_ptr = J->d.groups_v_idx = nullptr;
+    *out = J;
+    return CM_OK;
+}
+
+int cm_job_submit(cm_job* job, int32_t nframes, const int64_t* frame_index, const void* x_solute, const void* x_solvent,
+                  int32_t dtype, const double* cells, const double* weights)
+{
+    return submit_common(job, nframes, frame_index, x_solute, x_solvent, dtype, cells, weights, true);
+}
+
+int cm_job_submit_device(cm_job* job, int32_t nframes, const int64_t* frame_index, const void* d_x_solute,
+                         const void* d_x_solvent, int32_t dtype, const double* cells, const double* weights)
+{
+    return submit_common(job, nframes, frame_index, d_x_solute, d_x_solvent, dtype, cells, weights, false);
+}
+
+int cm_job_finish(cm_job* J, cm_raw* out)
+{
+    if (!J || !out) return fail(CM_EINVAL, "null argument");
+    if (J->pending) {
+        int r = flush(J, J->pending_w);
+        if (r) return r;
+        J->pending = false;
+    }
+    HIPCHK(hipStreamSynchronize(J->stream));
+    std::vector<double> acc(J->acc_len);
+    std::vector<ull> hits(4 * J->d.nbins);
+    HIPCHK(hipMemcpy(acc.data(), J->acc.p, 8 * J->acc_len, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hits.data(), J->hits.p, 8 * 4 * J->d.nbins, hipMemcpyDeviceToHost));
+    int64_t nb = J->d.nbins, ngu = J->d.n_groups_u, ngv = J->d.n_groups_v;
+    const double* a = acc.data();
+    auto cp = [&](double* dst, const double* src, int64_t n) { if (dst) memcpy(dst, src, 8 * n); };
+    cp(out->md_count, a, nb);
+    cp(out->md_count_random, a + nb, nb);
+    cp(out->rdf_count, a + 2 * nb, nb);
+    cp(out->rdf_count_random, a + 3 * nb, nb);
+    const double* g = a + 4 * nb;
+    cp(out->solute_group_count, g, ngu * nb);
+    cp(out->solvent_group_count, g + ngu * nb, ngv * nb);
+    cp(out->solute_group_count_random, g + (ngu + ngv) * nb, ngu * nb);
+    cp(out->solvent_group_count_random, g + (2 * ngu + ngv) * nb, ngv * nb);
+    out->volume_total = J->volume_total;
+    auto cpu = [&](uint64_t* dst, const ull* src) { if (dst) memcpy(dst, src, 8 * nb); };
+    cpu(out->hits_md, hits.data());
+    cpu(out->hits_md_random, hits.data() + nb);
+    cpu(out->hits_rdf, hits.data() + 2 * nb);
+    cpu(out->hits_rdf_random, hits.data() + 3 * nb);
+    J->finished = true;
+    return CM_OK;
+}
+
+int cm_job_set_profiling(cm_job* J, int32_t on)
+{
+    if (!J) return fail(CM_EINVAL, "null job");
+    if (on && !J->e0) {
+        HIPCHK(hipEventCreate(&J->e0));
+        HIPCHK(hipEventCreate(&J->e1));
+        HIPCHK(hipEventCreate(&J->e2));
+    }
+    J->profiling = on != 0;
+    return CM_OK;
+}
+
+int cm_job_kernel_stats(cm_job* J, cm_kernel_stats* out)
+{
+    if (!J || !out) return fail(CM_EINVAL, "null argument");
+    ull st[2] = {0, 0};
+    HIPCHK(hipStreamSynchronize(J->stream));
+    HIPCHK(hipMemcpy(st, J->stats.p, 16, hipMemcpyDeviceToHost));
+    out->eval_ms = J->eval_ms;
+    out->eval_launches = J->eval_launches;
+    out->total_ms = J->total_ms;
+    out->total_launches = J->total_launches;
+    out->frames = J->frames;
+    out->candidate_pairs = -1;
+    out->fp64_rechecks = (int64_t)st[0];
+    return CM_OK;
+}
+
+void cm_job_destroy(cm_job* J)
+{
+    if (!J) return;
+    if (J->stream) hipStreamSynchronize(J->stream);
+    DevBuf* bufs[] = {&J->cnt, &J->acc, &J->hits, &J->stats, &J->gmap, &J->geo, &J->cells_cnt, &J->cells_start,
+                      &J->tmp4, &J->tmpcell, &J->sorted, &J->solw, &J->ucen, &J->vw, &J->vcell, &J->vcen, &J->P,
+                      &J->vw_r, &J->vcell_r, &J->vcen_r, &J->bflag, &J->blist, &J->nbulk, &J->uk, &J->drawn_ptr,
+                      &J->drawn_idx, &J->xin_u, &J->xin_v};
+    for (DevBuf* b : bufs) if (b->p) hipFree(b->p);
+    if (J->e0) hipEventDestroy(J->e0);
+    if (J->e1) hipEventDestroy(J->e1);
+    if (J->e2) hipEventDestroy(J->e2);
+    if (J->stream) hipStreamDestroy(J->stream);
+    delete J;
+}
+
+int cm_device_alloc(size_t bytes, void** dptr)
+{
+    if (!dptr) return fail(CM_EINVAL, "null pointer");
+    hipError_t e = hipMalloc(dptr, bytes ? bytes : 16);
+    if (e != hipSuccess) return fail(CM_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return CM_OK;
+}
+
+int cm_device_upload(void* dptr, const void* host, size_t bytes)
+{
+    HIPCHK(hipMemcpy(dptr, host, bytes, hipMemcpyHostToDevice));
+    return CM_OK;
+}
+
+int cm_device_free(void* dptr)
+{
+    HIPCHK(hipFree(dptr));
+    return CM_OK;
+}
+
+int cm_device_synchronize(void)
+{
+    HIPCHK(hipDeviceSynchronize());
+    return CM_OK;
+}
+
+}  // extern "C"

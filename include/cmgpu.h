@@ -1,0 +1,151 @@
+/*
+ * cmgpu.h -- C ABI of the MI355X MDDF library (libcmgpu.so).
+ *
+ * The library replaces the per-frame accumulation loop of ComplexMixtures.jl:
+ * the chunked frame loop of `mddf(trajectory, options; frame_weights,
+ * coordination_number_only, low_memory)` (src/mddf.jl:285-338), which calls
+ * `mddf_frame!` (src/mddf.jl:361-429) or `coordination_number_frame!`
+ * (src/mddf.jl:438-471) and reduces chunks with `sum!` (src/results.jl:629-649).
+ * The caller keeps everything above and below that loop: option validation,
+ * trajectory reading, `Result` allocation and `finalresults!`
+ * (src/results.jl:311-469).
+ *
+ * Conventions
+ *   - Plain C types only; every buffer is owned by the caller.  Host pointers
+ *     are not retained after a call returns.
+ *   - Coordinates: [frame][atom][3], float64 or float32 (CM_F64 / CM_F32).
+ *     Solute atoms are all atoms of the solute selection (nmols_u molecules of
+ *     natoms_per_mol_u atoms, molecule-major), likewise for the solvent.  In
+ *     autocorrelation mode the caller passes the same array for both.
+ *   - Unit cells: [frame][9] float64, column major: lattice vectors a, b, c as
+ *     consecutive triples (the reference's unit-cell matrix columns).
+ *   - Group counts: [group][bin] (Result.*_group_count, Vector{Vector{Float64}}).
+ *   - Errors: negative return codes; cm_last_error() gives the message of the
+ *     last failing call on the calling thread.  No exceptions cross the ABI.
+ *   - Threading: one job handle per thread at a time; distinct handles may be
+ *     driven concurrently (each has its own HIP stream).
+ */
+#ifndef CMGPU_H
+#define CMGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CM_OK = 0,
+    CM_EINVAL = -1,        /* invalid argument (maps to ArgumentError) */
+    CM_EHIP = -2,          /* HIP runtime error */
+    CM_ENOMEM = -3,        /* device allocation failed or over the memory budget */
+    CM_EUNSUPPORTED = -4,  /* geometry the kernels do not handle (e.g. cutoff >= half the cell) */
+    CM_ESTATE = -5         /* call out of order (submit after finish, ...) */
+};
+
+enum { CM_F64 = 0, CM_F32 = 1 };
+
+typedef struct cm_job cm_job;
+
+/* Job description: the fields of Options (src/Options.jl:6-42), of the two
+ * AtomSelections (src/AtomSelection.jl:48-59) and of TrajectoryMetaData
+ * (src/Trajectory.jl:180-249) that the frame loop reads. */
+typedef struct cm_job_desc {
+    int32_t nmols_u;            /* solute.nmols */
+    int32_t natoms_per_mol_u;   /* solute.natomspermol */
+    int32_t nmols_v;            /* solvent.nmols */
+    int32_t natoms_per_mol_v;   /* solvent.natomspermol */
+    int32_t autocorrelation;    /* Result.autocorrelation (solute == solvent) */
+    int32_t irefatom;           /* R.files[1].irefatom, 1-based */
+    int32_t nbins;              /* setbin(cutoff, binstep) */
+    int32_t usecutoff;          /* Options.usecutoff */
+    double binstep;             /* Options.binstep */
+    double cutoff;              /* Options.cutoff */
+    double dbulk;               /* Options.dbulk */
+    int32_t n_random_samples;   /* Options.n_random_samples */
+    int32_t coordination_number_only;
+    int32_t triclinic;          /* unit cell type frozen from frame 1 (convert_unitcell) */
+    int32_t lcell;              /* Options.lcell (>= 1); kept for the record, cells are sized on device */
+    uint64_t seed;              /* Options.seed (> 0) */
+    int32_t n_groups_u;         /* number of solute groups (natomspermol without custom groups) */
+    int32_t n_groups_v;         /* number of solvent groups */
+    /* Custom groups (update_group_count!, src/update_counters.jl:22-36) as CSR
+     * maps from atoms to group ids (0-based).  NULL means one group per atom
+     * type.  Solute side: over the natoms_per_mol_u molecule-local atoms (the
+     * reference looks up solute.indices[md.i] with a molecule-local i).  Solvent
+     * side: over all nmols_v*natoms_per_mol_v solvent atoms; in
+     * autocorrelation mode it maps into the solute groups. */
+    const int32_t *groups_u_ptr, *groups_u_idx;
+    const int32_t *groups_v_ptr, *groups_v_idx;
+    int64_t max_batch_frames;   /* frames per device batch; 0 = sized from the memory budget */
+    int64_t memory_budget;      /* bytes of device scratch per job; 0 = library default (4 GiB) */
+} cm_job_desc;
+
+/* Raw accumulators, the field set of sum! (src/results.jl:629-649) before
+ * finalresults!.  All arrays are caller-allocated float64. */
+typedef struct cm_raw {
+    double *md_count, *md_count_random;            /* [nbins] */
+    double *rdf_count, *rdf_count_random;          /* [nbins] */
+    double *solute_group_count, *solvent_group_count;                  /* [n_groups][nbins] */
+    double *solute_group_count_random, *solvent_group_count_random;    /* [n_groups][nbins] */
+    double volume_total;                           /* sum of w * det(cell) */
+    /* Optional integer hit counts (NULL to skip), [nbins] each.  In
+     * autocorrelation mode group counts are accumulated in half units. */
+    uint64_t *hits_md, *hits_md_random, *hits_rdf, *hits_rdf_random;
+} cm_raw;
+
+typedef struct cm_kernel_stats {
+    double eval_ms;             /* total time of the minimum-distance kernels */
+    int64_t eval_launches;
+    double total_ms;            /* total time of all kernels of the job */
+    int64_t total_launches;
+    int64_t frames;             /* frames processed */
+    int64_t candidate_pairs;    /* FP32 pair evaluations (when counting is on) */
+    int64_t fp64_rechecks;      /* molecules resolved with the exhaustive FP64 pass */
+} cm_kernel_stats;
+
+/* Creates a job: validates the description, copies the group maps, allocates
+ * device accumulators.  Replaces Result(trajectory, options) +
+ * build_particle_system (src/results.jl:124-193, src/minimum_distances.jl:157-176)
+ * for the device side. */
+int cm_job_create(const cm_job_desc *desc, cm_job **job);
+
+/* Accumulates frames from host memory.  frame_index holds the 1-based
+ * trajectory frame numbers (they key the random streams, so results do not
+ * depend on how frames are batched); weights are the frame weights (frames of
+ * weight 0 must not be passed, as in goto_nextframe!, src/mddf.jl:95-111).
+ * Replaces the per-frame calls of mddf_frame! / coordination_number_frame!
+ * (src/mddf.jl:329-331). */
+int cm_job_submit(cm_job *job, int32_t nframes, const int64_t *frame_index, const void *x_solute,
+                  const void *x_solvent, int32_t dtype, const double *cells, const double *weights);
+
+/* As cm_job_submit, with coordinates already resident in device memory
+ * (pointers from cm_device_alloc); cells and weights stay host arrays. */
+int cm_job_submit_device(cm_job *job, int32_t nframes, const int64_t *frame_index, const void *d_x_solute,
+                         const void *d_x_solvent, int32_t dtype, const double *cells, const double *weights);
+
+/* Waits for the job's work and writes the raw accumulators (the chunk sum!
+ * of src/mddf.jl:336).  The job may not be submitted to afterwards. */
+int cm_job_finish(cm_job *job, cm_raw *out);
+
+/* Kernel timing of the job (HIP events on the job's stream). */
+int cm_job_set_profiling(cm_job *job, int32_t on);
+int cm_job_kernel_stats(cm_job *job, cm_kernel_stats *out);
+
+void cm_job_destroy(cm_job *job);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *cm_last_error(void);
+
+/* Device memory for callers that keep trajectories resident in HBM. */
+int cm_device_alloc(size_t bytes, void **dptr);
+int cm_device_upload(void *dptr, const void *host, size_t bytes);
+int cm_device_free(void *dptr);
+int cm_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CMGPU_H */

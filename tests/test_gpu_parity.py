@@ -1,0 +1,303 @@
+"""GPU parity: libcmgpu.so against the FP64 oracle and the reference's known answers.
+
+Every test runs the HIP library through the C ABI (via the package) and
+compares raw accumulators with the oracle bit for bit (integer hit counts
+times dyadic weights are exact), or checks the closed-form answers of the
+reference's own tests.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, assert_raw_equal, gpu_raw, oracle_raw
+
+pytestmark = pytest.mark.gpu
+
+TOY = os.path.join(GOLDEN, "toy")
+
+
+def mem(pkg, s):
+    return pkg.MemoryTrajectory(s["x_solute"], s["x_solvent"], s["cells"], s["solute"], s["solvent"])
+
+
+def syn():
+    import complexmixtures_amd.synthetic as S
+    return S
+
+
+# ---------------------------------------------------------------------------
+# known answers of the reference's tests
+# ---------------------------------------------------------------------------
+def test_toy_cross_known_answers(pkg, oracle):
+    """src/mddf.jl:587-624"""
+    protein = pkg.AtomSelection([10], nmols=1)
+    water = pkg.AtomSelection(list(range(1, 10)), natomspermol=3)
+    f = os.path.join(TOY, "cross.pdb")
+    for lastframe in (1, 2):
+        opts = pkg.Options(seed=321, StableRNG=True, nthreads=1, silent=True, n_random_samples=10 ** 5,
+                           lastframe=lastframe)
+        R = pkg.mddf(f, protein, water, opts, trajectory_format="PDBTraj")
+        assert R.volume.total == 27000.0
+        assert math.isclose(R.volume.domain, R.volume.total - R.volume.bulk)
+        assert math.isclose(R.volume.domain, (4 * math.pi / 3) * R.dbulk ** 3, rel_tol=0.01)
+        assert math.isclose(R.density.solute, 1 / R.volume.total)
+        assert math.isclose(R.density.solvent, 3 / R.volume.total)
+        assert math.isclose(R.density.solvent_bulk, 2 / R.volume.bulk)
+        assert math.isclose(R.md_count.sum(), 1)
+        assert math.isclose(R.coordination_number.sum(), 51)
+        C = pkg.coordination_number(f, protein, water, opts, trajectory_format="PDBTraj")
+        assert C.volume.total == R.volume.total
+        assert C.volume.domain == 0.0
+        assert math.isclose(C.density.solute, 1 / C.volume.total)
+        assert math.isclose(C.density.solvent, 3 / C.volume.total)
+        assert C.density.solvent_bulk == 0.0
+        assert np.array_equal(C.md_count, R.md_count)
+        assert np.array_equal(pkg.coordination_number(C), pkg.coordination_number(R))
+        traj = pkg.Trajectory(f, protein, water, format="PDBTraj")
+        a, _ = oracle_raw(pkg, oracle, traj, opts)
+        b = gpu_raw(pkg, pkg.Trajectory(f, protein, water, format="PDBTraj"), opts)
+        assert_raw_equal(a, b)
+
+
+def test_toy_self_known_answers(pkg, oracle):
+    """src/mddf.jl:629-654 (self correlation, atoms 5 A apart)"""
+    atom = pkg.AtomSelection([1, 2], natomspermol=1)
+    f = os.path.join(TOY, "self_monoatomic.pdb")
+    opts = pkg.Options(seed=321, StableRNG=True, nthreads=1, silent=True, n_random_samples=10 ** 5, lastframe=1)
+    R = pkg.mddf(f, atom, opts, trajectory_format="PDBTraj")
+    assert R.volume.total == 27000.0
+    assert math.isclose(R.volume.domain, R.volume.total - R.volume.bulk)
+    assert math.isclose(R.volume.domain, (4 * math.pi / 3) * R.dbulk ** 3, rel_tol=0.1)
+    assert math.isclose(R.density.solute, 2 / R.volume.total)
+    assert math.isclose(R.density.solvent, 2 / R.volume.total)
+    assert math.isclose(R.md_count.sum(), 1.0)
+    C = pkg.coordination_number(f, atom, opts, trajectory_format="PDBTraj")
+    assert math.isclose(C.md_count.sum(), 1.0)
+    a, _ = oracle_raw(pkg, oracle, pkg.Trajectory(f, atom, format="PDBTraj"), opts)
+    b = gpu_raw(pkg, pkg.Trajectory(f, atom, format="PDBTraj"), opts)
+    assert_raw_equal(a, b)
+
+
+def test_toy_frame_weights(pkg):
+    """src/mddf.jl:666-757"""
+    atom = pkg.AtomSelection([1, 2], natomspermol=1)
+    f = os.path.join(TOY, "self_monoatomic.pdb")
+    fd = os.path.join(TOY, "self_monoatomic_duplicated_first_frame.pdb")
+    kw = dict(trajectory_format="PDBTraj")
+    R1 = pkg.mddf(f, atom, pkg.Options(seed=321, silent=True, firstframe=2), **kw)
+    R2 = pkg.mddf(f, atom, pkg.Options(seed=321, silent=True), frame_weights=[0.0, 1.0], **kw)
+    assert np.array_equal(R1.md_count, R2.md_count) and np.array_equal(R1.rdf_count, R2.rdf_count)
+    R2 = pkg.mddf(f, atom, pkg.Options(silent=True), frame_weights=np.array([[0, 1]]), **kw)
+    assert np.array_equal(R1.md_count, R2.md_count)
+    with pytest.raises(pkg.ArgumentError):
+        pkg.mddf(f, atom, pkg.Options(silent=True), frame_weights=np.array([[0, 1], [0, 1]]), **kw)
+    R1 = pkg.mddf(f, atom, pkg.Options(seed=321, silent=True, lastframe=1), **kw)
+    R2 = pkg.mddf(f, atom, pkg.Options(seed=321, silent=True, lastframe=1), frame_weights=[1.0, 0.0], **kw)
+    assert np.array_equal(R1.md_count, R2.md_count) and np.array_equal(R1.rdf_count, R2.rdf_count)
+    R1 = pkg.mddf(f, atom, pkg.Options(seed=321, silent=True), **kw)
+    R2 = pkg.mddf(f, atom, pkg.Options(seed=321, silent=True), frame_weights=[0.3, 0.3], **kw)
+    assert np.array_equal(R1.md_count, R2.md_count) and np.array_equal(R1.rdf_count, R2.rdf_count)
+    R1 = pkg.mddf(fd, atom, pkg.Options(seed=321, silent=True, firstframe=1, lastframe=3), **kw)
+    R2 = pkg.mddf(f, atom, pkg.Options(seed=321, silent=True), frame_weights=[2.0, 1.0], **kw)
+    assert np.array_equal(R1.md_count, R2.md_count) and np.array_equal(R1.rdf_count, R2.rdf_count)
+    R2 = pkg.mddf(f, atom, pkg.Options(silent=True), frame_weights=[2.0, 1.0], **kw)
+    assert math.isclose(R2.md_count.sum(), 2 / 3)
+    R2 = pkg.mddf(f, atom, pkg.Options(silent=True), frame_weights=[1.0, 2.0], **kw)
+    assert math.isclose(R2.md_count.sum(), 1 / 3)
+    # only with atoms in the bulk, then both frames
+    for firstframe, dens in ((2, 1.0), (1, 0.5)):
+        R = pkg.mddf(f, atom, pkg.Options(seed=321, silent=True, n_random_samples=10 ** 5,
+                                                           firstframe=firstframe), **kw)
+        assert R.volume.total == 27000.0
+        assert math.isclose(R.volume.domain, R.volume.total - R.volume.bulk)
+        assert math.isclose(R.volume.domain, (4 * math.pi / 3) * R.dbulk ** 3, rel_tol=0.1)
+        assert math.isclose(R.density.solvent_bulk, dens / R.volume.bulk)
+
+
+def test_stop_file_interrupt(pkg, tmp_path, monkeypatch):
+    """src/mddf.jl:656-664"""
+    atom = pkg.AtomSelection([1, 2], natomspermol=1)
+    f = os.path.join(TOY, "self_monoatomic.pdb")
+    monkeypatch.chdir(tmp_path)
+    (tmp_path / "stop_complexmixtures").write_text("")
+    R = pkg.mddf(f, atom, pkg.Options(silent=True), trajectory_format="PDBTraj")
+    assert R.md_count.sum() == 0.0
+
+
+def _dupdcd(pkg):
+    z = np.load(os.path.join(GOLDEN, "dupdcd_atoms_1_4012.npz"))
+    xyz = z["xyz"]
+    A, B, C = z["dcd_cell"][0][[0, 2, 5]]
+    cells = np.repeat(pkg.unitcell_matrix(A, B, C)[None], xyz.shape[0], axis=0)
+    protein = pkg.AtomSelection(list(range(1, 1464)), nmols=1)
+    tmao = pkg.AtomSelection(list(range(1479, 4013)), natomspermol=14,
+                             group_names=["N", "C1", "C2", "C3", "O1", "H11", "H12", "H13", "H21", "H22", "H23",
+                                          "H31", "H32", "H33"])
+    return xyz, cells, protein, tmao
+
+
+def test_dupdcd_known_answers(pkg, oracle):
+    """src/tools/coordination_number.jl:108-135 on frame 1 of the NAMD test trajectory."""
+    xyz, cells, protein, tmao = _dupdcd(pkg)
+    xu = xyz[:, 0:1463].astype(np.float64)
+    xv = xyz[:, 1478:4012].astype(np.float64)
+    traj = pkg.MemoryTrajectory(xu, xv, cells, protein, tmao)
+    opts = pkg.Options(lastframe=1, seed=321, StableRNG=True, nthreads=1, silent=True, n_random_samples=200)
+    R = pkg.mddf(traj, opts)
+    assert R.files[0].irefatom == 1
+    assert math.isclose(R.solute_group_count.sum(), R.solvent_group_count.sum())
+    assert R.coordination_number[np.argmax(R.d > 3)] == 7.0
+    assert R.coordination_number[np.argmax(R.d > 5)] == 14.0
+    assert np.cumsum(R.solvent_group_count[tmao.group_names.index("O1")]).sum() == 1171.0
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts)
+    assert_raw_equal(a, b)
+
+
+def test_dupdcd_self_tmao_and_weights(pkg, oracle):
+    """Self MDDF of TMAO (mols family) over the 3 frames; weights [2, 1] on frames
+    (1, 3) equal the duplicated first frame (src/mddf.jl:860-879)."""
+    xyz, cells, protein, tmao = _dupdcd(pkg)
+    xv = xyz[:, 1478:4012].astype(np.float64)
+    traj = pkg.MemoryTrajectory(xv, xv, cells, tmao, tmao)
+    opts = pkg.Options(seed=321, silent=True, n_random_samples=4)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts)
+    assert_raw_equal(a, b)
+    R1 = pkg.mddf(traj, pkg.Options(silent=True, n_random_samples=4))
+    R2 = pkg.mddf(traj, pkg.Options(silent=True, n_random_samples=4), frame_weights=[2.0, 0.0, 1.0])
+    assert np.array_equal(R1.md_count, R2.md_count)
+    assert np.array_equal(R1.solute_group_count, R2.solute_group_count)
+    assert np.array_equal(R1.solvent_group_count, R2.solvent_group_count)
+    assert R1.volume.total == R2.volume.total
+
+
+# ---------------------------------------------------------------------------
+# synthetic systems against the oracle
+# ---------------------------------------------------------------------------
+def test_parity_config1_reduced(pkg, oracle):
+    s = syn().config1(nframes=4)
+    traj = mem(pkg, s)
+    opts = pkg.Options(dbulk=10.0, binstep=0.02, n_random_samples=10, seed=20261203, silent=True)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts)
+    assert_raw_equal(a, b)
+    assert b["hits_md"].sum() == round(a["md_count"].sum())
+
+
+def test_parity_float32_input(pkg, oracle):
+    s = syn().config1(nframes=3, n_solute=300, n_water=900, box=30.0, seed=5)
+    s["x_solute"] = s["x_solute"].astype(np.float32)
+    s["x_solvent"] = s["x_solvent"].astype(np.float32)
+    traj = mem(pkg, s)
+    opts = pkg.Options(dbulk=8.0, n_random_samples=6, seed=77, silent=True)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts)
+    assert_raw_equal(a, b)
+
+
+def test_parity_triclinic_urea(pkg, oracle):
+    s = syn().config3(nframes=2, n_solute=400, n_urea=800, n_water=10, seed=3, a=40.0)
+    traj = mem(pkg, s)
+    opts = pkg.Options(dbulk=10.0, n_random_samples=5, seed=9, silent=True)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts)
+    assert_raw_equal(a, b)
+
+
+def test_parity_self_urea_triclinic(pkg, oracle):
+    s = syn().config3(nframes=2, n_solute=10, n_urea=600, n_water=10, solvent="self", seed=4, a=36.0)
+    traj = mem(pkg, s)
+    opts = pkg.Options(dbulk=9.0, n_random_samples=3, seed=10, silent=True)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts)
+    assert_raw_equal(a, b)
+
+
+def test_parity_cross_many_solute_molecules(pkg, oracle):
+    S = syn()
+    rng = np.random.default_rng(12)
+    box = 28.0
+    wt = S.water_template()
+    xu = S.place_molecules(rng, wt, rng.uniform(0, box, (40, 3)))
+    xv = S.place_molecules(rng, wt, rng.uniform(0, box, (700, 3)))
+    xu = np.stack([xu + rng.normal(scale=0.3, size=xu.shape) for _ in range(3)])
+    xv = np.stack([xv + rng.normal(scale=0.3, size=xv.shape) for _ in range(3)])
+    su = pkg.AtomSelection(list(range(1, 121)), natomspermol=3)
+    sv = pkg.AtomSelection(list(range(121, 121 + 2100)), natomspermol=3)
+    traj = pkg.MemoryTrajectory(xu, xv, np.repeat(np.diag([box] * 3)[None], 3, axis=0), su, sv)
+    for opts in (pkg.Options(dbulk=9.0, n_random_samples=7, seed=3, silent=True),
+                 pkg.Options(bulk_range=(6.0, 9.0), n_random_samples=7, seed=3, silent=True)):
+        a, _ = oracle_raw(pkg, oracle, traj, opts)
+        b = gpu_raw(pkg, traj, opts)
+        assert_raw_equal(a, b)
+
+
+def test_parity_custom_groups_and_usecutoff(pkg, oracle):
+    s = syn().config1(nframes=2, n_solute=150, n_water=500, box=26.0, seed=21)
+    n_at = 150
+    g1 = list(range(1, 60))
+    g2 = list(range(40, 151))          # overlaps g1: atoms count in both groups
+    g3 = [3, 77, 149]
+    su = pkg.AtomSelection(list(range(1, n_at + 1)), nmols=1, group_atom_indices=[g1, g2, g3],
+                           group_names=["a", "b", "c"])
+    ov = list(range(n_at + 1, n_at + 1501, 3))
+    hv = [i for i in range(n_at + 1, n_at + 1501) if i not in set(ov)]
+    sv = pkg.AtomSelection(list(range(n_at + 1, n_at + 1501)), natomspermol=3,
+                           group_atom_indices=[ov, hv[:500]], group_names=["O", "Hsome"])
+    traj = pkg.MemoryTrajectory(s["x_solute"], s["x_solvent"], s["cells"], su, sv)
+    for opts in (pkg.Options(bulk_range=(6.0, 9.0), n_random_samples=5, seed=4, silent=True),
+                 pkg.Options(dbulk=9.0, n_random_samples=5, seed=4, silent=True, irefatom=2)):
+        a, _ = oracle_raw(pkg, oracle, traj, opts)
+        b = gpu_raw(pkg, traj, opts)
+        assert_raw_equal(a, b)
+
+
+def test_parity_weights_stride_coordination(pkg, oracle):
+    s = syn().config1(nframes=6, n_solute=200, n_water=600, box=26.0, seed=8)
+    traj = mem(pkg, s)
+    w = [2.0, 0.5, 0.0, 1.0, 0.25, 4.0]
+    opts = pkg.Options(dbulk=8.0, n_random_samples=4, seed=15, silent=True, firstframe=1, stride=1)
+    a, _ = oracle_raw(pkg, oracle, traj, opts, frame_weights=w)
+    b = gpu_raw(pkg, traj, opts, frame_weights=w)
+    assert_raw_equal(a, b)                     # dyadic weights: exact
+    w2 = [0.3, 0.7, 0.1, 0.9, 0.3, 0.3]
+    a, _ = oracle_raw(pkg, oracle, traj, opts, frame_weights=w2)
+    b = gpu_raw(pkg, traj, opts, frame_weights=w2)
+    assert_raw_equal(a, b, rtol=1e-13)         # integer hits times the weight vs sequential sums
+    opts = pkg.Options(dbulk=8.0, n_random_samples=4, seed=15, silent=True, firstframe=2, lastframe=6, stride=2)
+    for co in (False, True):
+        a, _ = oracle_raw(pkg, oracle, traj, opts, coordination_number_only=co)
+        b = gpu_raw(pkg, traj, opts, coordination_number_only=co)
+        assert_raw_equal(a, b)
+
+
+def test_batching_does_not_change_results(pkg):
+    """Results depend on the frame numbers only, not on how frames are batched."""
+    s = syn().config1(nframes=5, n_solute=200, n_water=600, box=26.0, seed=8)
+    traj = mem(pkg, s)
+    opts = pkg.Options(dbulk=8.0, n_random_samples=4, seed=15, silent=True)
+    a = gpu_raw(pkg, traj, opts)
+    b = gpu_raw(pkg, traj, opts, batch_frames=2)
+    c = gpu_raw(pkg, traj, opts, job_kw={"max_batch_frames": 1})
+    assert_raw_equal(a, b)
+    assert_raw_equal(a, c)
+
+
+def test_exact_ties(pkg, oracle):
+    """Equidistant atoms (exact FP ties) resolve like the oracle: smallest (i, j)."""
+    # solute: two atoms symmetric about the x axis; solvent: waters on the axis
+    xu = np.array([[[0.0, 1.0, 0.0], [0.0, -1.0, 0.0]]])
+    wt = np.array([[4.0, 0.0, 0.0], [4.5, 0.5, 0.5], [4.5, -0.5, 0.5],
+                   [0.0, 0.0, 5.0], [0.5, 0.5, 5.5], [-0.5, 0.5, 5.5]])
+    xv = wt[None]
+    su = pkg.AtomSelection([1, 2], nmols=1)
+    sv = pkg.AtomSelection([3, 4, 5, 6, 7, 8], natomspermol=3)
+    traj = pkg.MemoryTrajectory(xu, xv, np.diag([20.0] * 3)[None], su, sv)
+    opts = pkg.Options(dbulk=8.0, n_random_samples=2, seed=1, silent=True)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts)
+    assert_raw_equal(a, b)
+    assert b["solute_group_count"][0].sum() == 2 and b["solute_group_count"][1].sum() == 0
