@@ -129,8 +129,7 @@ def test_stop_file_interrupt(pkg, tmp_path, monkeypatch):
 def _dupdcd(pkg):
     z = np.load(os.path.join(GOLDEN, "dupdcd_atoms_1_4012.npz"))
     xyz = z["xyz"]
-    A, B, C = z["dcd_cell"][0][[0, 2, 5]]
-    cells = np.repeat(pkg.unitcell_matrix(A, B, C)[None], xyz.shape[0], axis=0)
+    cells = np.stack([pkg.unitcell_matrix(*c[[0, 2, 5]]) for c in z["dcd_cell"]])
     protein = pkg.AtomSelection(list(range(1, 1464)), nmols=1)
     tmao = pkg.AtomSelection(list(range(1479, 4013)), natomspermol=14,
                              group_names=["N", "C1", "C2", "C3", "O1", "H11", "H12", "H13", "H21", "H22", "H23",
