@@ -55,7 +55,7 @@ class Volume:
 
 def _seqsum(a):
     a = np.asarray(a, dtype=np.float64)
-    return float(np.cumsum(a)[-1]) if a.size else 0.0
+    return np.float64(np.cumsum(a)[-1]) if a.size else np.float64(0.0)
 
 
 class Result:
@@ -132,7 +132,7 @@ def set_samples(R):
 
 def sum_frame_weights(R):
     """src/results.jl:288-297"""
-    Q = 0.0
+    Q = np.float64(0.0)
     for file in R.files:
         o = file.options
         for i in range(o.firstframe, file.lastframe_read + 1, o.stride):
@@ -141,10 +141,13 @@ def sum_frame_weights(R):
 
 
 def finalresults(R, options, coordination_number_only):
-    """finalresults!, src/results.jl:311-318"""
-    if not coordination_number_only:
-        return _mddf_final_results(R, options)
-    return _coordination_number_final_results(R, options)
+    """finalresults!, src/results.jl:311-318.  Scalars are numpy float64, so that
+    divisions by zero give Inf/NaN as Julia's Float64 arithmetic does."""
+    R.volume.total = np.float64(R.volume.total)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if not coordination_number_only:
+            return _mddf_final_results(R, options)
+        return _coordination_number_final_results(R, options)
 
 
 def _mddf_final_results(R, options):
@@ -197,7 +200,7 @@ def renormalize(R, density_fix, silent=True):
         warnings.warn("\nIdeal-gas histogram bins with zero samples. \n"
                       "Increase n_random_samples, number of trajectory frames, and/or bin size.\n")
     R.mddf[pos] = R.md_count[pos] / R.md_count_random[pos]
-    inv = 1 / R.density.solvent_bulk
+    inv = np.float64(1) / R.density.solvent_bulk
     R.kb[:] = Angs3tocm3permol * inv * (R.coordination_number - R.coordination_number_random)
     posr = R.rdf_count_random > 0.0
     R.rdf[posr] = R.rdf_count[posr] / R.rdf_count_random[posr]
