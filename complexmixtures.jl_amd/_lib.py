@@ -46,7 +46,7 @@ class KernelStats(ctypes.Structure):
 
 
 EXPORTS = ["cm_job_create", "cm_job_submit", "cm_job_submit_device", "cm_job_finish", "cm_job_set_profiling",
-           "cm_job_kernel_stats", "cm_job_destroy", "cm_last_error", "cm_device_alloc", "cm_device_upload",
+           "cm_job_set_tuning", "cm_job_kernel_stats", "cm_job_destroy", "cm_last_error", "cm_device_alloc", "cm_device_upload",
            "cm_device_free", "cm_device_synchronize"]
 
 _lib = None
@@ -69,6 +69,7 @@ def lib():
         L.cm_job_submit_device.argtypes = L.cm_job_submit.argtypes
         L.cm_job_finish.argtypes = [ctypes.c_void_p, _P(RawOut)]
         L.cm_job_set_profiling.argtypes = [ctypes.c_void_p, _i32]
+        L.cm_job_set_tuning.argtypes = [ctypes.c_void_p, ctypes.c_double, _i32, _i32]
         L.cm_job_kernel_stats.argtypes = [ctypes.c_void_p, _P(KernelStats)]
         L.cm_job_destroy.argtypes = [ctypes.c_void_p]
         L.cm_job_destroy.restype = None
@@ -77,7 +78,7 @@ def lib():
         L.cm_device_upload.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         L.cm_device_free.argtypes = [ctypes.c_void_p]
         for f in ("cm_job_create", "cm_job_submit", "cm_job_submit_device", "cm_job_finish", "cm_job_set_profiling",
-                  "cm_job_kernel_stats", "cm_device_alloc", "cm_device_upload", "cm_device_free",
+                  "cm_job_set_tuning", "cm_job_kernel_stats", "cm_device_alloc", "cm_device_upload", "cm_device_free",
                   "cm_device_synchronize"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
@@ -130,7 +131,7 @@ class Job:
     def __init__(self, *, nmols_u, natoms_per_mol_u, nmols_v, natoms_per_mol_v, autocorrelation, irefatom, nbins,
                  binstep, cutoff, dbulk, usecutoff, n_random_samples, coordination_number_only, triclinic, seed,
                  n_groups_u, n_groups_v, groups_u=(None, None), groups_v=(None, None), lcell=1, max_batch_frames=0,
-                 memory_budget=0):
+                 memory_budget=0, tuning=None):
         self._keep = [g for g in (*groups_u, *groups_v) if g is not None]
         nul = _P(_i32)()
 
@@ -146,6 +147,8 @@ class Job:
                     int(max_batch_frames), int(memory_budget))
         self.h = ctypes.c_void_p()
         check(lib().cm_job_create(ctypes.byref(d), ctypes.byref(self.h)))
+        if tuning:
+            self.set_tuning(**tuning)
 
     def submit(self, frame_index, x_solute, x_solvent, cells, weights):
         """Host arrays: frame_index [F] (1-based), x_* [F][N][3] float64/float32,
@@ -192,6 +195,9 @@ class Job:
 
     def set_profiling(self, on=True):
         check(lib().cm_job_set_profiling(self.h, int(on)))
+
+    def set_tuning(self, cell_fraction=0.5, columns=1, eval_workgroups=0):
+        check(lib().cm_job_set_tuning(self.h, float(cell_fraction), int(columns), int(eval_workgroups)))
 
     def stats(self):
         s = KernelStats()

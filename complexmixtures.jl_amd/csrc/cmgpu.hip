@@ -1,31 +1,41 @@
 // cmgpu.hip -- MI355X (gfx950) implementation of the MDDF accumulation of
 // ComplexMixtures.jl behind the C ABI of include/cmgpu.h.
 //
-// Per batch of frames, on the job's stream:
-//   k_radii      molecule radii about their centres (cell sizing, guard band)
-//   k_grid       per-frame cell grid and FP32 thresholds
-//   k_wrap_atoms / k_prep_solute_mols, k_scan_cells, k_scatter
-//                device cell list: counting sort by cell + exclusive scan
-//   k_prep_solvent  solvent molecules made whole, in the frame of their cell
-//   k_eval_atoms / k_eval_mols  (real pass) minimum distance per solvent
-//                molecule in FP32, exact FP64 resolve, LDS histograms
-//   k_bulk_compact  ordered list of bulk molecules (inbulk, src/mddf.jl:55-57)
-//   k_place      Philox random rigid-body placement (src/rigid_body.jl:107-137)
-//   k_eval_*     (random passes) the same kernels on the placed molecules
-//   k_flush      integer hits -> FP64 accumulators times the frame weight
+// Two kernel families, chosen per job:
 //
-// Two kernel families: "atoms" for a single solute molecule of any size (its
-// atoms in a cell list, one thread per solvent molecule scanning the 27
-// neighbour cells of the molecule's cell) and "mols" for many solute
-// molecules (molecule centres in a cell list, all atom pairs of close
-// molecule pairs).
+// "tiles" -- one solute molecule of any size (proteins, polymers), solvent
+// molecules of up to 64 atoms.  Per batch of frames, on the job's stream:
+//   k_radii             solvent molecule radius about its centre
+//   k_grid              per-frame FP32 thresholds (guard band)
+//   k_wrap_atoms, k_scan_cells, k_scatter
+//                       solute atoms in a fine cell grid (counting sort + scan)
+//   k_near_mark         cells within cutoff + radius of some solute atom
+//   k_prep_solvent_tiles  solvent molecules made whole, kept when near
+//   k_scan_rows, k_mol_scatter, k_chunk_table
+//                       near molecules sorted by cell per (frame, pass), cut
+//                       into chunks of whole molecules, one wavefront each
+//   k_eval_tiles        per chunk: AABB, solute atoms within the cutoff of it
+//                       staged in LDS, FP32 minimum per atom (LDS broadcast),
+//                       segmented minimum per molecule, exact FP64 resolve,
+//                       LDS histograms
+//   k_bulk_compact      ordered list of bulk molecules (inbulk, src/mddf.jl:55-57)
+//   k_place_tiles       Philox placement (src/rigid_body.jl:107-137) of the
+//                       slots whose new centre is near the solute
+//   then sort, chunk and k_eval_tiles again for the random passes.
+//
+// "mols" -- many solute molecules (self MDDF, small solutes): molecule
+// centres in a cell list, one thread per solvent molecule evaluating all atom
+// pairs of close molecule pairs (k_prep_solute_mols, k_prep_solvent,
+// k_eval_mols_real, k_place, k_eval_mols_rand).
+//
+// k_flush turns integer hits into FP64 accumulators times the frame weight.
 //
 // Exactness.  Candidate pairs are evaluated in FP32 from coordinates in the
-// frame of each molecule's cell.  Per solvent molecule the kernel keeps the
-// best and second-best squared distance; when the gap exceeds a guard band
-// that bounds the FP32 error, the FP32 argmin is the FP64 argmin and only that
-// pair is recomputed in FP64 (the recorded distance).  Otherwise every pair
-// of the molecule is recomputed in FP64 (exhaustive resolve).  Records,
+// frame of each molecule's cell.  Per atom (tiles) or molecule pair (mols) the
+// kernels keep the best and second-best squared distance; when the gap exceeds
+// a guard band that bounds the FP32 error, the FP32 argmin is the FP64 argmin
+// and only that pair is recomputed in FP64 (the recorded distance).
+// Otherwise every pair near the minimum is recomputed in FP64.  Records,
 // indices and bins therefore equal those of an FP64 evaluation.
 #include <hip/hip_runtime.h>
 
@@ -61,6 +71,9 @@ struct Geo {                 // one frame of a batch
     float rpre2;             // molecule-centre pre-check radius^2 (mols family)
     unsigned ru_bits, rv_bits;   // molecule radii (float bits, atomicMax)
     int err;                 // geometry the kernels cannot handle
+    float rstage;            // staging radius around a chunk (cutoff + margin), tiles family
+    float rnear;             // near-cell radius around a solute atom (cutoff + radius + margin)
+    int kbits;               // bits per dimension of the Morton keys of the fine cells
 };
 
 struct KP {                  // job parameters, by value
@@ -74,6 +87,9 @@ struct KP {                  // job parameters, by value
     int cap;                 // cell slots per frame
     int F;                   // frames in the batch
     int64_t Nu, Nv;
+    int mpt;                 // molecules per chunk (tiles family)
+    int NP;                  // passes per frame, 1 + K (tiles family)
+    int kcap;                // molecule sort keys per (frame, pass): Morton codes of the fine cells
 };
 
 struct Bufs {
@@ -103,6 +119,14 @@ struct Bufs {
     int* uk;                 // [F][K] drawn solute molecule of each random pass
     int* drawn_ptr;          // [F][Mu+1] CSR: solute molecule -> random passes
     int* drawn_idx;          // [F][K]
+    // tiles family: near cells and molecules sorted by cell per (frame, pass)
+    unsigned char* near;     // [F][cap]
+    int* mkey;               // [F][NP][Mv] Morton key of the centre's cell, -1 when far
+    int* mcnt;               // [F][NP][kcap+1]
+    int* mstart;             // [F][NP][kcap+1]
+    int* msorted;            // [F][NP][Mv]
+    int* nnear;              // [F][NP]
+    int* chunk_base;         // [F*npass + 1] chunk offsets of one launch
     // accumulators
     ull* cnt_md; ull* cnt_rdf; ull* cnt_md_r; ull* cnt_rdf_r;
     ull* cnt_gu; ull* cnt_gv; ull* cnt_gu_r; ull* cnt_gv_r;
@@ -227,21 +251,23 @@ __global__ void k_grid(KP P, Bufs B, int mols_family)
     Geo& g = B.geo[f];
     double ru = mols_family ? (double)__uint_as_float(g.ru_bits) : 0.0;
     double rv = (double)__uint_as_float(g.rv_bits);
-    double side = P.cutsys + ru + rv + 1e-3 + 1e-7 * (g.sp[0] + g.sp[1] + g.sp[2]);
-    int ncell = 1;
-    for (int k = 0; k < 3; k++) {
-        int n = (int)floor(g.sp[k] / side);
-        g.nc[k] = n < 1 ? 1 : n;
-        ncell *= g.nc[k];
-        if (mols_family && !(side < g.sp[k])) g.err = 2;   // image enumeration assumes ru+rv+cutoff < spacing
+    if (mols_family) {
+        double side = P.cutsys + ru + rv + 1e-3 + 1e-7 * (g.sp[0] + g.sp[1] + g.sp[2]);
+        int ncell = 1;
+        for (int k = 0; k < 3; k++) {
+            int n = (int)floor(g.sp[k] / side);
+            g.nc[k] = n < 1 ? 1 : n;
+            ncell *= g.nc[k];
+            if (!(side < g.sp[k])) g.err = 2;   // image enumeration assumes ru+rv+cutoff < spacing
+        }
+        if (ncell > P.cap) g.err = 3;
+        g.ncell = ncell;
     }
-    if (ncell > P.cap) g.err = 3;
-    g.ncell = ncell;
     const double* H = g.cell.H;
     double xm = 0.0;
     for (int r = 0; r < 3; r++) xm = fmax(xm, fabs(H[r]) + fabs(H[3 + r]) + fabs(H[6 + r]));
     xm += fmax(ru, rv) + P.cutsys + 1.0;
-    double E = 6.0 * halfulp32(2.0 * xm) + 1e-7;
+    double E = 10.0 * halfulp32(2.0 * xm) + 1e-7;
     double dmax = P.cutsys + 0.5;
     double G = E * (2.0 * 1.7320508075688772 * dmax + 3.0 * E) + 8.0 * 0x1p-24 * dmax * dmax;
     float hi = (float)(P.cut2 + G);
@@ -250,6 +276,8 @@ __global__ void k_grid(KP P, Bufs B, int mols_family)
     g.amb = nextafterf((float)(2.0 * G) + 8.0f * ulpc, INFINITY);
     float rp = (float)(P.cutsys + ru + rv + 1e-2);
     g.rpre2 = rp * rp;
+    g.rstage = (float)(P.cutsys + 20.0 * E + 1e-3);
+    g.rnear = (float)(P.cutsys + rv + 20.0 * E + 1e-2);
 }
 
 // ---------------------------------------------------------------------------
@@ -467,119 +495,6 @@ __device__ __forceinline__ int nb_cell(int c, int o, int n, int& m)
     int t = c + o;
     m = t < 0 ? -1 : (t >= n ? 1 : 0);
     return t - m * n;
-}
-
-// ---------------------------------------------------------------------------
-// minimum distances, atoms family (one solute molecule, any size)
-// ---------------------------------------------------------------------------
-template <class TU, class TV, bool RANDOM>
-__global__ void __launch_bounds__(BLOCK) k_eval_atoms(KP P, Bufs B, const TU* xu, const TV* xv)
-{
-    extern __shared__ unsigned sh[];
-    hist_clear(sh, 2 * P.nbins);
-    const int K = RANDOM ? P.K : 1;
-    int64_t total = (int64_t)P.F * K * P.Mv;
-    ull* gu = RANDOM ? B.cnt_gu_r : B.cnt_gu;
-    ull* gv = RANDOM ? B.cnt_gv_r : B.cnt_gv;
-    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
-        int64_t fk = it / P.Mv;
-        int v = (int)(it - fk * P.Mv);
-        int f = (int)(fk / K);
-        const Geo& g = B.geo[f];
-        const float4* vw = RANDOM ? B.vw_r + fk * P.Nv : B.vw + (int64_t)f * P.Nv;
-        int vcell = RANDOM ? B.vcell_r[fk * P.Mv + v] : B.vcell[(int64_t)f * P.Mv + v];
-        const float4* srt = B.sorted + (int64_t)f * B.nitems;
-        const int* st = B.start + (int64_t)f * (P.cap + 1);
-        int nc0 = g.nc[0], nc1 = g.nc[1], nc2 = g.nc[2];
-        int c0 = vcell % nc0, c1 = (vcell / nc0) % nc1, c2 = vcell / (nc0 * nc1);
-        Best bm, br;
-        bm.init(); br.init();
-        for (int a = 0; a < P.nv; a++) {
-            float4 y = vw[(int64_t)v * P.nv + a];
-            bool isref = (a == P.iref);
-            for (int oz = -1; oz <= 1; oz++) {
-                int mz; int cz = nb_cell(c2, oz, nc2, mz);
-                for (int oy = -1; oy <= 1; oy++) {
-                    int my; int cy = nb_cell(c1, oy, nc1, my);
-                    for (int ox = -1; ox <= 1; ox++) {
-                        int mx; int cx = nb_cell(c0, ox, nc0, mx);
-                        int cell = (cz * nc1 + cy) * nc0 + cx;
-                        float fx = (float)mx, fy = (float)my, fz = (float)mz;
-                        float tx = g.H32[0] * fx + g.H32[3] * fy + g.H32[6] * fz;
-                        float ty = g.H32[1] * fx + g.H32[4] * fy + g.H32[7] * fz;
-                        float tz = g.H32[2] * fx + g.H32[5] * fy + g.H32[8] * fz;
-                        float yx = y.x - tx, yy = y.y - ty, yz = y.z - tz;
-                        int p1 = st[cell + 1];
-                        for (int p = st[cell]; p < p1; p++) {
-                            float4 s = srt[p];
-                            float dx = yx - s.x, dy = yy - s.y, dz = yz - s.z;
-                            float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                            int i = __float_as_int(s.w);
-                            bm.offer(d2, i, a);
-                            if (isref) br.offer(d2, i, a);
-                        }
-                    }
-                }
-            }
-        }
-        // exact resolve in FP64
-        cm::Cell64 cl = g.cell;
-        const TU* xuf = xu + (int64_t)f * P.Nu * 3;
-        Rec r = rec_zero();
-        bool exhaustive = false;
-        int64_t jbase = (int64_t)v * P.nv;
-        auto ysrc = [&](int a, double& y0, double& y1, double& y2) {
-            if (RANDOM) { const double* pp = B.P + (fk * P.Nv + jbase + a) * 3; y0 = pp[0]; y1 = pp[1]; y2 = pp[2]; }
-            else load3(xv + (int64_t)f * P.Nv * 3, jbase + a, y0, y1, y2);
-        };
-        if (bm.best <= g.cut2_hi) {
-            if (bm.second - bm.best > g.amb) {
-                double x0, x1, x2, y0, y1, y2;
-                load3(xuf, bm.bi, x0, x1, x2);
-                ysrc(bm.ba, y0, y1, y2);
-                double d2 = cm::canon_d2(cl, x0, x1, x2, y0, y1, y2);
-                if (d2 <= P.cut2) { r.within = 1; r.i = bm.bi; r.j = jbase + bm.ba; r.d = sqrt(d2); }
-            } else exhaustive = true;
-        }
-        if (br.best <= g.cut2_hi) {
-            if (br.second - br.best > g.amb) {
-                double x0, x1, x2, y0, y1, y2;
-                load3(xuf, br.bi, x0, x1, x2);
-                ysrc(P.iref, y0, y1, y2);
-                double d2 = cm::canon_d2(cl, x0, x1, x2, y0, y1, y2);
-                if (d2 <= P.cut2) { r.ref = 1; r.dref = sqrt(d2); }
-            } else exhaustive = true;
-        }
-        if (exhaustive) {
-            atomicAdd(&B.stats[0], 1ull);
-            r = rec_zero();
-            for (int a = 0; a < P.nv; a++) {
-                double y0, y1, y2;
-                ysrc(a, y0, y1, y2);
-                for (int oz = -1; oz <= 1; oz++) {
-                    int mz; int cz = nb_cell(c2, oz, nc2, mz);
-                    for (int oy = -1; oy <= 1; oy++) {
-                        int my; int cy = nb_cell(c1, oy, nc1, my);
-                        for (int ox = -1; ox <= 1; ox++) {
-                            int mx; int cx = nb_cell(c0, ox, nc0, mx);
-                            int cell = (cz * nc1 + cy) * nc0 + cx;
-                            for (int p = st[cell]; p < st[cell + 1]; p++) {
-                                int i = __float_as_int(srt[p].w);
-                                double x0, x1, x2;
-                                load3(xuf, i, x0, x1, x2);
-                                double d2 = cm::canon_d2(cl, x0, x1, x2, y0, y1, y2);
-                                if (d2 <= P.cut2) rec_offer(r, i, jbase + a, sqrt(d2), a == P.iref);
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        hist_record(P, sh, gu, gv, r);
-        if (!RANDOM && P.K > 0) B.bflag[(int64_t)f * P.kb * P.Mv + v] = inbulk(P, r) ? 1 : 0;
-    }
-    if (RANDOM) hist_flush(sh, P.nbins, B.cnt_md_r, B.cnt_rdf_r);
-    else hist_flush(sh, P.nbins, B.cnt_md, B.cnt_rdf);
 }
 
 // ---------------------------------------------------------------------------
@@ -861,6 +776,545 @@ __global__ void k_place(KP P, Bufs B, const TV* xv)
     }
 }
 
+// ---------------------------------------------------------------------------
+// tiles family: one solute molecule, solvent molecules of up to 64 atoms
+// ---------------------------------------------------------------------------
+constexpr int TCAP = 1024;   // staged solute atoms per round (16 KiB of LDS)
+
+__device__ __forceinline__ int wrap_index(int i, int n, int& m)
+{
+    int q = i >= 0 ? i / n : -((-i + n - 1) / n);
+    m = q;
+    return i - q * n;
+}
+
+// Morton code of a fine cell: chunks of consecutive molecules in this order
+// stay compact in space
+__device__ __forceinline__ int morton_key(const Geo& g, int cell)
+{
+    int c0 = cell % g.nc[0], c1 = (cell / g.nc[0]) % g.nc[1], c2 = cell / (g.nc[0] * g.nc[1]);
+    unsigned k = 0;
+    for (int b = 0; b < g.kbits; b++) {
+        k |= (((unsigned)c0 >> b) & 1u) << (3 * b);
+        k |= (((unsigned)c1 >> b) & 1u) << (3 * b + 1);
+        k |= (((unsigned)c2 >> b) & 1u) << (3 * b + 2);
+    }
+    return (int)k;
+}
+
+// Fine-grid index ranges covering the box [lo - r, hi + r] (fractional
+// coordinates of its 8 corners, with a small outward margin).
+__device__ __forceinline__ void cell_range(const Geo& g, float lx, float ly, float lz, float hx, float hy, float hz,
+                                           float r, int ilo[3], int ihi[3])
+{
+    const double* Hi = g.cell.Hi;
+    double fmn[3] = {INFINITY, INFINITY, INFINITY}, fmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int c = 0; c < 8; c++) {
+        double x = (c & 1) ? (double)hx + r : (double)lx - r;
+        double y = (c & 2) ? (double)hy + r : (double)ly - r;
+        double z = (c & 4) ? (double)hz + r : (double)lz - r;
+        double f[3] = {Hi[0] * x + Hi[3] * y + Hi[6] * z, Hi[1] * x + Hi[4] * y + Hi[7] * z,
+                       Hi[2] * x + Hi[5] * y + Hi[8] * z};
+        for (int k = 0; k < 3; k++) { fmn[k] = fmin(fmn[k], f[k]); fmx[k] = fmax(fmx[k], f[k]); }
+    }
+    for (int k = 0; k < 3; k++) {
+        ilo[k] = (int)floor(fmn[k] * g.nc[k] - 1e-6);
+        ihi[k] = (int)floor(fmx[k] * g.nc[k] + 1e-6);
+    }
+}
+
+__device__ __forceinline__ void shift_of(const Geo& g, int mx, int my, int mz, float& tx, float& ty, float& tz)
+{
+    float fx = (float)mx, fy = (float)my, fz = (float)mz;
+    tx = g.H32[0] * fx + g.H32[3] * fy + g.H32[6] * fz;
+    ty = g.H32[1] * fx + g.H32[4] * fy + g.H32[7] * fz;
+    tz = g.H32[2] * fx + g.H32[5] * fy + g.H32[8] * fz;
+}
+
+// cells near the solute: every cell within rnear of a solute atom (index box)
+__global__ void k_near_mark(KP P, Bufs B)
+{
+    int64_t total = (int64_t)P.F * P.nu;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int f = (int)(it / P.nu);
+        const Geo& g = B.geo[f];
+        float4 s = B.tmp4[it];
+        int ilo[3], ihi[3];
+        cell_range(g, s.x, s.y, s.z, s.x, s.y, s.z, g.rnear, ilo, ihi);
+        unsigned char* nr = B.near + (int64_t)f * P.cap;
+        for (int iz = ilo[2]; iz <= ihi[2]; iz++) {
+            int mz; int cz = wrap_index(iz, g.nc[2], mz);
+            for (int iy = ilo[1]; iy <= ihi[1]; iy++) {
+                int my; int cy = wrap_index(iy, g.nc[1], my);
+                for (int ix = ilo[0]; ix <= ihi[0]; ix++) {
+                    int mx; int cx = wrap_index(ix, g.nc[0], mx);
+                    nr[(cz * g.nc[1] + cy) * g.nc[0] + cx] = 1;
+                }
+            }
+        }
+    }
+}
+
+// real pass: solvent molecules made whole, centre wrapped into the cell; near
+// ones get a key and their atoms in the frame of that cell
+template <class T>
+__global__ void k_prep_solvent_tiles(KP P, Bufs B, const T* xv)
+{
+    int64_t total = (int64_t)P.F * P.Mv;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int f = (int)(it / P.Mv);
+        int v = (int)(it - (int64_t)f * P.Mv);
+        const Geo& g = B.geo[f];
+        const T* x = xv + (int64_t)f * P.Nv * 3;
+        int64_t a0 = (int64_t)v * P.nv;
+        double c[3];
+        mol_centre(g, x, a0, P.nv, c);
+        double fr[3], n[3];
+        wrap_point(g, c[0], c[1], c[2], fr, n);
+        int cell;
+        cell_of_frac(g, fr[0], fr[1], fr[2], cell);
+        int64_t kb = (int64_t)f * P.NP * P.Mv + v;
+        if (!B.near[(int64_t)f * P.cap + cell]) { B.mkey[kb] = -1; continue; }
+        const int key = morton_key(g, cell);
+        B.mkey[kb] = key;
+        atomicAdd(&B.mcnt[(int64_t)f * P.NP * (P.kcap + 1) + key], 1);
+        const double* H = g.cell.H;
+        double tx = H[0] * n[0] + H[3] * n[1] + H[6] * n[2];
+        double ty = H[1] * n[0] + H[4] * n[1] + H[7] * n[2];
+        double tz = H[2] * n[0] + H[5] * n[1] + H[8] * n[2];
+        double x0, y0, z0;
+        load3(x, a0, x0, y0, z0);
+        for (int a = 0; a < P.nv; a++) {
+            double xa, ya, za, ox, oy, oz;
+            load3(x, a0 + a, xa, ya, za);
+            cm::minimage(g.cell, xa - x0, ya - y0, za - z0, &ox, &oy, &oz);
+            B.vw[(int64_t)f * P.Nv + a0 + a] = mk4((float)(x0 + ox - tx), (float)(y0 + oy - ty), (float)(z0 + oz - tz), 0.f);
+        }
+    }
+}
+
+// exclusive scan of the cell counts of passes [p0, p0+np) of every frame (one
+// block per (frame, pass)); zeroes the counts and writes the number of near
+// molecules of the pass
+__global__ void __launch_bounds__(1024) k_scan_rows(KP P, Bufs B, int p0, int np)
+{
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    const int f = blockIdx.x / np, pp = p0 + (int)(blockIdx.x - (blockIdx.x / np) * np);
+    const int64_t row = (int64_t)f * P.NP + pp;
+    int* c = B.mcnt + row * (P.kcap + 1);
+    int* st = B.mstart + row * (P.kcap + 1);
+    const int n = P.kcap;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int base = 0; base < n; base += blockDim.x) {
+        int idx = base + threadIdx.x;
+        int v = idx < n ? c[idx] : 0;
+        int incl = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        int woff = 0, tot = 0;
+        for (int w = 0; w < nw; w++) { if (w < wid) woff += wsum[w]; tot += wsum[w]; }
+        if (idx < n) { st[idx] = carry + woff + incl - v; c[idx] = 0; }
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        st[n] = carry;
+        B.nnear[row] = carry;
+    }
+}
+
+// molecules of passes [p0, p0+np) sorted by cell
+__global__ void k_mol_scatter(KP P, Bufs B, int p0, int np)
+{
+    int64_t total = (int64_t)P.F * np * P.Mv;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int64_t fq = it / P.Mv;
+        int v = (int)(it - fq * P.Mv);
+        int f = (int)(fq / np);
+        int p = p0 + (int)(fq - (int64_t)f * np);
+        int64_t fp = (int64_t)f * P.NP + p;
+        int key = B.mkey[fp * P.Mv + v];
+        if (key < 0) continue;
+        int64_t cb = fp * (P.kcap + 1);
+        int pos = B.mstart[cb + key] + atomicAdd(&B.mcnt[cb + key], 1);
+        B.msorted[fp * P.Mv + pos] = v;
+    }
+}
+
+// chunk offsets of the passes [p0, p0+np) of every frame (one block)
+__global__ void __launch_bounds__(1024) k_chunk_table(KP P, Bufs B, int p0, int np)
+{
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    int n = P.F * np;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int base = 0; base < n; base += blockDim.x) {
+        int idx = base + threadIdx.x;
+        int v = 0;
+        if (idx < n) {
+            int f = idx / np, p = p0 + idx - f * np;
+            int nn = B.nnear[(int64_t)f * P.NP + p];
+            v = (nn + P.mpt - 1) / P.mpt;
+        }
+        int incl = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        int woff = 0, tot = 0;
+        for (int w = 0; w < nw; w++) { if (w < wid) woff += wsum[w]; tot += wsum[w]; }
+        if (idx < n) B.chunk_base[idx] = carry + woff + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) B.chunk_base[n] = carry;
+}
+
+// randomize_solvent! (src/mddf.jl:65-88) for the slots whose new centre falls
+// in a near cell; the other slots cannot have an atom within the cutoff
+template <class TV>
+__global__ void k_place_tiles(KP P, Bufs B, const TV* xv)
+{
+    int64_t total = (int64_t)P.F * P.K * P.Mv;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int64_t fk = it / P.Mv;
+        int v = (int)(it - fk * P.Mv);
+        int f = (int)(fk / P.K);
+        int k = (int)(fk - (int64_t)f * P.K);
+        const Geo& g = B.geo[f];
+        int64_t fp = (int64_t)f * P.NP + 1 + k;
+        cm::u32x4 r = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT}, P.key0, P.key1);
+        double uc[3] = {cm::u01(r.y), cm::u01(r.z), cm::u01(r.w)};
+        int cell;
+        cell_of_frac(g, uc[0], uc[1], uc[2], cell);
+        if (!B.near[(int64_t)f * P.cap + cell]) { B.mkey[fp * P.Mv + v] = -1; continue; }
+        cm::u32x4 r2 = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT + 1u}, P.key0, P.key1);
+        int64_t lk = (int64_t)f * P.kb;
+        int nb = B.nbulk[lk];
+        int64_t jmol = nb > 0 ? (int64_t)B.blist[lk * P.Mv + cm::draw_index(r.x, nb)] : cm::draw_index(r.x, P.Mv);
+        const TV* src = xv + (int64_t)f * P.Nv * 3 + jmol * P.nv * 3;
+        double* dst = B.P + (fk * P.Nv + (int64_t)v * P.nv) * 3;
+        for (int q = 0; q < 3 * P.nv; q++) dst[q] = (double)src[q];
+        MolAcc<TV> acc{dst};
+        double ua[3] = {cm::u01(r2.x), cm::u01(r2.y), cm::u01(r2.z)};
+        cm::random_move(acc, P.nv, P.iref, g.cell, uc, ua);
+        for (int a = 0; a < P.nv; a++)
+            B.vw_r[fk * P.Nv + (int64_t)v * P.nv + a] = mk4((float)dst[3 * a], (float)dst[3 * a + 1], (float)dst[3 * a + 2], 0.f);
+        const int key = morton_key(g, cell);
+        B.mkey[fp * P.Mv + v] = key;
+        atomicAdd(&B.mcnt[fp * (P.kcap + 1) + key], 1);
+    }
+}
+
+__device__ __forceinline__ float wave_min(float x)
+{
+    for (int o = 32; o >= 1; o >>= 1) x = fminf(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__device__ __forceinline__ float wave_max(float x)
+{
+    for (int o = 32; o >= 1; o >>= 1) x = fmaxf(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__device__ __forceinline__ int wave_incl_scan(int v, int lane)
+{
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// lexicographic minimum of two exact records, OR of the reference flags
+__device__ __forceinline__ void rec_merge(Rec& r, const Rec& o)
+{
+    if (o.within && (!r.within || o.d < r.d || (o.d == r.d && (o.i < r.i || (o.i == r.i && o.j < r.j))))) {
+        r.within = 1; r.i = o.i; r.j = o.j; r.d = o.d;
+    }
+    if (o.ref) { r.ref = 1; r.dref = fmin(r.dref, o.dref); }
+}
+
+__device__ __forceinline__ Rec rec_shfl_xor(const Rec& r, int o)
+{
+    Rec q;
+    q.within = __shfl_xor(r.within, o, 64);
+    q.i = __shfl_xor(r.i, o, 64);
+    q.j = (int64_t)__shfl_xor((long long)r.j, o, 64);
+    q.d = __shfl_xor(r.d, o, 64);
+    q.ref = __shfl_xor(r.ref, o, 64);
+    q.dref = __shfl_xor(r.dref, o, 64);
+    return q;
+}
+
+// Minimum distances of the solvent molecules of one chunk per wavefront
+// (64 threads).  Lane = (atom row, column); a row is one solvent atom and
+// the C columns of a row split the staged solute atoms between them.
+template <class TU, class TV, bool RANDOM, int C>
+__global__ void __launch_bounds__(64) k_eval_tiles(KP P, Bufs B, const TU* xu, const TV* xv)
+{
+    __shared__ float4 stg[TCAP];
+    __shared__ int nstg;
+    extern __shared__ unsigned sh[];
+    hist_clear(sh, 2 * P.nbins);
+    const int lane = threadIdx.x;
+    const int row = lane / C, col = lane - (lane / C) * C;
+    const int p0 = RANDOM ? 1 : 0;
+    const int npass = RANDOM ? P.K : 1;
+    const int nfp = P.F * npass;
+    const int total = B.chunk_base[nfp];
+    ull* gu = RANDOM ? B.cnt_gu_r : B.cnt_gu;
+    ull* gv = RANDOM ? B.cnt_gv_r : B.cnt_gv;
+    for (int ch = blockIdx.x; ch < total; ch += gridDim.x) {
+        int lo_ = 0, hi_ = nfp;
+        while (hi_ - lo_ > 1) {
+            int mid = (lo_ + hi_) >> 1;
+            if (B.chunk_base[mid] <= ch) lo_ = mid; else hi_ = mid;
+        }
+        const int fq = lo_;
+        const int f = fq / npass, pidx = fq - (fq / npass) * npass;
+        const int p = p0 + pidx;
+        const int64_t fp = (int64_t)f * P.NP + p;
+        const Geo& g = B.geo[f];
+        const int m0 = (ch - B.chunk_base[fq]) * P.mpt;
+        const int nm = min(P.mpt, B.nnear[fp] - m0);
+        const int ml = row / P.nv, a = row - (row / P.nv) * P.nv;
+        const bool active = ml < nm;
+        const int v = active ? B.msorted[fp * P.Mv + m0 + ml] : 0;
+        const float4* vwb = RANDOM ? B.vw_r + ((int64_t)f * P.K + pidx) * P.Nv : B.vw + (int64_t)f * P.Nv;
+        const float4 y = active ? vwb[(int64_t)v * P.nv + a] : mk4(0.f, 0.f, 0.f, 0.f);
+        const float lx = wave_min(active ? y.x : INFINITY), ly = wave_min(active ? y.y : INFINITY),
+                    lz = wave_min(active ? y.z : INFINITY);
+        const float hx = wave_max(active ? y.x : -INFINITY), hy = wave_max(active ? y.y : -INFINITY),
+                    hz = wave_max(active ? y.z : -INFINITY);
+        const float rs = g.rstage, rs2 = rs * rs;
+        int ilo[3], ihi[3];
+        cell_range(g, lx, ly, lz, hx, hy, hz, rs, ilo, ihi);
+        const int nx = ihi[0] - ilo[0] + 1, ny = ihi[1] - ilo[1] + 1, nz = ihi[2] - ilo[2] + 1;
+        const int ncl = nx * ny * nz;
+        const int* st = B.start + (int64_t)f * (P.cap + 1);
+        const float4* srt = B.sorted + (int64_t)f * B.nitems;
+        Best bb;
+        bb.init();
+        int rounds = 0;
+        if (lane == 0) nstg = 0;
+        __syncthreads();
+        auto stage_atom = [&](float4 s, float tx, float ty, float tz) {
+            float qx = s.x + tx, qy = s.y + ty, qz = s.z + tz;
+            float dx = fmaxf(fmaxf(lx - qx, qx - hx), 0.f);
+            float dy = fmaxf(fmaxf(ly - qy, qy - hy), 0.f);
+            float dz = fmaxf(fmaxf(lz - qz, qz - hz), 0.f);
+            if (dx * dx + dy * dy + dz * dz <= rs2) {
+                int slot = atomicAdd(&nstg, 1);
+                stg[slot] = mk4(qx, qy, qz, s.w);
+            }
+        };
+        auto evaluate = [&]() {
+            __syncthreads();
+            const int n = nstg;
+            int k = col;
+            for (; k + 3 * C < n; k += 4 * C) {
+                float4 s0 = stg[k], s1 = stg[k + C], s2 = stg[k + 2 * C], s3 = stg[k + 3 * C];
+                float dx, dy, dz;
+                dx = y.x - s0.x; dy = y.y - s0.y; dz = y.z - s0.z;
+                bb.offer(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), __float_as_int(s0.w), 0);
+                dx = y.x - s1.x; dy = y.y - s1.y; dz = y.z - s1.z;
+                bb.offer(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), __float_as_int(s1.w), 0);
+                dx = y.x - s2.x; dy = y.y - s2.y; dz = y.z - s2.z;
+                bb.offer(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), __float_as_int(s2.w), 0);
+                dx = y.x - s3.x; dy = y.y - s3.y; dz = y.z - s3.z;
+                bb.offer(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), __float_as_int(s3.w), 0);
+            }
+            for (; k < n; k += C) {
+                float4 s0 = stg[k];
+                float dx = y.x - s0.x, dy = y.y - s0.y, dz = y.z - s0.z;
+                bb.offer(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), __float_as_int(s0.w), 0);
+            }
+            rounds++;
+            __syncthreads();
+            if (lane == 0) nstg = 0;
+            __syncthreads();
+        };
+        for (int cb = 0; cb < ncl; cb += 64) {
+            const int ci = cb + lane;
+            int cstart = 0, cnt = 0;
+            float tx = 0.f, ty = 0.f, tz = 0.f;
+            if (ci < ncl) {
+                int ix = ilo[0] + ci % nx, iy = ilo[1] + (ci / nx) % ny, iz = ilo[2] + ci / (nx * ny);
+                int mx, my, mz;
+                int cx = wrap_index(ix, g.nc[0], mx), cy = wrap_index(iy, g.nc[1], my), cz = wrap_index(iz, g.nc[2], mz);
+                int cell = (cz * g.nc[1] + cy) * g.nc[0] + cx;
+                cstart = st[cell];
+                cnt = st[cell + 1] - cstart;
+                shift_of(g, mx, my, mz, tx, ty, tz);
+            }
+            bool done = cnt == 0;
+            while (true) {
+                const int pend = done ? 0 : cnt;
+                const int incl = wave_incl_scan(pend, lane);
+                const int tot = __shfl(incl, 63, 64);
+                if (tot == 0) break;
+                const int room = TCAP - nstg;
+                const bool take = !done && incl <= room;
+                if (__ballot(take) == 0ull) {
+                    if (nstg > 0) { evaluate(); continue; }
+                    // a single cell larger than the staging buffer: in pieces
+                    const int L = __ffsll((long long)__ballot(pend > 0)) - 1;
+                    const int s0 = __shfl(cstart, L, 64), c0 = __shfl(cnt, L, 64);
+                    const float t0 = __shfl(tx, L, 64), t1 = __shfl(ty, L, 64), t2 = __shfl(tz, L, 64);
+                    for (int off = 0; off < c0; off += TCAP) {
+                        const int nq = min(TCAP, c0 - off);
+                        for (int q = lane; q < nq; q += 64) stage_atom(srt[s0 + off + q], t0, t1, t2);
+                        evaluate();
+                    }
+                    if (lane == L) done = true;
+                    continue;
+                }
+                if (take) {
+                    for (int q = 0; q < cnt; q++) stage_atom(srt[cstart + q], tx, ty, tz);
+                    done = true;
+                }
+                __syncthreads();
+            }
+        }
+        const bool one_round = rounds == 0;
+        const int last_n = nstg;   // the last staged batch stays in LDS until the next chunk
+        if (last_n > 0) evaluate();
+        // columns of a row -> the row's atom
+        for (int o = 1; o < C; o <<= 1) {
+            float b2 = __shfl_xor(bb.best, o, 64), s2 = __shfl_xor(bb.second, o, 64);
+            int i2 = __shfl_xor(bb.bi, o, 64);
+            float ns = fminf(fmaxf(bb.best, b2), fminf(bb.second, s2));
+            if (b2 < bb.best) bb.bi = i2;
+            bb.best = fminf(bb.best, b2);
+            bb.second = ns;
+        }
+        // atoms of a molecule -> the molecule (leader lane: atom 0, column 0)
+        const int mrow0 = ml * P.nv;
+        float mb = bb.best, ms = bb.second;
+        int mi = bb.bi, ma = 0;
+        for (int q = 1; q < P.nv; q++) {
+            const int src = ((mrow0 + q) * C) & 63;
+            float b2 = __shfl(bb.best, src, 64), s2 = __shfl(bb.second, src, 64);
+            int i2 = __shfl(bb.bi, src, 64);
+            float ns = fminf(fmaxf(mb, b2), fminf(ms, s2));
+            if (b2 < mb) { mi = i2; ma = q; }
+            mb = fminf(mb, b2);
+            ms = ns;
+        }
+        const int srcr = ((mrow0 + P.iref) * C) & 63;
+        const float rb = __shfl(bb.best, srcr, 64), rsec = __shfl(bb.second, srcr, 64);
+        const int ri = __shfl(bb.bi, srcr, 64);
+        const bool leader = active && a == 0 && col == 0;
+        const TU* xuf = xu + (int64_t)f * P.Nu * 3;
+        const int64_t jbase = (int64_t)v * P.nv;
+        auto ysrc = [&](int q, double& y0, double& y1, double& y2) {
+            if (RANDOM) {
+                const double* pp = B.P + (((int64_t)f * P.K + pidx) * P.Nv + jbase + q) * 3;
+                y0 = pp[0]; y1 = pp[1]; y2 = pp[2];
+            } else load3(xv + (int64_t)f * P.Nv * 3, jbase + q, y0, y1, y2);
+        };
+        Rec r = rec_zero();
+        bool exh = false;
+        if (leader) {
+            if (mb <= g.cut2_hi) {
+                if (ms - mb > g.amb) {
+                    double x0, x1, x2, y0, y1, y2;
+                    load3(xuf, mi, x0, x1, x2);
+                    ysrc(ma, y0, y1, y2);
+                    double d2 = cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
+                    if (d2 <= P.cut2) { r.within = 1; r.i = mi; r.j = jbase + ma; r.d = sqrt(d2); }
+                } else exh = true;
+            }
+            if (rb <= g.cut2_hi) {
+                if (rsec - rb > g.amb) {
+                    double x0, x1, x2, y0, y1, y2;
+                    load3(xuf, ri, x0, x1, x2);
+                    ysrc(P.iref, y0, y1, y2);
+                    double d2 = cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
+                    if (d2 <= P.cut2) { r.ref = 1; r.dref = sqrt(d2); }
+                } else exh = true;
+            }
+        }
+        // ambiguous molecules: every staged pair within the band, in FP64
+        unsigned long long amask = __ballot(exh);
+        if (amask && lane == 0) atomicAdd(&B.stats[0], (ull)__popcll(amask));
+        while (amask) {
+            const int L = __ffsll((long long)amask) - 1;
+            amask &= amask - 1ull;
+            const float bandm = __shfl(mb, L, 64) + g.amb, bandr = __shfl(rb, L, 64) + g.amb;
+            const int mlL = (L / C) / P.nv;
+            const int vL = __shfl(v, L, 64);
+            const int64_t jb = (int64_t)vL * P.nv;
+            Rec loc = rec_zero();
+            for (int q = 0; q < P.nv; q++) {
+                const int sl = ((mlL * P.nv + q) * C) & 63;
+                const float yx = __shfl(y.x, sl, 64), yy = __shfl(y.y, sl, 64), yz = __shfl(y.z, sl, 64);
+                double Y0, Y1, Y2;
+                if (RANDOM) {
+                    const double* pp = B.P + (((int64_t)f * P.K + pidx) * P.Nv + jb + q) * 3;
+                    Y0 = pp[0]; Y1 = pp[1]; Y2 = pp[2];
+                } else load3(xv + (int64_t)f * P.Nv * 3, jb + q, Y0, Y1, Y2);
+                const bool isref = q == P.iref;
+                if (one_round) {
+                    for (int k = lane; k < last_n; k += 64) {
+                        float4 s = stg[k];
+                        float dx = yx - s.x, dy = yy - s.y, dz = yz - s.z;
+                        float d2f = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        if (d2f <= bandm || (isref && d2f <= bandr)) {
+                            int i = __float_as_int(s.w);
+                            double x0, x1, x2;
+                            load3(xuf, i, x0, x1, x2);
+                            double d2 = cm::canon_d2(g.cell, x0, x1, x2, Y0, Y1, Y2);
+                            if (d2 <= P.cut2) rec_offer(loc, i, jb + q, sqrt(d2), isref);
+                        }
+                    }
+                } else {
+                    // staged buffer recycled: scan the cells around the atom
+                    int jlo[3], jhi[3];
+                    cell_range(g, yx, yy, yz, yx, yy, yz, rs, jlo, jhi);
+                    const int mxn = jhi[0] - jlo[0] + 1, myn = jhi[1] - jlo[1] + 1, mzn = jhi[2] - jlo[2] + 1;
+                    const int ncq = mxn * myn * mzn;
+                    for (int ci = lane; ci < ncq; ci += 64) {
+                        int ix = jlo[0] + ci % mxn, iy = jlo[1] + (ci / mxn) % myn, iz = jlo[2] + ci / (mxn * myn);
+                        int mx, my, mz;
+                        int cx = wrap_index(ix, g.nc[0], mx), cy = wrap_index(iy, g.nc[1], my), cz = wrap_index(iz, g.nc[2], mz);
+                        int cell = (cz * g.nc[1] + cy) * g.nc[0] + cx;
+                        for (int pq = st[cell]; pq < st[cell + 1]; pq++) {
+                            int i = __float_as_int(srt[pq].w);
+                            double x0, x1, x2;
+                            load3(xuf, i, x0, x1, x2);
+                            double d2 = cm::canon_d2(g.cell, x0, x1, x2, Y0, Y1, Y2);
+                            if (d2 <= P.cut2) rec_offer(loc, i, jb + q, sqrt(d2), isref);
+                        }
+                    }
+                }
+            }
+            for (int o = 32; o >= 1; o >>= 1) rec_merge(loc, rec_shfl_xor(loc, o));
+            if (lane == L) r = loc;
+        }
+        if (leader) {
+            hist_record(P, sh, gu, gv, r);
+            if (!RANDOM && P.K > 0) B.bflag[(int64_t)f * P.kb * P.Mv + v] = inbulk(P, r) ? 1 : 0;
+        }
+        __syncthreads();
+    }
+    if (RANDOM) hist_flush(sh, P.nbins, B.cnt_md_r, B.cnt_rdf_r);
+    else hist_flush(sh, P.nbins, B.cnt_md, B.cnt_rdf);
+}
+
 // integer hits -> FP64 accumulators
 __global__ void k_flush(ull* cnt, double* acc, ull* hits, int64_t n, int64_t nhist, int64_t half_lo, int64_t half_hi,
                         double w)
@@ -947,7 +1401,11 @@ struct cm_job {
     // batch buffers
     DevBuf geo, cells_cnt, cells_start, tmp4, tmpcell, sorted, solw, ucen, vw, vcell, vcen;
     DevBuf P, vw_r, vcell_r, vcen_r, bflag, blist, nbulk, uk, drawn_ptr, drawn_idx;
+    DevBuf near, mkey, mcnt, mstart, msorted, nnear, chunk_base;
     DevBuf xin_u, xin_v;
+    double cell_frac = 0.5;   // tiles family: fine cell side as a fraction of the cutoff
+    int tile_c = 1;           // tiles family: columns per atom row (1, 2 or 4)
+    int eval_grid = 4096;     // tiles family: workgroups of the persistent evaluation kernel
     int Fcap = 0;
     int cap_cells = 0;
     bool finished = false;
@@ -961,29 +1419,76 @@ struct cm_job {
 
 namespace {
 
-int64_t per_frame_bytes(const cm_job* J, int dtype_size, bool host_input)
+int64_t per_frame_bytes(const cm_job* J, int dtype_size, bool host_input, int64_t cap, int64_t kcap)
 {
-    int64_t Mu = J->d.nmols_u, Mv = J->d.nmols_v;
+    int64_t Mu = J->d.nmols_u, Mv = J->d.nmols_v, NP = 1 + J->K;
     int64_t b = sizeof(Geo) + 64;
     if (host_input) b += (J->Nu + J->Nv) * 3 * dtype_size;
     int64_t items = J->atoms_family ? J->d.natoms_per_mol_u : Mu;
-    b += items * (16 + 16 + 4);
-    if (!J->atoms_family) b += J->Nu * 16 + Mu * 16 + (Mu + 1) * 4 + J->K * 8;
-    b += J->Nv * 16 + Mv * 20;
-    b += (int64_t)J->K * (J->Nv * (24 + 16) + Mv * 20);
+    b += items * (16 + 16 + 4) + (cap + 1) * 8;
+    if (!J->atoms_family) {
+        b += J->Nu * 16 + Mu * 16 + (Mu + 1) * 4 + J->K * 8;
+        b += J->Nv * 16 + Mv * 20;
+        b += (int64_t)J->K * (J->Nv * (24 + 16) + Mv * 20);
+    } else {
+        b += cap + NP * ((kcap + 1) * 8 + Mv * 8 + 4) + 8;
+        b += J->Nv * 16 + (int64_t)J->K * J->Nv * (24 + 16);
+    }
     b += (int64_t)J->kb * Mv * 5 + J->kb * 4;
     return b;
 }
 
+int64_t cells_for(const cm_job* J, const double* cell9, int64_t* keys)
+{
+    cm::Cell64 c;
+    cm::cell_setup(cell9, J->d.triclinic, &c);
+    double sp[3];
+    plane_spacings(c.H, sp);
+    double side = J->atoms_family ? J->kp.cutsys * J->cell_frac : J->kp.cutsys;
+    int64_t n = 1, mx = 1;
+    for (int k = 0; k < 3; k++) {
+        int64_t nk = std::max<int64_t>(1, (int64_t)std::floor(sp[k] / side));
+        n *= nk;
+        mx = std::max(mx, nk);
+    }
+    int bits = 0;
+    while (((int64_t)1 << bits) < mx) bits++;
+    *keys = (int64_t)1 << (3 * bits);
+    return n;
+}
+
+template <int C>
+void launch_tiles(bool f32, bool random, hipStream_t s, int grid, size_t shm, const KP& P, const Bufs& B,
+                  const void* xu, const void* xv)
+{
+    if (f32) {
+        if (random) hipLaunchKernelGGL((k_eval_tiles<float, double, true, C>), dim3(grid), dim3(64), shm, s, P, B, (const float*)xu, (const double*)nullptr);
+        else hipLaunchKernelGGL((k_eval_tiles<float, float, false, C>), dim3(grid), dim3(64), shm, s, P, B, (const float*)xu, (const float*)xv);
+    } else {
+        if (random) hipLaunchKernelGGL((k_eval_tiles<double, double, true, C>), dim3(grid), dim3(64), shm, s, P, B, (const double*)xu, (const double*)nullptr);
+        else hipLaunchKernelGGL((k_eval_tiles<double, double, false, C>), dim3(grid), dim3(64), shm, s, P, B, (const double*)xu, (const double*)xv);
+    }
+}
+
+void launch_tiles_c(int c, bool f32, bool random, hipStream_t s, int grid, size_t shm, const KP& P, const Bufs& B,
+                    const void* xu, const void* xv)
+{
+    if (c == 4) launch_tiles<4>(f32, random, s, grid, shm, P, B, xu, xv);
+    else if (c == 2) launch_tiles<2>(f32, random, s, grid, shm, P, B, xu, xv);
+    else launch_tiles<1>(f32, random, s, grid, shm, P, B, xu, xv);
+}
+
 int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, const void* xv_dev, int dtype,
-              const double* cells, int64_t cap)
+              const double* cells, int64_t cap, int64_t kcap)
 {
     KP P = J->kp;
     P.F = F;
     P.cap = (int)cap;
-    // geometry on the host: cells, inverse, spacings; the device adds radii and grid
+    P.kcap = (int)kcap;
+    const bool tiles = J->atoms_family;
+    // geometry on the host: cells, inverse, spacings (and the fine grid of the tiles family)
     J->hgeo.assign(F, Geo());
-    double cutsys = P.cutsys;
+    const double cutsys = P.cutsys;
     for (int f = 0; f < F; f++) {
         Geo& g = J->hgeo[f];
         memset(&g, 0, sizeof(Geo));
@@ -994,8 +1499,22 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         if (!(cutsys < 0.5 * spmin * (1.0 - 1e-9)))
             return fail(CM_EUNSUPPORTED, "cutoff must be smaller than half of the smallest plane spacing of the unit cell");
         g.frame = (uint32_t)frame_index[f];
+        if (tiles) {
+            double side = cutsys * J->cell_frac;
+            g.ncell = 1;
+            for (int k = 0; k < 3; k++) {
+                g.nc[k] = std::max(1, (int)std::floor(g.sp[k] / side));
+                g.ncell *= g.nc[k];
+            }
+            if (g.ncell > cap) return fail(CM_EHIP, "internal error: fine grid larger than its allocation");
+            int mx = std::max(g.nc[0], std::max(g.nc[1], g.nc[2]));
+            g.kbits = 0;
+            while ((1 << g.kbits) < mx) g.kbits++;
+            if ((int64_t)1 << (3 * g.kbits) > kcap) return fail(CM_EHIP, "internal error: key space larger than its allocation");
+        }
     }
-    HIPCHK(hipMemcpyAsync(J->geo.p, J->hgeo.data(), sizeof(Geo) * F, hipMemcpyHostToDevice, J->stream));
+    hipStream_t s = J->stream;
+    HIPCHK(hipMemcpyAsync(J->geo.p, J->hgeo.data(), sizeof(Geo) * F, hipMemcpyHostToDevice, s));
     Bufs B;
     memset(&B, 0, sizeof(B));
     B.geo = (Geo*)J->geo.p;
@@ -1004,7 +1523,7 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
     B.tmp4 = (float4*)J->tmp4.p;
     B.tmpcell = (int*)J->tmpcell.p;
     B.sorted = (float4*)J->sorted.p;
-    B.nitems = J->atoms_family ? P.nu : P.Mu;
+    B.nitems = tiles ? P.nu : P.Mu;
     B.solw = (float4*)J->solw.p;
     B.ucen = (float4*)J->ucen.p;
     B.vw = (float4*)J->vw.p;
@@ -1020,14 +1539,22 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
     B.uk = (int*)J->uk.p;
     B.drawn_ptr = (int*)J->drawn_ptr.p;
     B.drawn_idx = (int*)J->drawn_idx.p;
+    B.near = (unsigned char*)J->near.p;
+    B.mkey = (int*)J->mkey.p;
+    B.mcnt = (int*)J->mcnt.p;
+    B.mstart = (int*)J->mstart.p;
+    B.msorted = (int*)J->msorted.p;
+    B.nnear = (int*)J->nnear.p;
+    B.chunk_base = (int*)J->chunk_base.p;
     ull* c = (ull*)J->cnt.p;
     int64_t nb = P.nbins, ngu = J->d.n_groups_u, ngv = J->d.n_groups_v;
     B.cnt_md = c; B.cnt_md_r = c + nb; B.cnt_rdf = c + 2 * nb; B.cnt_rdf_r = c + 3 * nb;
     B.cnt_gu = c + 4 * nb; B.cnt_gv = B.cnt_gu + ngu * nb; B.cnt_gu_r = B.cnt_gv + ngv * nb; B.cnt_gv_r = B.cnt_gu_r + ngu * nb;
     B.stats = (ull*)J->stats.p;
 
-    // random draws of the reference solute molecules (mddf_frame!, src/mddf.jl:374-376)
-    if (!J->atoms_family && J->K > 0) {
+    // random draws of the reference solute molecules (mddf_frame!, src/mddf.jl:374-376);
+    // with one solute molecule every draw is molecule 0
+    if (!tiles && J->K > 0) {
         J->huk.assign((size_t)F * J->K, 0);
         J->hdp.assign((size_t)F * (P.Mu + 1), 0);
         J->hdi.assign((size_t)F * J->K, 0);
@@ -1046,71 +1573,96 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
                 J->hdi[(size_t)f * J->K + dp[u] + fill[u]++] = k;
             }
         }
-        HIPCHK(hipMemcpyAsync(J->uk.p, J->huk.data(), sizeof(int) * J->huk.size(), hipMemcpyHostToDevice, J->stream));
-        HIPCHK(hipMemcpyAsync(J->drawn_ptr.p, J->hdp.data(), sizeof(int) * J->hdp.size(), hipMemcpyHostToDevice, J->stream));
-        HIPCHK(hipMemcpyAsync(J->drawn_idx.p, J->hdi.data(), sizeof(int) * J->hdi.size(), hipMemcpyHostToDevice, J->stream));
+        HIPCHK(hipMemcpyAsync(J->uk.p, J->huk.data(), sizeof(int) * J->huk.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(J->drawn_ptr.p, J->hdp.data(), sizeof(int) * J->hdp.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(J->drawn_idx.p, J->hdi.data(), sizeof(int) * J->hdi.size(), hipMemcpyHostToDevice, s));
     } else if (J->K > 0) {
-        HIPCHK(hipMemsetAsync(J->uk.p, 0, sizeof(int) * (size_t)F * J->K, J->stream));
+        HIPCHK(hipMemsetAsync(J->uk.p, 0, sizeof(int) * (size_t)F * J->K, s));
     }
-    HIPCHK(hipMemsetAsync(J->cells_cnt.p, 0, sizeof(int) * (size_t)F * (cap + 1), J->stream));
+    HIPCHK(hipMemsetAsync(J->cells_cnt.p, 0, sizeof(int) * (size_t)F * (cap + 1), s));
+    if (tiles) {
+        HIPCHK(hipMemsetAsync(J->near.p, 0, (size_t)F * cap, s));
+        HIPCHK(hipMemsetAsync(J->mcnt.p, 0, sizeof(int) * (size_t)F * P.NP * (kcap + 1), s));
+    }
 
-    bool f32 = dtype == CM_F32;
-    hipStream_t s = J->stream;
-    if (J->profiling) HIPCHK(hipEventRecord(J->e0, s));
-    int mols = J->atoms_family ? 0 : 1;
-    int64_t nrad = (int64_t)F * (P.Mv + (mols ? P.Mu : 0));
-    if (f32) hipLaunchKernelGGL(k_radii<float>, dim3(grid_for(nrad)), dim3(BLOCK), 0, s, P, B, (const float*)xu_dev, (const float*)xv_dev, mols);
-    else hipLaunchKernelGGL(k_radii<double>, dim3(grid_for(nrad)), dim3(BLOCK), 0, s, P, B, (const double*)xu_dev, (const double*)xv_dev, mols);
+    const bool f32 = dtype == CM_F32;
+    const float* uf = (const float*)xu_dev;
+    const float* vf = (const float*)xv_dev;
+    const double* ud = (const double*)xu_dev;
+    const double* vd = (const double*)xv_dev;
+    const size_t shm = sizeof(unsigned) * 2 * P.nbins;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (J->profiling) {
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventRecord(J->e0, s));
+    }
+    const int mols = tiles ? 0 : 1;
+    {
+        int64_t n = (int64_t)F * (P.Mv + (mols ? P.Mu : 0));
+        if (f32) hipLaunchKernelGGL(k_radii<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, uf, vf, mols);
+        else hipLaunchKernelGGL(k_radii<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, ud, vd, mols);
+    }
     hipLaunchKernelGGL(k_grid, dim3(1), dim3(64), 0, s, P, B, mols);
-    if (J->atoms_family) {
+    if (tiles) {
         int64_t n = (int64_t)F * P.nu;
-        if (f32) hipLaunchKernelGGL(k_wrap_atoms<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const float*)xu_dev);
-        else hipLaunchKernelGGL(k_wrap_atoms<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const double*)xu_dev);
+        if (f32) hipLaunchKernelGGL(k_wrap_atoms<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, uf);
+        else hipLaunchKernelGGL(k_wrap_atoms<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, ud);
     } else {
         int64_t n = (int64_t)F * P.Mu;
-        if (f32) hipLaunchKernelGGL(k_prep_solute_mols<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const float*)xu_dev);
-        else hipLaunchKernelGGL(k_prep_solute_mols<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const double*)xu_dev);
+        if (f32) hipLaunchKernelGGL(k_prep_solute_mols<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, uf);
+        else hipLaunchKernelGGL(k_prep_solute_mols<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, ud);
     }
     hipLaunchKernelGGL(k_scan_cells, dim3(F), dim3(1024), 0, s, P, B);
     hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)F * B.nitems)), dim3(BLOCK), 0, s, P, B, B.nitems);
-    {
+    if (tiles) {
+        hipLaunchKernelGGL(k_near_mark, dim3(grid_for((int64_t)F * P.nu)), dim3(BLOCK), 0, s, P, B);
         int64_t n = (int64_t)F * P.Mv;
-        if (f32) hipLaunchKernelGGL(k_prep_solvent<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const float*)xv_dev);
-        else hipLaunchKernelGGL(k_prep_solvent<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const double*)xv_dev);
+        if (f32) hipLaunchKernelGGL(k_prep_solvent_tiles<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);
+        else hipLaunchKernelGGL(k_prep_solvent_tiles<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vd);
+        hipLaunchKernelGGL(k_scan_rows, dim3(F), dim3(1024), 0, s, P, B, 0, 1);
+        hipLaunchKernelGGL(k_mol_scatter, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, 0, 1);
+        hipLaunchKernelGGL(k_chunk_table, dim3(1), dim3(1024), 0, s, P, B, 0, 1);
+    } else {
+        int64_t n = (int64_t)F * P.Mv;
+        if (f32) hipLaunchKernelGGL(k_prep_solvent<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);
+        else hipLaunchKernelGGL(k_prep_solvent<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vd);
     }
     if (P.K > 0) hipLaunchKernelGGL(k_bulk_init, dim3(grid_for((int64_t)F * P.kb * P.Mv)), dim3(BLOCK), 0, s, P, B);
-    size_t shm = sizeof(unsigned) * 2 * P.nbins;
-    if (J->profiling) HIPCHK(hipEventRecord(J->e1, s));
-    {
+    HIPCHK(hipGetLastError());
+    // real pass
+    if (J->profiling) HIPCHK(hipEventRecord(ev[0], s));
+    if (tiles) {
+        launch_tiles_c(J->tile_c, f32, false, s, J->eval_grid, shm, P, B, xu_dev, xv_dev);
+    } else {
         int64_t n = (int64_t)F * P.Mv;
-        if (J->atoms_family) {
-            if (f32) hipLaunchKernelGGL((k_eval_atoms<float, float, false>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const float*)xu_dev, (const float*)xv_dev);
-            else hipLaunchKernelGGL((k_eval_atoms<double, double, false>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const double*)xu_dev, (const double*)xv_dev);
-        } else {
-            if (f32) hipLaunchKernelGGL((k_eval_mols_real<float, float>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const float*)xu_dev, (const float*)xv_dev);
-            else hipLaunchKernelGGL((k_eval_mols_real<double, double>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const double*)xu_dev, (const double*)xv_dev);
-        }
+        if (f32) hipLaunchKernelGGL((k_eval_mols_real<float, float>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, uf, vf);
+        else hipLaunchKernelGGL((k_eval_mols_real<double, double>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, ud, vd);
     }
-    if (J->profiling) HIPCHK(hipEventRecord(J->e2, s));
+    if (J->profiling) HIPCHK(hipEventRecord(ev[1], s));
+    HIPCHK(hipGetLastError());
+    // random passes
     if (P.K > 0) {
         hipLaunchKernelGGL(k_bulk_compact, dim3(F * P.kb), dim3(1024), 0, s, P, B);
         int64_t n = (int64_t)F * P.K * P.Mv;
-        if (f32) hipLaunchKernelGGL(k_place<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const float*)xv_dev);
-        else hipLaunchKernelGGL(k_place<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, (const double*)xv_dev);
-    }
-    HIPCHK(hipGetLastError());
-    hipEvent_t er0 = nullptr, er1 = nullptr;
-    if (P.K > 0) {
-        if (J->profiling) { HIPCHK(hipEventCreate(&er0)); HIPCHK(hipEventCreate(&er1)); HIPCHK(hipEventRecord(er0, s)); }
-        int64_t n = (int64_t)F * P.K * P.Mv;
-        if (J->atoms_family) {
-            if (f32) hipLaunchKernelGGL((k_eval_atoms<float, double, true>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const float*)xu_dev, (const double*)nullptr);
-            else hipLaunchKernelGGL((k_eval_atoms<double, double, true>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const double*)xu_dev, (const double*)nullptr);
+        if (tiles) {
+            if (f32) hipLaunchKernelGGL(k_place_tiles<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);
+            else hipLaunchKernelGGL(k_place_tiles<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vd);
+            hipLaunchKernelGGL(k_scan_rows, dim3(F * P.K), dim3(1024), 0, s, P, B, 1, P.K);
+            hipLaunchKernelGGL(k_mol_scatter, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, 1, P.K);
+            hipLaunchKernelGGL(k_chunk_table, dim3(1), dim3(1024), 0, s, P, B, 1, P.K);
         } else {
-            if (f32) hipLaunchKernelGGL(k_eval_mols_rand<float>, dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const float*)xu_dev);
-            else hipLaunchKernelGGL(k_eval_mols_rand<double>, dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, (const double*)xu_dev);
+            if (f32) hipLaunchKernelGGL(k_place<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);
+            else hipLaunchKernelGGL(k_place<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vd);
         }
-        if (J->profiling) HIPCHK(hipEventRecord(er1, s));
+        HIPCHK(hipGetLastError());
+        if (J->profiling) HIPCHK(hipEventRecord(ev[2], s));
+        if (tiles) {
+            launch_tiles_c(J->tile_c, f32, true, s, J->eval_grid, shm, P, B, xu_dev, nullptr);
+        } else {
+            if (f32) hipLaunchKernelGGL(k_eval_mols_rand<float>, dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, uf);
+            else hipLaunchKernelGGL(k_eval_mols_rand<double>, dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, ud);
+        }
+        if (J->profiling) HIPCHK(hipEventRecord(ev[3], s));
     }
     HIPCHK(hipGetLastError());
     // geometry errors detected on the device
@@ -1124,32 +1676,30 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
             return fail(CM_EHIP, "internal error: cell grid larger than its allocation");
     }
     if (J->profiling) {
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, J->e1, J->e2));
+        float ms = 0.f, tot = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
         J->eval_ms += ms;
         J->eval_launches += 1;
-        float tot = 0.f;
         if (P.K > 0) {
-            HIPCHK(hipEventElapsedTime(&ms, er0, er1));
+            HIPCHK(hipEventElapsedTime(&ms, ev[2], ev[3]));
             J->eval_ms += ms;
             J->eval_launches += 1;
-            HIPCHK(hipEventElapsedTime(&tot, J->e0, er1));
+            HIPCHK(hipEventElapsedTime(&tot, J->e0, ev[3]));
         } else {
-            HIPCHK(hipEventElapsedTime(&tot, J->e0, J->e2));
+            HIPCHK(hipEventElapsedTime(&tot, J->e0, ev[1]));
         }
         J->total_ms += tot;
         J->total_launches += 1;
-        hipEventDestroy(er0);
-        hipEventDestroy(er1);
+        for (auto& e : ev) hipEventDestroy(e);
     }
     J->frames += F;
     return CM_OK;
 }
 
-int alloc_batch(cm_job* J, int F, int64_t cap, int dtype_size, bool host_input)
+int alloc_batch(cm_job* J, int F, int64_t cap, int64_t kcap, int dtype_size, bool host_input)
 {
     int r;
-    int64_t Mu = J->d.nmols_u, Mv = J->d.nmols_v;
+    int64_t Mu = J->d.nmols_u, Mv = J->d.nmols_v, NP = 1 + J->K;
     int64_t items = J->atoms_family ? J->d.natoms_per_mol_u : Mu;
     if ((r = ensure(J->geo, sizeof(Geo) * F))) return r;
     if ((r = ensure(J->cells_cnt, sizeof(int) * F * (cap + 1)))) return r;
@@ -1157,20 +1707,30 @@ int alloc_batch(cm_job* J, int F, int64_t cap, int dtype_size, bool host_input)
     if ((r = ensure(J->tmp4, 16 * F * items))) return r;
     if ((r = ensure(J->tmpcell, 4 * F * items))) return r;
     if ((r = ensure(J->sorted, 16 * F * items))) return r;
+    if ((r = ensure(J->vw, 16 * F * J->Nv))) return r;
     if (!J->atoms_family) {
         if ((r = ensure(J->solw, 16 * F * J->Nu))) return r;
         if ((r = ensure(J->ucen, 16 * F * Mu))) return r;
         if ((r = ensure(J->drawn_ptr, 4 * F * (Mu + 1)))) return r;
         if ((r = ensure(J->drawn_idx, 4 * F * std::max(J->K, 1)))) return r;
+        if ((r = ensure(J->vcell, 4 * F * Mv))) return r;
+        if ((r = ensure(J->vcen, 16 * F * Mv))) return r;
+    } else {
+        if ((r = ensure(J->near, (size_t)F * cap))) return r;
+        if ((r = ensure(J->mkey, (size_t)4 * F * NP * Mv))) return r;
+        if ((r = ensure(J->mcnt, (size_t)4 * F * NP * (kcap + 1)))) return r;
+        if ((r = ensure(J->mstart, (size_t)4 * F * NP * (kcap + 1)))) return r;
+        if ((r = ensure(J->msorted, (size_t)4 * F * NP * Mv))) return r;
+        if ((r = ensure(J->nnear, (size_t)4 * F * NP))) return r;
+        if ((r = ensure(J->chunk_base, (size_t)4 * (F * std::max(J->K, 1) + 1)))) return r;
     }
-    if ((r = ensure(J->vw, 16 * F * J->Nv))) return r;
-    if ((r = ensure(J->vcell, 4 * F * Mv))) return r;
-    if ((r = ensure(J->vcen, 16 * F * Mv))) return r;
     if (J->K > 0) {
         if ((r = ensure(J->P, (size_t)24 * F * J->K * J->Nv))) return r;
         if ((r = ensure(J->vw_r, (size_t)16 * F * J->K * J->Nv))) return r;
-        if ((r = ensure(J->vcell_r, (size_t)4 * F * J->K * Mv))) return r;
-        if ((r = ensure(J->vcen_r, (size_t)16 * F * J->K * Mv))) return r;
+        if (!J->atoms_family) {
+            if ((r = ensure(J->vcell_r, (size_t)4 * F * J->K * Mv))) return r;
+            if ((r = ensure(J->vcen_r, (size_t)16 * F * J->K * Mv))) return r;
+        }
         if ((r = ensure(J->bflag, (size_t)F * J->kb * Mv))) return r;
         if ((r = ensure(J->blist, (size_t)4 * F * J->kb * Mv))) return r;
         if ((r = ensure(J->nbulk, (size_t)4 * F * J->kb))) return r;
@@ -1210,12 +1770,24 @@ int submit_common(cm_job* J, int32_t nframes, const int64_t* frame_index, const 
         if (!(weights[f] > 0.0) || !std::isfinite(weights[f])) return fail(CM_EINVAL, "frame weights must be positive and finite");
         if (frame_index[f] < 1 || frame_index[f] > 0xFFFFFFFFll) return fail(CM_EINVAL, "frame_index must be in [1, 2^32)");
     }
-    int64_t pfb = per_frame_bytes(J, ds, host_input);
+    if (nframes == 0) return CM_OK;
+    // cell slots per frame: the largest grid of the submitted frames (with 10% headroom
+    // so that batches of later frames of a fluctuating box reuse the allocation)
+    int64_t capmax = 1, kcapmax = 1;
+    for (int f = 0; f < nframes; f++) {
+        int64_t kk;
+        capmax = std::max(capmax, cells_for(J, cells + 9 * f, &kk));
+        kcapmax = std::max(kcapmax, kk);
+    }
+    if (capmax > (int64_t)1 << 26 || kcapmax > (int64_t)1 << 27)
+        return fail(CM_EUNSUPPORTED, "unit cell too large for the cell grid");
+    capmax = capmax + capmax / 10 + 1;
+    if (!J->atoms_family) kcapmax = 1;
+    int64_t pfb = per_frame_bytes(J, ds, host_input, capmax, kcapmax);
     int64_t budget = J->d.memory_budget > 0 ? J->d.memory_budget : (int64_t)4 << 30;
     int64_t Fmax = std::max<int64_t>(1, budget / std::max<int64_t>(pfb, 1));
     if (J->d.max_batch_frames > 0) Fmax = std::min<int64_t>(Fmax, J->d.max_batch_frames);
     Fmax = std::min<int64_t>(Fmax, 64);
-    double cutsys = J->kp.cutsys;
     int f0 = 0;
     while (f0 < nframes) {
         // a batch holds frames of one weight (the weight multiplies the integer hits at flush)
@@ -1228,19 +1800,13 @@ int submit_common(cm_job* J, int32_t nframes, const int64_t* frame_index, const 
         }
         J->pending = true;
         J->pending_w = w;
-        int64_t cap = 1;
+        const int64_t cap = capmax;
         for (int f = 0; f < F; f++) {
             cm::Cell64 c;
             cm::cell_setup(cells + 9 * (f0 + f), J->d.triclinic, &c);
-            double sp[3];
-            plane_spacings(c.H, sp);
-            int64_t n = 1;
-            for (int k = 0; k < 3; k++) n *= std::max<int64_t>(1, (int64_t)std::floor(sp[k] / cutsys));
-            if (n > (int64_t)1 << 26) return fail(CM_EUNSUPPORTED, "unit cell too large for the cell grid");
-            cap = std::max(cap, n);
             J->volume_total += w * cm::cell_volume(c.H);
         }
-        int r = alloc_batch(J, F, cap, ds, host_input);
+        int r = alloc_batch(J, F, cap, kcapmax, ds, host_input);
         if (r) return r;
         const void* du;
         const void* dv;
@@ -1255,7 +1821,7 @@ int submit_common(cm_job* J, int32_t nframes, const int64_t* frame_index, const 
             du = (const char*)xu + (size_t)f0 * J->Nu * 3 * ds;
             dv = (const char*)xv + (size_t)f0 * J->Nv * 3 * ds;
         }
-        r = run_batch(J, F, frame_index + f0, du, dv, dtype, cells + 9 * f0, cap);
+        r = run_batch(J, F, frame_index + f0, du, dv, dtype, cells + 9 * f0, cap, kcapmax);
         if (r) return r;
         f0 += F;
     }
@@ -1296,7 +1862,8 @@ int cm_job_create(const cm_job_desc* desc, cm_job** out)
     J->d = d;
     J->Nu = Nu;
     J->Nv = Nv;
-    J->atoms_family = (d.nmols_u == 1 && !d.autocorrelation);
+    // tiles family: one solute molecule, solvent molecules that fit a wavefront
+    J->atoms_family = (d.nmols_u == 1 && !d.autocorrelation && d.natoms_per_mol_v <= 64);
     J->K = d.coordination_number_only ? 0 : d.n_random_samples;
     J->kb = J->K > 0 ? (J->atoms_family ? 1 : J->K) : 0;
     KP& P = J->kp;
@@ -1316,6 +1883,9 @@ int cm_job_create(const cm_job_desc* desc, cm_job** out)
     P.key1 = (uint32_t)(d.seed >> 32);
     P.Nu = Nu;
     P.Nv = Nv;
+    P.NP = 1 + J->K;
+    J->tile_c = 1;
+    P.mpt = (64 / J->tile_c) / P.nv;
     int r;
     auto bail = [&](int code) { cm_job_destroy(J); return code; };
     if (hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(CM_EHIP, "hipStreamCreate failed"));
@@ -1419,6 +1989,21 @@ int cm_job_set_profiling(cm_job* J, int32_t on)
         HIPCHK(hipEventCreate(&J->e2));
     }
     J->profiling = on != 0;
+    return CM_OK;
+}
+
+int cm_job_set_tuning(cm_job* J, double cell_fraction, int32_t columns, int32_t eval_workgroups)
+{
+    if (!J) return fail(CM_EINVAL, "null job");
+    if (J->frames > 0) return fail(CM_ESTATE, "tuning must be set before the first submit");
+    if (!(cell_fraction >= 0.2 && cell_fraction <= 1.0)) return fail(CM_EINVAL, "cell_fraction must be in [0.2, 1]");
+    if (columns != 1 && columns != 2 && columns != 4) return fail(CM_EINVAL, "columns must be 1, 2 or 4");
+    if ((64 / columns) / J->kp.nv < 1) return fail(CM_EINVAL, "too many columns for the solvent molecule size");
+    if (eval_workgroups < 0) return fail(CM_EINVAL, "eval_workgroups must be >= 0");
+    J->cell_frac = cell_fraction;
+    J->tile_c = columns;
+    J->kp.mpt = (64 / columns) / J->kp.nv;
+    J->eval_grid = eval_workgroups > 0 ? eval_workgroups : 4096;
     return CM_OK;
 }
 

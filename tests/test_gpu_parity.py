@@ -300,3 +300,37 @@ def test_exact_ties(pkg, oracle):
     b = gpu_raw(pkg, traj, opts)
     assert_raw_equal(a, b)
     assert b["solute_group_count"][0].sum() == 2 and b["solute_group_count"][1].sum() == 0
+
+
+@pytest.mark.parametrize("tuning", [dict(cell_fraction=0.5, columns=1), dict(cell_fraction=0.3, columns=2),
+                                    dict(cell_fraction=1.0, columns=4),
+                                    dict(cell_fraction=0.2, columns=1, eval_workgroups=7)])
+def test_tuning_does_not_change_results(pkg, oracle, tuning):
+    """Cell size, columns and grid of the single-solute kernels change the work
+    decomposition, never the records."""
+    s = syn().config1(nframes=2, n_solute=300, n_water=1200, box=30.0, seed=41)
+    traj = mem(pkg, s)
+    opts = pkg.Options(dbulk=9.0, n_random_samples=4, seed=6, silent=True)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts, job_kw={"tuning": tuning})
+    assert_raw_equal(a, b)
+
+
+def test_dense_solute_staging_rounds(pkg, oracle):
+    """A solute dense enough that a chunk's candidates exceed one LDS staging
+    round, and a cell holding more atoms than the staging buffer."""
+    S = syn()
+    rng = np.random.default_rng(5)
+    box = 30.0
+    clump = 15.0 + rng.uniform(-1.5, 1.5, size=(1500, 3))       # > 1024 atoms in one 10 A cell
+    shell = 15.0 + rng.normal(scale=4.0, size=(600, 3))
+    xu = np.concatenate([clump, shell])[None]
+    xv = S.place_molecules(rng, S.water_template(), rng.uniform(0, box, (900, 3)))[None]
+    su = pkg.AtomSelection(list(range(1, 2101)), nmols=1)
+    sv = pkg.AtomSelection(list(range(2101, 2101 + 2700)), natomspermol=3)
+    traj = pkg.MemoryTrajectory(xu, xv, np.diag([box] * 3)[None], su, sv)
+    opts = pkg.Options(dbulk=10.0, n_random_samples=3, seed=8, silent=True)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    for tuning in (dict(cell_fraction=1.0, columns=1), dict(cell_fraction=0.5, columns=4)):
+        b = gpu_raw(pkg, traj, opts, job_kw={"tuning": tuning})
+        assert_raw_equal(a, b)
