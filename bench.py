@@ -132,7 +132,7 @@ def main():
                    "frames_per_step": nf, "solute_atoms": n_u, "solvent_molecules": s["solvent"].nmols},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": None,
-                     "kernel": "k_eval_atoms (real + random passes)", "pairs_per_frame": round(W),
+                     "kernel": "k_eval_tiles (real + random passes)", "pairs_per_frame": round(W),
                      "eval_ms_per_frame": round(st["eval_ms"] / nf, 5),
                      "hbm_GBps_algorithmic": round(bpf * nf / eval_s / 1e9, 2),
                      "fp64_exhaustive_resolves": st["fp64_rechecks"]},

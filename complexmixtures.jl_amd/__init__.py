@@ -12,13 +12,15 @@ of include/cmgpu.h.
 """
 from .options import ArgumentError, Options, TrajectoryFileOptions
 from .selection import AtomSelection
-from .trajectory import MemoryTrajectory, NamdDCD, PDBTraj, Trajectory, convert_unitcell, unitcell_matrix
+from .trajectory import (MemoryTrajectory, NamdDCD, PDBTraj, Trajectory, convert_unitcell, unitcell_matrix,
+                         write_dcd)
 from .results import (Density, Result, Volume, coordination_number as _cn_of_result, finalresults, setbin,
                       shellradius, sphericalshellvolume, sum_frame_weights)
 from .mddf import TrajectoryMetaData, coordination_number, mddf
+from .persistence import load, results_equal, save
 from . import _lib
 
 __all__ = ["ArgumentError", "Options", "TrajectoryFileOptions", "AtomSelection", "MemoryTrajectory", "NamdDCD",
            "PDBTraj", "Trajectory", "convert_unitcell", "unitcell_matrix", "Density", "Result", "Volume",
            "finalresults", "setbin", "shellradius", "sphericalshellvolume", "sum_frame_weights",
-           "TrajectoryMetaData", "coordination_number", "mddf"]
+           "TrajectoryMetaData", "coordination_number", "mddf", "load", "save", "results_equal", "write_dcd"]

@@ -10,17 +10,21 @@
 //   k_wrap_atoms, k_scan_cells, k_scatter
 //                       solute atoms in a fine cell grid (counting sort + scan)
 //   k_near_mark         cells within cutoff + radius of some solute atom
-//   k_prep_solvent_tiles  solvent molecules made whole, kept when near
+//   k_prep_solvent_tiles  centre of every solvent molecule, key when near
 //   k_scan_rows, k_mol_scatter, k_chunk_table
-//                       near molecules sorted by cell per (frame, pass), cut
-//                       into chunks of whole molecules, one wavefront each
+//                       near molecules sorted by the Morton key of their cell
+//                       per (frame, pass), cut into chunks of whole molecules,
+//                       one wavefront each
+//   k_fill_real         FP32 atoms of the near molecules in sorted order
 //   k_eval_tiles        per chunk: AABB, solute atoms within the cutoff of it
 //                       staged in LDS, FP32 minimum per atom (LDS broadcast),
 //                       segmented minimum per molecule, exact FP64 resolve,
 //                       LDS histograms
 //   k_bulk_compact      ordered list of bulk molecules (inbulk, src/mddf.jl:55-57)
-//   k_place_tiles       Philox placement (src/rigid_body.jl:107-137) of the
-//                       slots whose new centre is near the solute
+//   k_place_keys, k_place_near
+//                       Philox placement (src/rigid_body.jl:107-137): centres
+//                       of all slots, then the full placement of the slots whose
+//                       centre is near the solute, in sorted order
 //   then sort, chunk and k_eval_tiles again for the random passes.
 //
 // "mols" -- many solute molecules (self MDDF, small solutes): molecule
@@ -62,6 +66,7 @@ constexpr int MAX_NBINS = 8192;
 struct Geo {                 // one frame of a batch
     cm::Cell64 cell;
     float H32[9];
+    float Hi32[9];
     double sp[3];            // plane spacings
     int nc[3];
     int ncell;
@@ -208,6 +213,20 @@ __device__ void mol_centre(const Geo& g, const T* x, int64_t a0, int n, double c
 // ---------------------------------------------------------------------------
 // radii and grid
 // ---------------------------------------------------------------------------
+// Radius bound of every molecule about its centre (molecule made whole around
+// its first atom), FP32 with a margin that covers the FP32 rounding.
+__device__ __forceinline__ void minimage32(const Geo& g, float dx, float dy, float dz, float& ox, float& oy, float& oz)
+{
+    const float* Hi = g.Hi32;
+    const float* H = g.H32;
+    float nx = rintf(Hi[0] * dx + Hi[3] * dy + Hi[6] * dz);
+    float ny = rintf(Hi[1] * dx + Hi[4] * dy + Hi[7] * dz);
+    float nz = rintf(Hi[2] * dx + Hi[5] * dy + Hi[8] * dz);
+    ox = dx - (H[0] * nx + H[3] * ny + H[6] * nz);
+    oy = dy - (H[1] * nx + H[4] * ny + H[7] * nz);
+    oz = dz - (H[2] * nx + H[5] * ny + H[8] * nz);
+}
+
 template <class T>
 __global__ void k_radii(KP P, Bufs B, const T* xu, const T* xv, int do_solute)
 {
@@ -221,19 +240,27 @@ __global__ void k_radii(KP P, Bufs B, const T* xu, const T* xv, int do_solute)
         const T* x = solv ? xv + (int64_t)f * P.Nv * 3 : xu + (int64_t)f * P.Nu * 3;
         int n = solv ? P.nv : P.nu;
         int64_t a0 = solv ? m * P.nv : (m - P.Mv) * P.nu;
-        double c[3];
-        mol_centre(g, x, a0, n, c);
         double x0, y0, z0;
         load3(x, a0, x0, y0, z0);
-        double r2 = 0.0;
+        float sx = 0.f, sy = 0.f, sz = 0.f;
         for (int a = 0; a < n; a++) {
-            double xa, ya, za, ox, oy, oz;
+            double xa, ya, za;
             load3(x, a0 + a, xa, ya, za);
-            cm::minimage(g.cell, xa - x0, ya - y0, za - z0, &ox, &oy, &oz);
-            double dx = x0 + ox - c[0], dy = y0 + oy - c[1], dz = z0 + oz - c[2];
-            r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+            float ox, oy, oz;
+            minimage32(g, (float)(xa - x0), (float)(ya - y0), (float)(za - z0), ox, oy, oz);
+            sx += ox; sy += oy; sz += oz;
         }
-        float r = (float)(sqrt(r2) * (1.0 + 1e-6) + 1e-4);
+        sx /= n; sy /= n; sz /= n;
+        float r2 = 0.f;
+        for (int a = 0; a < n; a++) {
+            double xa, ya, za;
+            load3(x, a0 + a, xa, ya, za);
+            float ox, oy, oz;
+            minimage32(g, (float)(xa - x0), (float)(ya - y0), (float)(za - z0), ox, oy, oz);
+            float dx = ox - sx, dy = oy - sy, dz = oz - sz;
+            r2 = fmaxf(r2, dx * dx + dy * dy + dz * dz);
+        }
+        float r = sqrtf(r2) * 1.0001f + 1e-3f;
         atomicMax(solv ? &g.rv_bits : &g.ru_bits, __float_as_uint(r));
     }
 }
@@ -855,8 +882,8 @@ __global__ void k_near_mark(KP P, Bufs B)
     }
 }
 
-// real pass: solvent molecules made whole, centre wrapped into the cell; near
-// ones get a key and their atoms in the frame of that cell
+// real pass: centre of every solvent molecule (made whole around its first
+// atom) wrapped into the cell; near molecules get the Morton key of the cell
 template <class T>
 __global__ void k_prep_solvent_tiles(KP P, Bufs B, const T* xv)
 {
@@ -865,10 +892,8 @@ __global__ void k_prep_solvent_tiles(KP P, Bufs B, const T* xv)
         int f = (int)(it / P.Mv);
         int v = (int)(it - (int64_t)f * P.Mv);
         const Geo& g = B.geo[f];
-        const T* x = xv + (int64_t)f * P.Nv * 3;
-        int64_t a0 = (int64_t)v * P.nv;
         double c[3];
-        mol_centre(g, x, a0, P.nv, c);
+        mol_centre(g, xv + (int64_t)f * P.Nv * 3, (int64_t)v * P.nv, P.nv, c);
         double fr[3], n[3];
         wrap_point(g, c[0], c[1], c[2], fr, n);
         int cell;
@@ -878,17 +903,40 @@ __global__ void k_prep_solvent_tiles(KP P, Bufs B, const T* xv)
         const int key = morton_key(g, cell);
         B.mkey[kb] = key;
         atomicAdd(&B.mcnt[(int64_t)f * P.NP * (P.kcap + 1) + key], 1);
+    }
+}
+
+// real pass: atoms of the near molecules in sorted order, FP32, in the frame of
+// the centre's cell
+template <class T>
+__global__ void k_fill_real(KP P, Bufs B, const T* xv)
+{
+    int64_t total = (int64_t)P.F * P.Mv;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int f = (int)(it / P.Mv);
+        int pos = (int)(it - (int64_t)f * P.Mv);
+        const int64_t fp = (int64_t)f * P.NP;
+        if (pos >= B.nnear[fp]) continue;
+        const int v = B.msorted[fp * P.Mv + pos];
+        const Geo& g = B.geo[f];
+        const T* x = xv + (int64_t)f * P.Nv * 3;
+        int64_t a0 = (int64_t)v * P.nv;
+        double c[3];
+        mol_centre(g, x, a0, P.nv, c);
+        double fr[3], n[3];
+        wrap_point(g, c[0], c[1], c[2], fr, n);
         const double* H = g.cell.H;
         double tx = H[0] * n[0] + H[3] * n[1] + H[6] * n[2];
         double ty = H[1] * n[0] + H[4] * n[1] + H[7] * n[2];
         double tz = H[2] * n[0] + H[5] * n[1] + H[8] * n[2];
         double x0, y0, z0;
         load3(x, a0, x0, y0, z0);
+        float4* dst = B.vw + (int64_t)f * P.Nv + (int64_t)pos * P.nv;
         for (int a = 0; a < P.nv; a++) {
             double xa, ya, za, ox, oy, oz;
             load3(x, a0 + a, xa, ya, za);
             cm::minimage(g.cell, xa - x0, ya - y0, za - z0, &ox, &oy, &oz);
-            B.vw[(int64_t)f * P.Nv + a0 + a] = mk4((float)(x0 + ox - tx), (float)(y0 + oy - ty), (float)(z0 + oz - tz), 0.f);
+            dst[a] = mk4((float)(x0 + ox - tx), (float)(y0 + oy - ty), (float)(z0 + oz - tz), 0.f);
         }
     }
 }
@@ -983,10 +1031,10 @@ __global__ void __launch_bounds__(1024) k_chunk_table(KP P, Bufs B, int p0, int 
     if (threadIdx.x == 0) B.chunk_base[n] = carry;
 }
 
-// randomize_solvent! (src/mddf.jl:65-88) for the slots whose new centre falls
-// in a near cell; the other slots cannot have an atom within the cutoff
-template <class TV>
-__global__ void k_place_tiles(KP P, Bufs B, const TV* xv)
+// randomize_solvent! (src/mddf.jl:65-88), first half: the new centre of every
+// slot; slots whose centre falls in a near cell get the Morton key of the cell
+// (the others cannot have an atom within the cutoff of the solute)
+__global__ void k_place_keys(KP P, Bufs B)
 {
     int64_t total = (int64_t)P.F * P.K * P.Mv;
     for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
@@ -997,25 +1045,44 @@ __global__ void k_place_tiles(KP P, Bufs B, const TV* xv)
         const Geo& g = B.geo[f];
         int64_t fp = (int64_t)f * P.NP + 1 + k;
         cm::u32x4 r = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT}, P.key0, P.key1);
-        double uc[3] = {cm::u01(r.y), cm::u01(r.z), cm::u01(r.w)};
         int cell;
-        cell_of_frac(g, uc[0], uc[1], uc[2], cell);
+        cell_of_frac(g, cm::u01(r.y), cm::u01(r.z), cm::u01(r.w), cell);
         if (!B.near[(int64_t)f * P.cap + cell]) { B.mkey[fp * P.Mv + v] = -1; continue; }
+        const int key = morton_key(g, cell);
+        B.mkey[fp * P.Mv + v] = key;
+        atomicAdd(&B.mcnt[fp * (P.kcap + 1) + key], 1);
+    }
+}
+
+// second half: the placement of the near slots, written in sorted order (FP64
+// for the exact resolve, FP32 for the evaluation)
+template <class TV>
+__global__ void k_place_near(KP P, Bufs B, const TV* xv)
+{
+    int64_t total = (int64_t)P.F * P.K * P.Mv;
+    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+        int64_t fk = it / P.Mv;
+        int pos = (int)(it - fk * P.Mv);
+        int f = (int)(fk / P.K);
+        int k = (int)(fk - (int64_t)f * P.K);
+        int64_t fp = (int64_t)f * P.NP + 1 + k;
+        if (pos >= B.nnear[fp]) continue;
+        const int v = B.msorted[fp * P.Mv + pos];
+        const Geo& g = B.geo[f];
+        cm::u32x4 r = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT}, P.key0, P.key1);
         cm::u32x4 r2 = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT + 1u}, P.key0, P.key1);
         int64_t lk = (int64_t)f * P.kb;
         int nb = B.nbulk[lk];
         int64_t jmol = nb > 0 ? (int64_t)B.blist[lk * P.Mv + cm::draw_index(r.x, nb)] : cm::draw_index(r.x, P.Mv);
         const TV* src = xv + (int64_t)f * P.Nv * 3 + jmol * P.nv * 3;
-        double* dst = B.P + (fk * P.Nv + (int64_t)v * P.nv) * 3;
+        double* dst = B.P + (fk * P.Nv + (int64_t)pos * P.nv) * 3;
         for (int q = 0; q < 3 * P.nv; q++) dst[q] = (double)src[q];
         MolAcc<TV> acc{dst};
+        double uc[3] = {cm::u01(r.y), cm::u01(r.z), cm::u01(r.w)};
         double ua[3] = {cm::u01(r2.x), cm::u01(r2.y), cm::u01(r2.z)};
         cm::random_move(acc, P.nv, P.iref, g.cell, uc, ua);
-        for (int a = 0; a < P.nv; a++)
-            B.vw_r[fk * P.Nv + (int64_t)v * P.nv + a] = mk4((float)dst[3 * a], (float)dst[3 * a + 1], (float)dst[3 * a + 2], 0.f);
-        const int key = morton_key(g, cell);
-        B.mkey[fp * P.Mv + v] = key;
-        atomicAdd(&B.mcnt[fp * (P.kcap + 1) + key], 1);
+        float4* dw = B.vw_r + fk * P.Nv + (int64_t)pos * P.nv;
+        for (int a = 0; a < P.nv; a++) dw[a] = mk4((float)dst[3 * a], (float)dst[3 * a + 1], (float)dst[3 * a + 2], 0.f);
     }
 }
 
@@ -1066,7 +1133,6 @@ template <class TU, class TV, bool RANDOM, int C>
 __global__ void __launch_bounds__(64) k_eval_tiles(KP P, Bufs B, const TU* xu, const TV* xv)
 {
     __shared__ float4 stg[TCAP];
-    __shared__ int nstg;
     extern __shared__ unsigned sh[];
     hist_clear(sh, 2 * P.nbins);
     const int lane = threadIdx.x;
@@ -1094,7 +1160,8 @@ __global__ void __launch_bounds__(64) k_eval_tiles(KP P, Bufs B, const TU* xu, c
         const bool active = ml < nm;
         const int v = active ? B.msorted[fp * P.Mv + m0 + ml] : 0;
         const float4* vwb = RANDOM ? B.vw_r + ((int64_t)f * P.K + pidx) * P.Nv : B.vw + (int64_t)f * P.Nv;
-        const float4 y = active ? vwb[(int64_t)v * P.nv + a] : mk4(0.f, 0.f, 0.f, 0.f);
+        const int64_t pbase = (int64_t)(m0 + ml) * P.nv;   // sorted position of the molecule
+        const float4 y = active ? vwb[pbase + a] : mk4(0.f, 0.f, 0.f, 0.f);
         const float lx = wave_min(active ? y.x : INFINITY), ly = wave_min(active ? y.y : INFINITY),
                     lz = wave_min(active ? y.z : INFINITY);
         const float hx = wave_max(active ? y.x : -INFINITY), hy = wave_max(active ? y.y : -INFINITY),
@@ -1109,21 +1176,10 @@ __global__ void __launch_bounds__(64) k_eval_tiles(KP P, Bufs B, const TU* xu, c
         Best bb;
         bb.init();
         int rounds = 0;
-        if (lane == 0) nstg = 0;
-        __syncthreads();
-        auto stage_atom = [&](float4 s, float tx, float ty, float tz) {
-            float qx = s.x + tx, qy = s.y + ty, qz = s.z + tz;
-            float dx = fmaxf(fmaxf(lx - qx, qx - hx), 0.f);
-            float dy = fmaxf(fmaxf(ly - qy, qy - hy), 0.f);
-            float dz = fmaxf(fmaxf(lz - qz, qz - hz), 0.f);
-            if (dx * dx + dy * dy + dz * dz <= rs2) {
-                int slot = atomicAdd(&nstg, 1);
-                stg[slot] = mk4(qx, qy, qz, s.w);
-            }
-        };
-        auto evaluate = [&]() {
+        int nst = 0;   // staged atoms (uniform)
+        const unsigned long long lt_mask = (1ull << lane) - 1ull;
+        auto evaluate = [&](int n) {
             __syncthreads();
-            const int n = nstg;
             int k = col;
             for (; k + 3 * C < n; k += 4 * C) {
                 float4 s0 = stg[k], s1 = stg[k + C], s2 = stg[k + 2 * C], s3 = stg[k + 3 * C];
@@ -1142,11 +1198,12 @@ __global__ void __launch_bounds__(64) k_eval_tiles(KP P, Bufs B, const TU* xu, c
                 float dx = y.x - s0.x, dy = y.y - s0.y, dz = y.z - s0.z;
                 bb.offer(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), __float_as_int(s0.w), 0);
             }
-            rounds++;
-            __syncthreads();
-            if (lane == 0) nstg = 0;
             __syncthreads();
         };
+        // Stage the solute atoms of the cells in range, 64 cells at a time: the
+        // atoms of those cells are ranked (wave scan of the counts), each lane
+        // loads the atoms of its ranks, and the ones within rs of the AABB are
+        // appended at ballot/popcount offsets.
         for (int cb = 0; cb < ncl; cb += 64) {
             const int ci = cb + lane;
             int cstart = 0, cnt = 0;
@@ -1160,38 +1217,41 @@ __global__ void __launch_bounds__(64) k_eval_tiles(KP P, Bufs B, const TU* xu, c
                 cnt = st[cell + 1] - cstart;
                 shift_of(g, mx, my, mz, tx, ty, tz);
             }
-            bool done = cnt == 0;
-            while (true) {
-                const int pend = done ? 0 : cnt;
-                const int incl = wave_incl_scan(pend, lane);
-                const int tot = __shfl(incl, 63, 64);
-                if (tot == 0) break;
-                const int room = TCAP - nstg;
-                const bool take = !done && incl <= room;
-                if (__ballot(take) == 0ull) {
-                    if (nstg > 0) { evaluate(); continue; }
-                    // a single cell larger than the staging buffer: in pieces
-                    const int L = __ffsll((long long)__ballot(pend > 0)) - 1;
-                    const int s0 = __shfl(cstart, L, 64), c0 = __shfl(cnt, L, 64);
-                    const float t0 = __shfl(tx, L, 64), t1 = __shfl(ty, L, 64), t2 = __shfl(tz, L, 64);
-                    for (int off = 0; off < c0; off += TCAP) {
-                        const int nq = min(TCAP, c0 - off);
-                        for (int q = lane; q < nq; q += 64) stage_atom(srt[s0 + off + q], t0, t1, t2);
-                        evaluate();
-                    }
-                    if (lane == L) done = true;
-                    continue;
+            const int incl = wave_incl_scan(cnt, lane);
+            const int excl = incl - cnt;
+            const int T = __shfl(incl, 63, 64);
+            for (int t0 = 0; t0 < T; t0 += 64) {
+                if (nst + 64 > TCAP) {
+                    evaluate(nst);
+                    rounds++;
+                    nst = 0;
                 }
-                if (take) {
-                    for (int q = 0; q < cnt; q++) stage_atom(srt[cstart + q], tx, ty, tz);
-                    done = true;
+                const int t = t0 + lane;
+                int j = 0;   // owner cell: largest j with excl[j] <= t
+                for (int step = 32; step >= 1; step >>= 1) {
+                    const int e = __shfl(excl, (j + step) & 63, 64);
+                    if (j + step < 64 && e <= t) j += step;
                 }
-                __syncthreads();
+                const int sj = __shfl(cstart, j, 64), ej = __shfl(excl, j, 64);
+                const float txj = __shfl(tx, j, 64), tyj = __shfl(ty, j, 64), tzj = __shfl(tz, j, 64);
+                bool pass = false;
+                float4 q = mk4(0.f, 0.f, 0.f, 0.f);
+                if (t < T) {
+                    const float4 sa = srt[sj + (t - ej)];
+                    q = mk4(sa.x + txj, sa.y + tyj, sa.z + tzj, sa.w);
+                    const float dx = fmaxf(fmaxf(lx - q.x, q.x - hx), 0.f);
+                    const float dy = fmaxf(fmaxf(ly - q.y, q.y - hy), 0.f);
+                    const float dz = fmaxf(fmaxf(lz - q.z, q.z - hz), 0.f);
+                    pass = dx * dx + dy * dy + dz * dz <= rs2;
+                }
+                const unsigned long long m = __ballot(pass);
+                if (pass) stg[nst + __popcll(m & lt_mask)] = q;
+                nst += __popcll(m);
             }
         }
         const bool one_round = rounds == 0;
-        const int last_n = nstg;   // the last staged batch stays in LDS until the next chunk
-        if (last_n > 0) evaluate();
+        const int last_n = nst;   // the last staged batch stays in LDS until the next chunk
+        if (last_n > 0) evaluate(last_n);
         // columns of a row -> the row's atom
         for (int o = 1; o < C; o <<= 1) {
             float b2 = __shfl_xor(bb.best, o, 64), s2 = __shfl_xor(bb.second, o, 64);
@@ -1222,7 +1282,7 @@ __global__ void __launch_bounds__(64) k_eval_tiles(KP P, Bufs B, const TU* xu, c
         const int64_t jbase = (int64_t)v * P.nv;
         auto ysrc = [&](int q, double& y0, double& y1, double& y2) {
             if (RANDOM) {
-                const double* pp = B.P + (((int64_t)f * P.K + pidx) * P.Nv + jbase + q) * 3;
+                const double* pp = B.P + (((int64_t)f * P.K + pidx) * P.Nv + pbase + q) * 3;
                 y0 = pp[0]; y1 = pp[1]; y2 = pp[2];
             } else load3(xv + (int64_t)f * P.Nv * 3, jbase + q, y0, y1, y2);
         };
@@ -1258,13 +1318,14 @@ __global__ void __launch_bounds__(64) k_eval_tiles(KP P, Bufs B, const TU* xu, c
             const int mlL = (L / C) / P.nv;
             const int vL = __shfl(v, L, 64);
             const int64_t jb = (int64_t)vL * P.nv;
+            const int64_t pb = (int64_t)(m0 + mlL) * P.nv;
             Rec loc = rec_zero();
             for (int q = 0; q < P.nv; q++) {
                 const int sl = ((mlL * P.nv + q) * C) & 63;
                 const float yx = __shfl(y.x, sl, 64), yy = __shfl(y.y, sl, 64), yz = __shfl(y.z, sl, 64);
                 double Y0, Y1, Y2;
                 if (RANDOM) {
-                    const double* pp = B.P + (((int64_t)f * P.K + pidx) * P.Nv + jb + q) * 3;
+                    const double* pp = B.P + (((int64_t)f * P.K + pidx) * P.Nv + pb + q) * 3;
                     Y0 = pp[0]; Y1 = pp[1]; Y2 = pp[2];
                 } else load3(xv + (int64_t)f * P.Nv * 3, jb + q, Y0, Y1, Y2);
                 const bool isref = q == P.iref;
@@ -1493,7 +1554,10 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         Geo& g = J->hgeo[f];
         memset(&g, 0, sizeof(Geo));
         cm::cell_setup(cells + 9 * f, J->d.triclinic, &g.cell);
-        for (int k = 0; k < 9; k++) g.H32[k] = (float)g.cell.H[k];
+        for (int k = 0; k < 9; k++) {
+            g.H32[k] = (float)g.cell.H[k];
+            g.Hi32[k] = (float)g.cell.Hi[k];
+        }
         plane_spacings(g.cell.H, g.sp);
         double spmin = std::min(g.sp[0], std::min(g.sp[1], g.sp[2]));
         if (!(cutsys < 0.5 * spmin * (1.0 - 1e-9)))
@@ -1621,6 +1685,8 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         else hipLaunchKernelGGL(k_prep_solvent_tiles<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vd);
         hipLaunchKernelGGL(k_scan_rows, dim3(F), dim3(1024), 0, s, P, B, 0, 1);
         hipLaunchKernelGGL(k_mol_scatter, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, 0, 1);
+        if (f32) hipLaunchKernelGGL(k_fill_real<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);
+        else hipLaunchKernelGGL(k_fill_real<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vd);
         hipLaunchKernelGGL(k_chunk_table, dim3(1), dim3(1024), 0, s, P, B, 0, 1);
     } else {
         int64_t n = (int64_t)F * P.Mv;
@@ -1645,10 +1711,11 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         hipLaunchKernelGGL(k_bulk_compact, dim3(F * P.kb), dim3(1024), 0, s, P, B);
         int64_t n = (int64_t)F * P.K * P.Mv;
         if (tiles) {
-            if (f32) hipLaunchKernelGGL(k_place_tiles<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);
-            else hipLaunchKernelGGL(k_place_tiles<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vd);
+            hipLaunchKernelGGL(k_place_keys, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B);
             hipLaunchKernelGGL(k_scan_rows, dim3(F * P.K), dim3(1024), 0, s, P, B, 1, P.K);
             hipLaunchKernelGGL(k_mol_scatter, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, 1, P.K);
+            if (f32) hipLaunchKernelGGL(k_place_near<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);
+            else hipLaunchKernelGGL(k_place_near<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vd);
             hipLaunchKernelGGL(k_chunk_table, dim3(1), dim3(1024), 0, s, P, B, 1, P.K);
         } else {
             if (f32) hipLaunchKernelGGL(k_place<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);

@@ -304,6 +304,52 @@ class MemoryTrajectory(TrajectoryBase):
         return self.xu[start:start + count], self.xv[start:start + count], self.cells[start:start + count]
 
 
+def _cell_params(m):
+    """Lengths and angles (degrees) of a cell matrix with lattice vectors as columns."""
+    m = np.asarray(m, dtype=np.float64)
+    a, b, c = (np.linalg.norm(m[:, k]) for k in range(3))
+    ang = lambda u, v: math.degrees(math.acos(max(-1.0, min(1.0, float(u @ v) / (np.linalg.norm(u) * np.linalg.norm(v))))))
+    return a, b, c, ang(m[:, 1], m[:, 2]), ang(m[:, 0], m[:, 2]), ang(m[:, 0], m[:, 1])
+
+
+def write_dcd(filename, xyz, unitcells):
+    """Writes a NAMD-style DCD (FP32 coordinates, unit-cell record
+    [A, gamma, B, beta, alpha, C] per frame) readable by NamdDCD and by the
+    reference's reader (src/trajectory_formats/NamdDCD.jl:53-229).
+    xyz: [nframes][natoms][3]; unitcells: [nframes][3][3] or [nframes][3]."""
+    xyz = np.asarray(xyz)
+    nf, na = xyz.shape[0], xyz.shape[1]
+    uc = np.asarray(unitcells, dtype=np.float64)
+    if uc.shape == (nf, 3):
+        uc = np.stack([np.diag(r) for r in uc])
+
+    def rec(fh, payload):
+        fh.write(struct.pack("<i", len(payload)))
+        fh.write(payload)
+        fh.write(struct.pack("<i", len(payload)))
+
+    with open(filename, "wb") as fh:
+        icntrl = [0] * 20
+        icntrl[0] = nf          # number of frames
+        icntrl[1] = 1           # first step
+        icntrl[2] = 1           # step interval
+        icntrl[3] = nf
+        icntrl[10] = 1          # unit cell present
+        icntrl[19] = 24         # CHARMM version (NAMD writes 24)
+        hdr = b"CORD" + struct.pack("<9i", *icntrl[:9]) + struct.pack("<f", 1.0) + struct.pack("<10i", *icntrl[10:])
+        rec(fh, hdr)
+        title = b"REMARKS written by complexmixtures_amd".ljust(80)
+        rec(fh, struct.pack("<i", 1) + title)
+        rec(fh, struct.pack("<i", na))
+        for f in range(nf):
+            a, b, c, al, be, ga = _cell_params(uc[f])
+            if abs(al - 90.0) < 1e-9 and abs(be - 90.0) < 1e-9 and abs(ga - 90.0) < 1e-9:
+                al = be = ga = 90.0
+            rec(fh, struct.pack("<6d", a, ga, b, be, al, c))
+            for k in range(3):
+                rec(fh, np.ascontiguousarray(xyz[f, :, k], dtype="<f4").tobytes())
+
+
 def Trajectory(filename, solute, solvent=None, *, format="", chemfiles=False, show_progress=False, lastframe=-1):
     """src/Trajectory.jl:31-63"""
     solvent = solute if solvent is None else solvent

@@ -223,3 +223,73 @@ def test_synthetic_geometry(pkg):
     np.testing.assert_allclose(d, 1.5)
     u = S.urea_template()
     assert u.shape == (8, 3) and np.allclose(u[:, 2], 0.0)
+
+
+@pytest.mark.skipif(not os.path.isdir(REFERENCE), reason="reference checkout not present")
+def test_json_load_save_reference_files(pkg, tmp_path):
+    """src/results.jl:590-621: the reference's Result files load, round-trip
+    through save/load unchanged, and the version gate rejects too new / too old
+    files and broken content."""
+    data = os.path.join(REFERENCE, "test", "data")
+    for rel in ("NAMD/tmao_tmao.json", "NAMD/water_tmao.json", "NAMD/water_water.json", "PDB/tmao_tmao.json",
+                "Gromacs/EMI_DCA.json", "Gromacs/EMI_EMI.json"):
+        R = pkg.load(os.path.join(data, rel))
+        f = tmp_path / "r.json"
+        pkg.save(str(f), R)
+        assert pkg.results_equal(R, pkg.load(str(f)))
+        pkg.save(R, str(f))
+        assert pkg.results_equal(R, pkg.load(str(f)))
+        assert open(f).read().startswith('{"Version":"2.18.0-DEV","nbins":500,')
+    with pytest.warns(UserWarning):
+        L = pkg.load(os.path.join(data, "legacy", "tmao_tmao.json"))
+    assert L.md_count.shape == (500,)
+    wv = os.path.join(data, "legacy", "wrong_version_jsons")
+    with pytest.raises(pkg.ArgumentError, match="newer version"):
+        pkg.load(os.path.join(wv, "too_new.json"))
+    with pytest.raises(pkg.ArgumentError, match="older"):
+        pkg.load(os.path.join(wv, "too_old.json"))
+    with pytest.raises(pkg.ArgumentError, match="Result object"):
+        pkg.load(os.path.join(wv, "broken_json_2.18.0.json"))
+    f = tmp_path / "v.json"
+    f.write_text('{"Version":"2.18.3-DEV"}')
+    with pytest.raises(pkg.ArgumentError):
+        pkg.load(str(f))
+
+
+def test_json_round_trip_of_a_fresh_result(pkg, tmp_path):
+    from complexmixtures_amd.results import make_result
+    t = _dup_traj(pkg)
+    R = make_result(t, pkg.Options(silent=True), pkg.TrajectoryMetaData(t, pkg.Options(silent=True)))
+    R.md_count[:] = np.linspace(0, 1, 500)
+    R.kb[3] = np.inf
+    R.mddf[4] = np.nan
+    f = tmp_path / "x.json"
+    pkg.save(str(f), R)
+    R2 = pkg.load(str(f))
+    assert pkg.results_equal(R, R2)
+    assert np.isinf(R2.kb[3]) and np.isnan(R2.mddf[4])
+
+
+def test_dcd_writer_round_trip(pkg, tmp_path):
+    """Synthetic trajectories written as NAMD DCD read back exactly (FP32)."""
+    import complexmixtures_amd.synthetic as S
+    s = S.config3(nframes=3, n_solute=50, n_urea=40, n_water=5, seed=2, a=40.0)
+    xyz = np.concatenate([s["x_solute"], s["x_solvent"]], axis=1)
+    f = tmp_path / "t.dcd"
+    pkg.write_dcd(str(f), xyz, s["cells"])
+    t = pkg.Trajectory(str(f), s["solute"], s["solvent"])
+    assert t.nframes == 3
+    t.opentraj()
+    t.firstframe()
+    for k in range(3):
+        t.nextframe()
+        np.testing.assert_array_equal(t.x_solute, s["x_solute"][k].astype(np.float32).astype(np.float64))
+        np.testing.assert_array_equal(t.x_solvent, s["x_solvent"][k].astype(np.float32).astype(np.float64))
+        np.testing.assert_allclose(t.getunitcell(), s["cells"][k], atol=1e-10)
+    t.closetraj()
+    box = S.config1(nframes=2, n_solute=20, n_water=10, box=25.0, seed=3)
+    pkg.write_dcd(str(f), np.concatenate([box["x_solute"], box["x_solvent"]], axis=1), box["cells"])
+    t = pkg.Trajectory(str(f), box["solute"], box["solvent"])
+    t.opentraj(); t.firstframe(); t.nextframe()
+    np.testing.assert_array_equal(t.getunitcell(), np.diag([25.0] * 3))
+    t.closetraj()
