@@ -68,11 +68,25 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=0, help="CPU baseline sample (0: one frame per thread)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain read-N/write-N elementwise kernel over MIB MiB three times (the "
+                         "reference for scaling FETCH_SIZE/WRITE_SIZE counters), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; the path shards by frames across GPUs "
-                                   "(see DESIGN.md) but multi-GPU runs are not part of it"}))
+        print(json.dumps({"error": "this benchmark measures one GPU; the path shards by frames across GPUs "
+                                   "but multi-GPU runs are not part of it"}))
         sys.exit(2)
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * 2 ** 20 // 4
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 3}))
+        return
     pkg = G.load_package()
     import complexmixtures_amd.synthetic as S
     from complexmixtures_amd import _lib
@@ -141,7 +155,7 @@ def main():
     if not args.no_cpu:
         O = G.load_oracle()
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        f = args.cpu_frames or threads
+        f = args.cpu_frames or min(nf, 10 * threads)
         v, dt = cpu_baseline(pkg, O, s, opts, tm, f, threads)
         out["cpu_baseline"] = {"value": round(v, 4), "unit": "frames/s", "cores": threads, "kind": "port",
                                "sample": f"first {f} frames of the same trajectory, {dt:.1f} s",
