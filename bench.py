@@ -68,6 +68,7 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=0, help="CPU baseline sample (0: one frame per thread)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep", action="store_true", help="only time the evaluation kernels per tuning, then exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain read-N/write-N elementwise kernel over MIB MiB three times (the "
                          "reference for scaling FETCH_SIZE/WRITE_SIZE counters), then exit")
@@ -118,6 +119,26 @@ def main():
         job.close()
         return raw, st
 
+    if args.sweep:
+        # eval-kernel time per tuning over the same frames (results are the
+        # same for every tuning; only the work decomposition changes)
+        base = None
+        for cf in (0.25, 0.35, 0.5, 0.75, 1.0):
+            for c in (1, 2, 4, 8):
+                job = _job_for(R, opts, tm, False, tuning=dict(cell_fraction=cf, columns=c))
+                job.set_profiling(True)
+                t1 = time.perf_counter()
+                job.submit_device(frames, du, dv, 0, cells, weights)
+                raw = job.finish()
+                dt = time.perf_counter() - t1
+                st = job.stats()
+                job.close()
+                same = base is None or all(np.array_equal(raw[k], base[k]) for k in raw if k != "volume_total")
+                base = base or raw
+                print(json.dumps({"cell_fraction": cf, "columns": c, "eval_ms": round(st["eval_ms"], 3),
+                                  "total_ms": round(st["total_ms"], 3), "wall_ms": round(1e3 * dt, 3),
+                                  "fp64_rechecks": st["fp64_rechecks"], "same_counts": same}), flush=True)
+        return
     for _ in range(args.warmup):
         one_job()
     _lib.check(_lib.lib().cm_device_synchronize())

@@ -196,7 +196,7 @@ class Job:
     def set_profiling(self, on=True):
         check(lib().cm_job_set_profiling(self.h, int(on)))
 
-    def set_tuning(self, cell_fraction=0.5, columns=1, eval_workgroups=0):
+    def set_tuning(self, cell_fraction=0.5, columns=0, eval_workgroups=0):
         check(lib().cm_job_set_tuning(self.h, float(cell_fraction), int(columns), int(eval_workgroups)))
 
     def stats(self):

@@ -165,33 +165,39 @@ CM_HD int64_t setbin(double d, double step)
     return ib < 1 ? 1 : ib;
 }
 
-// random_move! of src/rigid_body.jl:107-137 on one molecule (x: n atoms, xyz
-// interleaved, modified in place): wrap relative to the reference atom,
-// rotate by eulermat about the centre of coordinates (move!,
-// src/rigid_body.jl:73-80), translate the centre to H*uc.
-template <class Acc>
-CM_HD void random_move(Acc& x, int n, int iref, const Cell64& c, const double uc[3], const double ua[3])
+// random_move! of src/rigid_body.jl:107-137 on one molecule: wrap relative
+// to the reference atom, rotate by eulermat about the centre of coordinates
+// (move!, src/rigid_body.jl:73-80), translate the centre to H*uc.  x(a, k)
+// reads the source atoms, out(a, x, y, z) receives the placed ones; the
+// wrapped positions are formed twice (for the centre, then for the
+// placement) with the same operations, so nothing intermediate is stored.
+// In place (out writing where x reads) is allowed: atom a is read before it
+// is written and the reference atom is read first.
+template <class Src, class Dst>
+CM_HD void random_move(const Src& x, Dst& out, int n, int iref, const Cell64& c, const double uc[3],
+                       const double ua[3])
 {
-    double rx = x(iref, 0), ry = x(iref, 1), rz = x(iref, 2);
+    const double rx = x(iref, 0), ry = x(iref, 1), rz = x(iref, 2);
+    double cx = 0.0, cy = 0.0, cz = 0.0;
     for (int a = 0; a < n; a++) {
         double ox, oy, oz;
         minimage(c, x(a, 0) - rx, x(a, 1) - ry, x(a, 2) - rz, &ox, &oy, &oz);
-        x.set(a, rx + ox, ry + oy, rz + oz);
+        cx += rx + ox; cy += ry + oy; cz += rz + oz;
     }
+    cx = cx / (double)n; cy = cy / (double)n; cz = cz / (double)n;
     const double* H = c.H;
-    double ncm0 = H[0] * uc[0] + H[3] * uc[1] + H[6] * uc[2];
-    double ncm1 = H[1] * uc[0] + H[4] * uc[1] + H[7] * uc[2];
-    double ncm2 = H[2] * uc[0] + H[5] * uc[1] + H[8] * uc[2];
+    const double ncm0 = H[0] * uc[0] + H[3] * uc[1] + H[6] * uc[2];
+    const double ncm1 = H[1] * uc[0] + H[4] * uc[1] + H[7] * uc[2];
+    const double ncm2 = H[2] * uc[0] + H[5] * uc[1] + H[8] * uc[2];
     double A[9];
     eulermat_turns(ua[0], ua[1], ua[2], A);
-    double cx = 0.0, cy = 0.0, cz = 0.0;
-    for (int a = 0; a < n; a++) { cx += x(a, 0); cy += x(a, 1); cz += x(a, 2); }
-    cx = cx / (double)n; cy = cy / (double)n; cz = cz / (double)n;
     for (int a = 0; a < n; a++) {
-        double dx = x(a, 0) - cx, dy = x(a, 1) - cy, dz = x(a, 2) - cz;
-        x.set(a, A[0] * dx + A[1] * dy + A[2] * dz + ncm0,
-                 A[3] * dx + A[4] * dy + A[5] * dz + ncm1,
-                 A[6] * dx + A[7] * dy + A[8] * dz + ncm2);
+        double ox, oy, oz;
+        minimage(c, x(a, 0) - rx, x(a, 1) - ry, x(a, 2) - rz, &ox, &oy, &oz);
+        const double dx = (rx + ox) - cx, dy = (ry + oy) - cy, dz = (rz + oz) - cz;
+        out(a, A[0] * dx + A[1] * dy + A[2] * dz + ncm0,
+               A[3] * dx + A[4] * dy + A[5] * dz + ncm1,
+               A[6] * dx + A[7] * dy + A[8] * dz + ncm2);
     }
 }
 

@@ -227,19 +227,34 @@ __device__ __forceinline__ void minimage32(const Geo& g, float dx, float dy, flo
     oz = dz - (H[2] * nx + H[5] * ny + H[8] * nz);
 }
 
+// maximum over the threads of a block (blockDim.x a multiple of 64, at most 1024)
+__device__ __forceinline__ float block_max(float x)
+{
+    __shared__ float part[16];
+    for (int o = 32; o >= 1; o >>= 1) x = fmaxf(x, __shfl_xor(x, o, 64));
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) part[w] = x;
+    __syncthreads();
+    float r = part[0];
+    for (int k = 1; k < nw; k++) r = fmaxf(r, part[k]);
+    return r;
+}
+
+// molecule radii (bound on the distance of an atom from the centre of its
+// molecule, with a margin over FP32 rounding); blockIdx.y = frame
 template <class T>
 __global__ void k_radii(KP P, Bufs B, const T* xu, const T* xv, int do_solute)
 {
-    int64_t nmol = (int64_t)P.Mv + (do_solute ? P.Mu : 0);
-    int64_t total = (int64_t)P.F * nmol;
-    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
-        int f = (int)(it / nmol);
-        int64_t m = it - (int64_t)f * nmol;
-        Geo& g = B.geo[f];
-        bool solv = m < P.Mv;
+    const int f = blockIdx.y;
+    Geo& g = B.geo[f];
+    const int64_t nmol = (int64_t)P.Mv + (do_solute ? P.Mu : 0);
+    float rv = 0.f, ru = 0.f;
+    for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < nmol; m += (int64_t)gridDim.x * blockDim.x) {
+        const bool solv = m < P.Mv;
         const T* x = solv ? xv + (int64_t)f * P.Nv * 3 : xu + (int64_t)f * P.Nu * 3;
-        int n = solv ? P.nv : P.nu;
-        int64_t a0 = solv ? m * P.nv : (m - P.Mv) * P.nu;
+        const int n = solv ? P.nv : P.nu;
+        const int64_t a0 = solv ? m * P.nv : (m - P.Mv) * P.nu;
         double x0, y0, z0;
         load3(x, a0, x0, y0, z0);
         float sx = 0.f, sy = 0.f, sz = 0.f;
@@ -260,8 +275,14 @@ __global__ void k_radii(KP P, Bufs B, const T* xu, const T* xv, int do_solute)
             float dx = ox - sx, dy = oy - sy, dz = oz - sz;
             r2 = fmaxf(r2, dx * dx + dy * dy + dz * dz);
         }
-        float r = sqrtf(r2) * 1.0001f + 1e-3f;
-        atomicMax(solv ? &g.rv_bits : &g.ru_bits, __float_as_uint(r));
+        const float r = sqrtf(r2) * 1.0001f + 1e-3f;
+        if (solv) rv = fmaxf(rv, r); else ru = fmaxf(ru, r);
+    }
+    rv = block_max(rv);
+    ru = block_max(ru);
+    if (threadIdx.x == 0) {
+        if (rv > 0.f) atomicMax(&g.rv_bits, __float_as_uint(rv));
+        if (ru > 0.f) atomicMax(&g.ru_bits, __float_as_uint(ru));
     }
 }
 
@@ -760,10 +781,19 @@ __global__ void __launch_bounds__(1024) k_bulk_compact(KP P, Bufs B)
 }
 
 template <class TV>
-struct MolAcc {   // one molecule of the placement buffer, xyz interleaved
+struct MolSrc {   // atoms of one source molecule, xyz interleaved
+    const TV* p;
+    __device__ double operator()(int a, int k) const { return (double)p[3 * a + k]; }
+};
+
+struct MolDst {   // placed atoms: FP64 (exact resolve) and FP32 (evaluation)
     double* p;
-    __device__ double operator()(int a, int k) const { return p[3 * a + k]; }
-    __device__ void set(int a, double x, double y, double z) { p[3 * a] = x; p[3 * a + 1] = y; p[3 * a + 2] = z; }
+    float4* w;
+    __device__ void operator()(int a, double x, double y, double z) const
+    {
+        p[3 * a] = x; p[3 * a + 1] = y; p[3 * a + 2] = z;
+        w[a] = mk4((float)x, (float)y, (float)z, 0.f);
+    }
 };
 
 // randomize_solvent!, src/mddf.jl:65-88, for every slot of every random pass
@@ -782,13 +812,11 @@ __global__ void k_place(KP P, Bufs B, const TV* xv)
         int64_t lk = (int64_t)f * P.kb + (P.kb == 1 ? 0 : k);
         int nb = B.nbulk[lk];
         int64_t jmol = nb > 0 ? (int64_t)B.blist[lk * P.Mv + cm::draw_index(r.x, nb)] : cm::draw_index(r.x, P.Mv);
-        const TV* src = xv + (int64_t)f * P.Nv * 3 + jmol * P.nv * 3;
-        double* dst = B.P + (fk * P.Nv + (int64_t)v * P.nv) * 3;
-        for (int q = 0; q < 3 * P.nv; q++) dst[q] = (double)src[q];
-        MolAcc<TV> acc{dst};
+        MolSrc<TV> src{xv + (int64_t)f * P.Nv * 3 + jmol * P.nv * 3};
+        MolDst dst{B.P + (fk * P.Nv + (int64_t)v * P.nv) * 3, B.vw_r + fk * P.Nv + (int64_t)v * P.nv};
         double uc[3] = {cm::u01(r.y), cm::u01(r.z), cm::u01(r.w)};
         double ua[3] = {cm::u01(r2.x), cm::u01(r2.y), cm::u01(r2.z)};
-        cm::random_move(acc, P.nv, P.iref, g.cell, uc, ua);
+        cm::random_move(src, dst, P.nv, P.iref, g.cell, uc, ua);
         // the new centre H*uc lies inside the cell: its cell follows from uc
         int cell;
         cell_of_frac(g, uc[0], uc[1], uc[2], cell);
@@ -798,8 +826,6 @@ __global__ void k_place(KP P, Bufs B, const TV* xv)
         double c1 = H[1] * uc[0] + H[4] * uc[1] + H[7] * uc[2];
         double c2 = H[2] * uc[0] + H[5] * uc[1] + H[8] * uc[2];
         B.vcen_r[fk * P.Mv + v] = mk4((float)c0, (float)c1, (float)c2, 0.f);
-        for (int a = 0; a < P.nv; a++)
-            B.vw_r[fk * P.Nv + (int64_t)v * P.nv + a] = mk4((float)dst[3 * a], (float)dst[3 * a + 1], (float)dst[3 * a + 2], 0.f);
     }
 }
 
@@ -1074,15 +1100,11 @@ __global__ void k_place_near(KP P, Bufs B, const TV* xv)
         int64_t lk = (int64_t)f * P.kb;
         int nb = B.nbulk[lk];
         int64_t jmol = nb > 0 ? (int64_t)B.blist[lk * P.Mv + cm::draw_index(r.x, nb)] : cm::draw_index(r.x, P.Mv);
-        const TV* src = xv + (int64_t)f * P.Nv * 3 + jmol * P.nv * 3;
-        double* dst = B.P + (fk * P.Nv + (int64_t)pos * P.nv) * 3;
-        for (int q = 0; q < 3 * P.nv; q++) dst[q] = (double)src[q];
-        MolAcc<TV> acc{dst};
+        MolSrc<TV> src{xv + (int64_t)f * P.Nv * 3 + jmol * P.nv * 3};
+        MolDst dst{B.P + (fk * P.Nv + (int64_t)pos * P.nv) * 3, B.vw_r + fk * P.Nv + (int64_t)pos * P.nv};
         double uc[3] = {cm::u01(r.y), cm::u01(r.z), cm::u01(r.w)};
         double ua[3] = {cm::u01(r2.x), cm::u01(r2.y), cm::u01(r2.z)};
-        cm::random_move(acc, P.nv, P.iref, g.cell, uc, ua);
-        float4* dw = B.vw_r + fk * P.Nv + (int64_t)pos * P.nv;
-        for (int a = 0; a < P.nv; a++) dw[a] = mk4((float)dst[3 * a], (float)dst[3 * a + 1], (float)dst[3 * a + 2], 0.f);
+        cm::random_move(src, dst, P.nv, P.iref, g.cell, uc, ua);
     }
 }
 
@@ -1465,7 +1487,7 @@ struct cm_job {
     DevBuf near, mkey, mcnt, mstart, msorted, nnear, chunk_base;
     DevBuf xin_u, xin_v;
     double cell_frac = 0.5;   // tiles family: fine cell side as a fraction of the cutoff
-    int tile_c = 1;           // tiles family: columns per atom row (1, 2 or 4)
+    int tile_c = 1;           // tiles family: columns per atom row (1, 2, 4 or 8)
     int eval_grid = 4096;     // tiles family: workgroups of the persistent evaluation kernel
     int Fcap = 0;
     int cap_cells = 0;
@@ -1534,7 +1556,8 @@ void launch_tiles(bool f32, bool random, hipStream_t s, int grid, size_t shm, co
 void launch_tiles_c(int c, bool f32, bool random, hipStream_t s, int grid, size_t shm, const KP& P, const Bufs& B,
                     const void* xu, const void* xv)
 {
-    if (c == 4) launch_tiles<4>(f32, random, s, grid, shm, P, B, xu, xv);
+    if (c == 8) launch_tiles<8>(f32, random, s, grid, shm, P, B, xu, xv);
+    else if (c == 4) launch_tiles<4>(f32, random, s, grid, shm, P, B, xu, xv);
     else if (c == 2) launch_tiles<2>(f32, random, s, grid, shm, P, B, xu, xv);
     else launch_tiles<1>(f32, random, s, grid, shm, P, B, xu, xv);
 }
@@ -1663,8 +1686,12 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
     const int mols = tiles ? 0 : 1;
     {
         int64_t n = (int64_t)F * (P.Mv + (mols ? P.Mu : 0));
-        if (f32) hipLaunchKernelGGL(k_radii<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, uf, vf, mols);
-        else hipLaunchKernelGGL(k_radii<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, ud, vd, mols);
+        {
+            const int64_t nm = (int64_t)P.Mv + (mols ? P.Mu : 0);
+            const dim3 gr((unsigned)std::min<int64_t>((nm + BLOCK - 1) / BLOCK, 64), (unsigned)F);
+            if (f32) hipLaunchKernelGGL(k_radii<float>, gr, dim3(BLOCK), 0, s, P, B, uf, vf, mols);
+            else hipLaunchKernelGGL(k_radii<double>, gr, dim3(BLOCK), 0, s, P, B, ud, vd, mols);
+        }
     }
     hipLaunchKernelGGL(k_grid, dim3(1), dim3(64), 0, s, P, B, mols);
     if (tiles) {
@@ -1951,7 +1978,11 @@ int cm_job_create(const cm_job_desc* desc, cm_job** out)
     P.Nu = Nu;
     P.Nv = Nv;
     P.NP = 1 + J->K;
+    // columns: as many as still leave two molecules per wavefront (smaller
+    // chunks stage fewer solute atoms; the columns keep the lanes busy)
     J->tile_c = 1;
+    for (int c = 8; c > 1; c >>= 1)
+        if ((64 / c) / P.nv >= 2) { J->tile_c = c; break; }
     P.mpt = (64 / J->tile_c) / P.nv;
     int r;
     auto bail = [&](int code) { cm_job_destroy(J); return code; };
@@ -2064,7 +2095,9 @@ int cm_job_set_tuning(cm_job* J, double cell_fraction, int32_t columns, int32_t 
     if (!J) return fail(CM_EINVAL, "null job");
     if (J->frames > 0) return fail(CM_ESTATE, "tuning must be set before the first submit");
     if (!(cell_fraction >= 0.2 && cell_fraction <= 1.0)) return fail(CM_EINVAL, "cell_fraction must be in [0.2, 1]");
-    if (columns != 1 && columns != 2 && columns != 4) return fail(CM_EINVAL, "columns must be 1, 2 or 4");
+    if (columns != 0 && columns != 1 && columns != 2 && columns != 4 && columns != 8)
+        return fail(CM_EINVAL, "columns must be 0 (default), 1, 2, 4 or 8");
+    if (columns == 0) columns = J->tile_c;
     if ((64 / columns) / J->kp.nv < 1) return fail(CM_EINVAL, "too many columns for the solvent molecule size");
     if (eval_workgroups < 0) return fail(CM_EINVAL, "eval_workgroups must be >= 0");
     J->cell_frac = cell_fraction;

@@ -303,7 +303,7 @@ def test_exact_ties(pkg, oracle):
 
 
 @pytest.mark.parametrize("tuning", [dict(cell_fraction=0.5, columns=1), dict(cell_fraction=0.3, columns=2),
-                                    dict(cell_fraction=1.0, columns=4),
+                                    dict(cell_fraction=1.0, columns=4), dict(cell_fraction=0.25, columns=8),
                                     dict(cell_fraction=0.2, columns=1, eval_workgroups=7)])
 def test_tuning_does_not_change_results(pkg, oracle, tuning):
     """Cell size, columns and grid of the single-solute kernels change the work
@@ -331,6 +331,7 @@ def test_dense_solute_staging_rounds(pkg, oracle):
     traj = pkg.MemoryTrajectory(xu, xv, np.diag([box] * 3)[None], su, sv)
     opts = pkg.Options(dbulk=10.0, n_random_samples=3, seed=8, silent=True)
     a, _ = oracle_raw(pkg, oracle, traj, opts)
-    for tuning in (dict(cell_fraction=1.0, columns=1), dict(cell_fraction=0.5, columns=4)):
+    for tuning in (dict(cell_fraction=1.0, columns=1), dict(cell_fraction=0.5, columns=4),
+                   dict(cell_fraction=0.5, columns=8)):
         b = gpu_raw(pkg, traj, opts, job_kw={"tuning": tuning})
         assert_raw_equal(a, b)
