@@ -93,6 +93,7 @@ struct KP {                  // job parameters, by value
     int F;                   // frames in the batch
     int64_t Nu, Nv;
     int mpt;                 // molecules per chunk (tiles family)
+    int tcap;                // staged solute atoms per round (tiles family)
     int NP;                  // passes per frame, 1 + K (tiles family)
     int kcap;                // molecule sort keys per (frame, pass): Morton codes of the fine cells
 };
@@ -505,6 +506,22 @@ __device__ __forceinline__ void hist_record(const KP& P, unsigned* sh, ull* gu, 
     }
 }
 
+// the same with the md/rdf counts straight into the global counters
+__device__ __forceinline__ void hist_record_global(const KP& P, ull* md, ull* rdf, ull* gu, ull* gv, const Rec& r)
+{
+    if (!r.within) return;
+    int b = (int)cm::setbin(r.d, P.binstep) - 1;
+    if (b >= P.nbins) return;   // unreachable: d <= cutoff (validated at job creation)
+    atomicAdd(&md[b], 1ull);
+    group_hit(gu, P.nbins, b, r.i, P.gu_ptr, P.gu_idx, P.nu);
+    if (P.self_) group_hit(gu, P.nbins, b, r.j, P.gv_ptr, P.gv_idx, P.nu);
+    else group_hit(gv, P.nbins, b, r.j, P.gv_ptr, P.gv_idx, P.nv);
+    if (r.ref) {
+        int rb = (int)cm::setbin(r.dref, P.binstep) - 1;
+        if (rb < P.nbins) atomicAdd(&rdf[rb], 1ull);
+    }
+}
+
 __device__ __forceinline__ void hist_clear(unsigned* sh, int n)
 {
     for (int k = threadIdx.x; k < n; k += blockDim.x) sh[k] = 0u;
@@ -832,7 +849,9 @@ __global__ void k_place(KP P, Bufs B, const TV* xv)
 // ---------------------------------------------------------------------------
 // tiles family: one solute molecule, solvent molecules of up to 64 atoms
 // ---------------------------------------------------------------------------
-constexpr int TCAP = 1024;   // staged solute atoms per round (16 KiB of LDS)
+constexpr int TCAP_DEFAULT = 640;   // staged solute atoms per round (10 KiB of LDS)
+constexpr int CBCAP = 1024;         // chunk-table entries kept in LDS
+constexpr int SU = 4;               // staging: ranks per lane per step
 
 __device__ __forceinline__ int wrap_index(int i, int n, int& m)
 {
@@ -1152,24 +1171,33 @@ __device__ __forceinline__ Rec rec_shfl_xor(const Rec& r, int o)
 // (64 threads).  Lane = (atom row, column); a row is one solvent atom and
 // the C columns of a row split the staged solute atoms between them.
 template <class TU, class TV, bool RANDOM, int C>
-__global__ void __launch_bounds__(64) k_eval_tiles(KP P, Bufs B, const TU* xu, const TV* xv)
+__global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu, const TV* xv)
 {
-    __shared__ float4 stg[TCAP];
-    extern __shared__ unsigned sh[];
-    hist_clear(sh, 2 * P.nbins);
+    // dynamic LDS: the staging buffer (P.tcap float4), then the chunk table
+    // of this launch when it has at most CBCAP entries
+    extern __shared__ float4 dyn[];
+    float4* stg = dyn;
+    int* cbs = (int*)(dyn + P.tcap);
+    const int tcap = P.tcap;
     const int lane = threadIdx.x;
     const int row = lane / C, col = lane - (lane / C) * C;
     const int p0 = RANDOM ? 1 : 0;
     const int npass = RANDOM ? P.K : 1;
     const int nfp = P.F * npass;
+    const bool cb_lds = nfp + 1 <= CBCAP;
+    if (cb_lds)
+        for (int k = lane; k <= nfp; k += 64) cbs[k] = B.chunk_base[k];
+    __syncthreads();
     const int total = B.chunk_base[nfp];
     ull* gu = RANDOM ? B.cnt_gu_r : B.cnt_gu;
     ull* gv = RANDOM ? B.cnt_gv_r : B.cnt_gv;
+    ull* gmd = RANDOM ? B.cnt_md_r : B.cnt_md;
+    ull* grdf = RANDOM ? B.cnt_rdf_r : B.cnt_rdf;
     for (int ch = blockIdx.x; ch < total; ch += gridDim.x) {
         int lo_ = 0, hi_ = nfp;
         while (hi_ - lo_ > 1) {
             int mid = (lo_ + hi_) >> 1;
-            if (B.chunk_base[mid] <= ch) lo_ = mid; else hi_ = mid;
+            if ((cb_lds ? cbs[mid] : B.chunk_base[mid]) <= ch) lo_ = mid; else hi_ = mid;
         }
         const int fq = lo_;
         const int f = fq / npass, pidx = fq - (fq / npass) * npass;
@@ -1242,33 +1270,40 @@ __global__ void __launch_bounds__(64) k_eval_tiles(KP P, Bufs B, const TU* xu, c
             const int incl = wave_incl_scan(cnt, lane);
             const int excl = incl - cnt;
             const int T = __shfl(incl, 63, 64);
-            for (int t0 = 0; t0 < T; t0 += 64) {
-                if (nst + 64 > TCAP) {
-                    evaluate(nst);
-                    rounds++;
-                    nst = 0;
+            for (int t0 = 0; t0 < T; t0 += 64 * SU) {
+                float4 q[SU];
+#pragma unroll
+                for (int u = 0; u < SU; u++) {
+                    const int t = t0 + 64 * u + lane;
+                    int j = 0;   // owner cell: largest j with excl[j] <= t
+                    for (int step = 32; step >= 1; step >>= 1) {
+                        const int e = __shfl(excl, (j + step) & 63, 64);
+                        if (j + step < 64 && e <= t) j += step;
+                    }
+                    const int sj = __shfl(cstart, j, 64), ej = __shfl(excl, j, 64);
+                    const float txj = __shfl(tx, j, 64), tyj = __shfl(ty, j, 64), tzj = __shfl(tz, j, 64);
+                    q[u] = mk4(INFINITY, INFINITY, INFINITY, 0.f);
+                    if (t < T) {
+                        const float4 sa = srt[sj + (t - ej)];
+                        q[u] = mk4(sa.x + txj, sa.y + tyj, sa.z + tzj, sa.w);
+                    }
                 }
-                const int t = t0 + lane;
-                int j = 0;   // owner cell: largest j with excl[j] <= t
-                for (int step = 32; step >= 1; step >>= 1) {
-                    const int e = __shfl(excl, (j + step) & 63, 64);
-                    if (j + step < 64 && e <= t) j += step;
+#pragma unroll
+                for (int u = 0; u < SU; u++) {
+                    if (t0 + 64 * u >= T) break;
+                    if (nst + 64 > tcap) {
+                        evaluate(nst);
+                        rounds++;
+                        nst = 0;
+                    }
+                    const float dx = fmaxf(fmaxf(lx - q[u].x, q[u].x - hx), 0.f);
+                    const float dy = fmaxf(fmaxf(ly - q[u].y, q[u].y - hy), 0.f);
+                    const float dz = fmaxf(fmaxf(lz - q[u].z, q[u].z - hz), 0.f);
+                    const bool pass = t0 + 64 * u + lane < T && dx * dx + dy * dy + dz * dz <= rs2;
+                    const unsigned long long m = __ballot(pass);
+                    if (pass) stg[nst + __popcll(m & lt_mask)] = q[u];
+                    nst += __popcll(m);
                 }
-                const int sj = __shfl(cstart, j, 64), ej = __shfl(excl, j, 64);
-                const float txj = __shfl(tx, j, 64), tyj = __shfl(ty, j, 64), tzj = __shfl(tz, j, 64);
-                bool pass = false;
-                float4 q = mk4(0.f, 0.f, 0.f, 0.f);
-                if (t < T) {
-                    const float4 sa = srt[sj + (t - ej)];
-                    q = mk4(sa.x + txj, sa.y + tyj, sa.z + tzj, sa.w);
-                    const float dx = fmaxf(fmaxf(lx - q.x, q.x - hx), 0.f);
-                    const float dy = fmaxf(fmaxf(ly - q.y, q.y - hy), 0.f);
-                    const float dz = fmaxf(fmaxf(lz - q.z, q.z - hz), 0.f);
-                    pass = dx * dx + dy * dy + dz * dz <= rs2;
-                }
-                const unsigned long long m = __ballot(pass);
-                if (pass) stg[nst + __popcll(m & lt_mask)] = q;
-                nst += __popcll(m);
             }
         }
         const bool one_round = rounds == 0;
@@ -1389,13 +1424,11 @@ __global__ void __launch_bounds__(64) k_eval_tiles(KP P, Bufs B, const TU* xu, c
             if (lane == L) r = loc;
         }
         if (leader) {
-            hist_record(P, sh, gu, gv, r);
+            hist_record_global(P, gmd, grdf, gu, gv, r);
             if (!RANDOM && P.K > 0) B.bflag[(int64_t)f * P.kb * P.Mv + v] = inbulk(P, r) ? 1 : 0;
         }
         __syncthreads();
     }
-    if (RANDOM) hist_flush(sh, P.nbins, B.cnt_md_r, B.cnt_rdf_r);
-    else hist_flush(sh, P.nbins, B.cnt_md, B.cnt_rdf);
 }
 
 // integer hits -> FP64 accumulators
@@ -1488,7 +1521,8 @@ struct cm_job {
     DevBuf xin_u, xin_v;
     double cell_frac = 0.5;   // tiles family: fine cell side as a fraction of the cutoff
     int tile_c = 1;           // tiles family: columns per atom row (1, 2, 4 or 8)
-    int eval_grid = 4096;     // tiles family: workgroups of the persistent evaluation kernel
+    int eval_grid = 0;        // tiles family: workgroups of the persistent evaluation kernel (0: fill the device)
+    int cus = 256;            // compute units of the device
     int Fcap = 0;
     int cap_cells = 0;
     bool finished = false;
@@ -1540,26 +1574,41 @@ int64_t cells_for(const cm_job* J, const double* cell9, int64_t* keys)
     return n;
 }
 
+// grid 0: as many workgroups as the device holds at once (persistent kernel)
+template <class TU, class TV, bool RANDOM, int C>
+void launch_tiles_one(hipStream_t s, int grid, int cus, const KP& P, const Bufs& B, const void* xu, const void* xv)
+{
+    const int nfp = P.F * (RANDOM ? P.K : 1);
+    const size_t shm = sizeof(float4) * (size_t)P.tcap + sizeof(int) * (size_t)std::min(nfp + 1, CBCAP);
+    auto kern = k_eval_tiles<TU, TV, RANDOM, C>;
+    if (grid <= 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 64, shm) != hipSuccess || nb < 1) nb = 1;
+        grid = nb * cus;
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64), shm, s, P, B, (const TU*)xu, (const TV*)xv);
+}
+
 template <int C>
-void launch_tiles(bool f32, bool random, hipStream_t s, int grid, size_t shm, const KP& P, const Bufs& B,
-                  const void* xu, const void* xv)
+void launch_tiles(bool f32, bool random, hipStream_t s, int grid, int cus, const KP& P, const Bufs& B, const void* xu,
+                  const void* xv)
 {
     if (f32) {
-        if (random) hipLaunchKernelGGL((k_eval_tiles<float, double, true, C>), dim3(grid), dim3(64), shm, s, P, B, (const float*)xu, (const double*)nullptr);
-        else hipLaunchKernelGGL((k_eval_tiles<float, float, false, C>), dim3(grid), dim3(64), shm, s, P, B, (const float*)xu, (const float*)xv);
+        if (random) launch_tiles_one<float, double, true, C>(s, grid, cus, P, B, xu, nullptr);
+        else launch_tiles_one<float, float, false, C>(s, grid, cus, P, B, xu, xv);
     } else {
-        if (random) hipLaunchKernelGGL((k_eval_tiles<double, double, true, C>), dim3(grid), dim3(64), shm, s, P, B, (const double*)xu, (const double*)nullptr);
-        else hipLaunchKernelGGL((k_eval_tiles<double, double, false, C>), dim3(grid), dim3(64), shm, s, P, B, (const double*)xu, (const double*)xv);
+        if (random) launch_tiles_one<double, double, true, C>(s, grid, cus, P, B, xu, nullptr);
+        else launch_tiles_one<double, double, false, C>(s, grid, cus, P, B, xu, xv);
     }
 }
 
-void launch_tiles_c(int c, bool f32, bool random, hipStream_t s, int grid, size_t shm, const KP& P, const Bufs& B,
+void launch_tiles_c(int c, bool f32, bool random, hipStream_t s, int grid, int cus, const KP& P, const Bufs& B,
                     const void* xu, const void* xv)
 {
-    if (c == 8) launch_tiles<8>(f32, random, s, grid, shm, P, B, xu, xv);
-    else if (c == 4) launch_tiles<4>(f32, random, s, grid, shm, P, B, xu, xv);
-    else if (c == 2) launch_tiles<2>(f32, random, s, grid, shm, P, B, xu, xv);
-    else launch_tiles<1>(f32, random, s, grid, shm, P, B, xu, xv);
+    if (c == 8) launch_tiles<8>(f32, random, s, grid, cus, P, B, xu, xv);
+    else if (c == 4) launch_tiles<4>(f32, random, s, grid, cus, P, B, xu, xv);
+    else if (c == 2) launch_tiles<2>(f32, random, s, grid, cus, P, B, xu, xv);
+    else launch_tiles<1>(f32, random, s, grid, cus, P, B, xu, xv);
 }
 
 int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, const void* xv_dev, int dtype,
@@ -1725,7 +1774,7 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
     // real pass
     if (J->profiling) HIPCHK(hipEventRecord(ev[0], s));
     if (tiles) {
-        launch_tiles_c(J->tile_c, f32, false, s, J->eval_grid, shm, P, B, xu_dev, xv_dev);
+        launch_tiles_c(J->tile_c, f32, false, s, J->eval_grid, J->cus, P, B, xu_dev, xv_dev);
     } else {
         int64_t n = (int64_t)F * P.Mv;
         if (f32) hipLaunchKernelGGL((k_eval_mols_real<float, float>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, uf, vf);
@@ -1751,7 +1800,7 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         HIPCHK(hipGetLastError());
         if (J->profiling) HIPCHK(hipEventRecord(ev[2], s));
         if (tiles) {
-            launch_tiles_c(J->tile_c, f32, true, s, J->eval_grid, shm, P, B, xu_dev, nullptr);
+            launch_tiles_c(J->tile_c, f32, true, s, J->eval_grid, J->cus, P, B, xu_dev, nullptr);
         } else {
             if (f32) hipLaunchKernelGGL(k_eval_mols_rand<float>, dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, uf);
             else hipLaunchKernelGGL(k_eval_mols_rand<double>, dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, ud);
@@ -1984,6 +2033,13 @@ int cm_job_create(const cm_job_desc* desc, cm_job** out)
     for (int c = 8; c > 1; c >>= 1)
         if ((64 / c) / P.nv >= 2) { J->tile_c = c; break; }
     P.mpt = (64 / J->tile_c) / P.nv;
+    P.tcap = TCAP_DEFAULT;
+    {
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess)
+            if (hipDeviceGetAttribute(&J->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || J->cus < 1)
+                J->cus = 256;
+    }
     int r;
     auto bail = [&](int code) { cm_job_destroy(J); return code; };
     if (hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(CM_EHIP, "hipStreamCreate failed"));
@@ -2103,7 +2159,7 @@ int cm_job_set_tuning(cm_job* J, double cell_fraction, int32_t columns, int32_t 
     J->cell_frac = cell_fraction;
     J->tile_c = columns;
     J->kp.mpt = (64 / columns) / J->kp.nv;
-    J->eval_grid = eval_workgroups > 0 ? eval_workgroups : 4096;
+    J->eval_grid = eval_workgroups;
     return CM_OK;
 }
 
