@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true", help="only time the evaluation kernels per tuning, then exit")
+    ap.add_argument("--cell-fraction", type=float, default=0.0, help="tuning: fine cell side / cutoff (0: default)")
+    ap.add_argument("--columns", type=int, default=0, help="tuning: columns per solvent atom row (0: default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain read-N/write-N elementwise kernel over MIB MiB three times (the "
                          "reference for scaling FETCH_SIZE/WRITE_SIZE counters), then exit")
@@ -109,8 +111,12 @@ def main():
     frames = np.arange(1, nf + 1)
     weights = np.ones(nf)
 
+    tuning = None
+    if args.cell_fraction or args.columns:
+        tuning = dict(cell_fraction=args.cell_fraction or 0.5, columns=args.columns)
+
     def one_job(profile=False):
-        job = _job_for(R, opts, tm, False)
+        job = _job_for(R, opts, tm, False, tuning=tuning)
         if profile:
             job.set_profiling(True)
         job.submit_device(frames, du, dv, 0, cells, weights)

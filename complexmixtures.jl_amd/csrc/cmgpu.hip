@@ -2027,12 +2027,10 @@ int cm_job_create(const cm_job_desc* desc, cm_job** out)
     P.Nu = Nu;
     P.Nv = Nv;
     P.NP = 1 + J->K;
-    // columns: as many as still leave two molecules per wavefront (smaller
-    // chunks stage fewer solute atoms; the columns keep the lanes busy)
+    // one column: every lane of a wavefront a solvent atom (the per-chunk
+    // costs of smaller chunks outweigh their shorter staged lists)
     J->tile_c = 1;
-    for (int c = 8; c > 1; c >>= 1)
-        if ((64 / c) / P.nv >= 2) { J->tile_c = c; break; }
-    P.mpt = (64 / J->tile_c) / P.nv;
+    P.mpt = 64 / P.nv;
     P.tcap = TCAP_DEFAULT;
     {
         int dev = 0;
