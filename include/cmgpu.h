@@ -133,8 +133,8 @@ int cm_job_finish(cm_job *job, cm_raw *out);
  * reference's Options.lcell plays this role for CellListMap): fine cell side
  * as a fraction of the cutoff (0.2-1.0), solute-atom columns per solvent atom
  * row of a wavefront (1, 2, 4 or 8; 0 keeps the default, 1) and workgroups of
- * the persistent
- * evaluation kernel (0 = default).  Results do not depend on these. */
+ * the persistent evaluation kernel (0 = as many as the device holds at once).
+ * Results do not depend on these. */
 int cm_job_set_tuning(cm_job *job, double cell_fraction, int32_t columns, int32_t eval_workgroups);
 
 /* Kernel timing of the job (HIP events on the job's stream). */
