@@ -45,9 +45,24 @@ class KernelStats(ctypes.Structure):
                 ("frames", _i64), ("candidate_pairs", _i64), ("fp64_rechecks", _i64)]
 
 
+class ServerDesc(ctypes.Structure):
+    _fields_ = [("slots", _i32), ("pad", _i32), ("memory_budget", _i64)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("job", _P(JobDesc)), ("nframes", _i32), ("dtype", _i32), ("frame_index", _P(_i64)),
+                ("d_x_solute", ctypes.c_void_p), ("d_x_solvent", ctypes.c_void_p), ("cells", _P(_d)),
+                ("weights", _P(_d)), ("cell_fraction", _d), ("columns", _i32), ("pad", _i32)]
+
+
+class RequestTimes(ctypes.Structure):
+    _fields_ = [("enqueued", _d), ("started", _d), ("finished", _d)]
+
+
 EXPORTS = ["cm_job_create", "cm_job_submit", "cm_job_submit_device", "cm_job_finish", "cm_job_set_profiling",
            "cm_job_set_tuning", "cm_job_kernel_stats", "cm_job_destroy", "cm_last_error", "cm_device_alloc", "cm_device_upload",
-           "cm_device_free", "cm_device_synchronize"]
+           "cm_device_free", "cm_device_synchronize", "cm_job_footprint", "cm_server_create", "cm_server_enqueue",
+           "cm_server_wait", "cm_server_destroy"]
 
 _lib = None
 
@@ -77,9 +92,16 @@ def lib():
         L.cm_device_alloc.argtypes = [ctypes.c_size_t, _P(ctypes.c_void_p)]
         L.cm_device_upload.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         L.cm_device_free.argtypes = [ctypes.c_void_p]
+        L.cm_job_footprint.argtypes = [_P(JobDesc), _i32, _i32, _P(_d), _P(_i64)]
+        L.cm_server_create.argtypes = [_P(ServerDesc), _P(ctypes.c_void_p)]
+        L.cm_server_enqueue.argtypes = [ctypes.c_void_p, _P(Request), _P(_u64)]
+        L.cm_server_wait.argtypes = [ctypes.c_void_p, _u64, _P(RawOut), _P(RequestTimes)]
+        L.cm_server_destroy.argtypes = [ctypes.c_void_p]
+        L.cm_server_destroy.restype = None
         for f in ("cm_job_create", "cm_job_submit", "cm_job_submit_device", "cm_job_finish", "cm_job_set_profiling",
                   "cm_job_set_tuning", "cm_job_kernel_stats", "cm_device_alloc", "cm_device_upload", "cm_device_free",
-                  "cm_device_synchronize"):
+                  "cm_device_synchronize", "cm_job_footprint", "cm_server_create", "cm_server_enqueue",
+                  "cm_server_wait"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -125,26 +147,58 @@ class DeviceArray:
             pass
 
 
+def job_desc(*, nmols_u, natoms_per_mol_u, nmols_v, natoms_per_mol_v, autocorrelation, irefatom, nbins, binstep,
+             cutoff, dbulk, usecutoff, n_random_samples, coordination_number_only, triclinic, seed, n_groups_u,
+             n_groups_v, groups_u=(None, None), groups_v=(None, None), lcell=1, max_batch_frames=0, memory_budget=0):
+    """JobDesc for the C ABI; returns (desc, arrays the desc points into)."""
+    keep = [g for g in (*groups_u, *groups_v) if g is not None]
+    nul = _P(_i32)()
+
+    def ip(a):
+        return a.ctypes.data_as(_P(_i32)) if a is not None else nul
+
+    d = JobDesc(int(nmols_u), int(natoms_per_mol_u), int(nmols_v), int(natoms_per_mol_v), int(bool(autocorrelation)),
+                int(irefatom), int(nbins), int(bool(usecutoff)), float(binstep), float(cutoff), float(dbulk),
+                int(n_random_samples), int(bool(coordination_number_only)), int(bool(triclinic)), int(lcell),
+                int(seed) & 0xFFFFFFFFFFFFFFFF, int(n_groups_u), int(n_groups_v),
+                ip(groups_u[0]), ip(groups_u[1]), ip(groups_v[0]), ip(groups_v[1]),
+                int(max_batch_frames), int(memory_budget))
+    return d, keep
+
+
+def _cells9(cells):
+    return np.ascontiguousarray(np.transpose(np.asarray(cells, dtype=np.float64), (0, 2, 1)).reshape(-1, 9))
+
+
+def _raw_buffers(nbins, ngu, ngv):
+    out = {k: np.zeros(nbins) for k in ("md_count", "md_count_random", "rdf_count", "rdf_count_random")}
+    for k, g in (("solute_group_count", ngu), ("solvent_group_count", ngv),
+                 ("solute_group_count_random", ngu), ("solvent_group_count_random", ngv)):
+        out[k] = np.zeros((g, nbins))
+    hits = {k: np.zeros(nbins, dtype=np.uint64) for k in ("hits_md", "hits_md_random", "hits_rdf", "hits_rdf_random")}
+    r = RawOut(*[_dp(out[k]) for k in ("md_count", "md_count_random", "rdf_count", "rdf_count_random",
+                                       "solute_group_count", "solvent_group_count", "solute_group_count_random",
+                                       "solvent_group_count_random")], 0.0,
+               *[hits[k].ctypes.data_as(_P(_u64)) for k in ("hits_md", "hits_md_random", "hits_rdf", "hits_rdf_random")])
+    return out, hits, r
+
+
+def footprint(frame_cells, dtype=np.float32, **job_kw):
+    """Device bytes a job holds while it runs these frames (cm_job_footprint)."""
+    d, _keep = job_desc(**job_kw)
+    c9 = _cells9(frame_cells)
+    b = _i64()
+    check(lib().cm_job_footprint(ctypes.byref(d), len(c9), CM_F32 if np.dtype(dtype) == np.float32 else CM_F64,
+                                 _dp(c9), ctypes.byref(b)))
+    return b.value
+
+
 class Job:
     """One accumulation job on the device (cm_job_*)."""
 
-    def __init__(self, *, nmols_u, natoms_per_mol_u, nmols_v, natoms_per_mol_v, autocorrelation, irefatom, nbins,
-                 binstep, cutoff, dbulk, usecutoff, n_random_samples, coordination_number_only, triclinic, seed,
-                 n_groups_u, n_groups_v, groups_u=(None, None), groups_v=(None, None), lcell=1, max_batch_frames=0,
-                 memory_budget=0, tuning=None):
-        self._keep = [g for g in (*groups_u, *groups_v) if g is not None]
-        nul = _P(_i32)()
-
-        def ip(a):
-            return a.ctypes.data_as(_P(_i32)) if a is not None else nul
-
-        self.nbins, self.ngu, self.ngv = int(nbins), int(n_groups_u), int(n_groups_v)
-        d = JobDesc(int(nmols_u), int(natoms_per_mol_u), int(nmols_v), int(natoms_per_mol_v), int(bool(autocorrelation)),
-                    int(irefatom), int(nbins), int(bool(usecutoff)), float(binstep), float(cutoff), float(dbulk),
-                    int(n_random_samples), int(bool(coordination_number_only)), int(bool(triclinic)), int(lcell),
-                    int(seed) & 0xFFFFFFFFFFFFFFFF, int(n_groups_u), int(n_groups_v),
-                    ip(groups_u[0]), ip(groups_u[1]), ip(groups_v[0]), ip(groups_v[1]),
-                    int(max_batch_frames), int(memory_budget))
+    def __init__(self, *, tuning=None, **job_kw):
+        d, self._keep = job_desc(**job_kw)
+        self.nbins, self.ngu, self.ngv = int(job_kw["nbins"]), int(job_kw["n_groups_u"]), int(job_kw["n_groups_v"])
         self.h = ctypes.c_void_p()
         check(lib().cm_job_create(ctypes.byref(d), ctypes.byref(self.h)))
         if tuning:
@@ -159,7 +213,7 @@ class Job:
             raise ArgumentError("coordinates must be both float64 or both float32")
         dt = CM_F32 if xu.dtype == np.float32 else CM_F64
         fi = np.ascontiguousarray(frame_index, dtype=np.int64)
-        c9 = np.ascontiguousarray(np.transpose(np.asarray(cells, dtype=np.float64), (0, 2, 1)).reshape(-1, 9))
+        c9 = _cells9(cells)
         w = np.ascontiguousarray(weights, dtype=np.float64)
         check(lib().cm_job_submit(self.h, len(fi), fi.ctypes.data_as(_P(_i64)), xu.ctypes.data_as(ctypes.c_void_p),
                                   xv.ctypes.data_as(ctypes.c_void_p), dt, _dp(c9), _dp(w)))
@@ -169,7 +223,7 @@ class Job:
         frame0 .. frame0+len(frame_index)-1 of those arrays."""
         dt = CM_F32 if d_solute.dtype == np.float32 else CM_F64
         fi = np.ascontiguousarray(frame_index, dtype=np.int64)
-        c9 = np.ascontiguousarray(np.transpose(np.asarray(cells, dtype=np.float64), (0, 2, 1)).reshape(-1, 9))
+        c9 = _cells9(cells)
         w = np.ascontiguousarray(weights, dtype=np.float64)
         fu = d_solute.nbytes // d_solute.shape[0]
         fv = d_solvent.nbytes // d_solvent.shape[0]
@@ -177,17 +231,7 @@ class Job:
                                          d_solvent.offset(frame0 * fv), dt, _dp(c9), _dp(w)))
 
     def finish(self):
-        nb = self.nbins
-        out = {k: np.zeros(nb) for k in ("md_count", "md_count_random", "rdf_count", "rdf_count_random")}
-        for k, g in (("solute_group_count", self.ngu), ("solvent_group_count", self.ngv),
-                     ("solute_group_count_random", self.ngu), ("solvent_group_count_random", self.ngv)):
-            out[k] = np.zeros((g, nb))
-        hits = {k: np.zeros(nb, dtype=np.uint64) for k in ("hits_md", "hits_md_random", "hits_rdf", "hits_rdf_random")}
-        r = RawOut(*[_dp(out[k]) for k in ("md_count", "md_count_random", "rdf_count", "rdf_count_random",
-                                           "solute_group_count", "solvent_group_count", "solute_group_count_random",
-                                           "solvent_group_count_random")], 0.0,
-                   *[hits[k].ctypes.data_as(_P(_u64)) for k in ("hits_md", "hits_md_random", "hits_rdf",
-                                                               "hits_rdf_random")])
+        out, hits, r = _raw_buffers(self.nbins, self.ngu, self.ngv)
         check(lib().cm_job_finish(self.h, ctypes.byref(r)))
         out["volume_total"] = r.volume_total
         out.update(hits)
@@ -207,6 +251,54 @@ class Job:
     def close(self):
         if self.h:
             lib().cm_job_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request server (cm_server_*): continuous batching of jobs over
+    device-resident frames.  enqueue() returns a ticket; wait(ticket) blocks
+    until that request completes and returns (raw, times)."""
+
+    def __init__(self, slots=4, memory_budget=0):
+        self.h = ctypes.c_void_p()
+        check(lib().cm_server_create(ctypes.byref(ServerDesc(int(slots), 0, int(memory_budget))), ctypes.byref(self.h)))
+        self._meta = {}
+
+    def enqueue(self, frame_index, d_solute, d_solvent, frame0, cells, weights, tuning=None, **job_kw):
+        d, keep = job_desc(**job_kw)
+        dt = CM_F32 if d_solute.dtype == np.float32 else CM_F64
+        fi = np.ascontiguousarray(frame_index, dtype=np.int64)
+        c9 = _cells9(cells)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        fu = d_solute.nbytes // d_solute.shape[0]
+        fv = d_solvent.nbytes // d_solvent.shape[0]
+        t = tuning or {}
+        req = Request(ctypes.pointer(d), len(fi), dt, fi.ctypes.data_as(_P(_i64)), d_solute.offset(frame0 * fu),
+                      d_solvent.offset(frame0 * fv), _dp(c9), _dp(w), float(t.get("cell_fraction", 0.0)),
+                      int(t.get("columns", 0)), 0)
+        ticket = _u64()
+        check(lib().cm_server_enqueue(self.h, ctypes.byref(req), ctypes.byref(ticket)))
+        self._meta[ticket.value] = (int(job_kw["nbins"]), int(job_kw["n_groups_u"]), int(job_kw["n_groups_v"]))
+        return ticket.value
+
+    def wait(self, ticket):
+        nb, ngu, ngv = self._meta.pop(ticket)
+        out, hits, r = _raw_buffers(nb, ngu, ngv)
+        tm = RequestTimes()
+        check(lib().cm_server_wait(self.h, _u64(ticket), ctypes.byref(r), ctypes.byref(tm)))
+        out["volume_total"] = r.volume_total
+        out.update(hits)
+        return out, {"enqueued": tm.enqueued, "started": tm.started, "finished": tm.finished}
+
+    def close(self):
+        if self.h:
+            lib().cm_server_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):

@@ -44,10 +44,16 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "cm_math.h"
@@ -1532,6 +1538,9 @@ struct cm_job {
     int64_t eval_launches = 0, total_launches = 0, frames = 0;
     std::vector<Geo> hgeo, hgeo_out;
     std::vector<int> huk, hdp, hdi;
+    std::vector<int32_t> hgm;        // group maps (host copy of the upload)
+    std::vector<double> hacc;        // accumulators read back at finish
+    std::vector<ull> hhits;
 };
 
 namespace {
@@ -1900,6 +1909,34 @@ int flush(cm_job* J, double w)
     return CM_OK;
 }
 
+// cell slots per frame (the largest grid of the frames, with 10% headroom so
+// that batches of later frames of a fluctuating box reuse the allocation) and
+// frames per batch
+int batch_plan(const cm_job* J, int32_t nframes, const double* cells, int ds, bool host_input, int64_t* capmax_,
+               int64_t* kcapmax_, int64_t* Fmax_, int64_t* pfb_)
+{
+    int64_t capmax = 1, kcapmax = 1;
+    for (int f = 0; f < nframes; f++) {
+        int64_t kk;
+        capmax = std::max(capmax, cells_for(J, cells + 9 * f, &kk));
+        kcapmax = std::max(kcapmax, kk);
+    }
+    if (capmax > (int64_t)1 << 26 || kcapmax > (int64_t)1 << 27)
+        return fail(CM_EUNSUPPORTED, "unit cell too large for the cell grid");
+    capmax = capmax + capmax / 10 + 1;
+    if (!J->atoms_family) kcapmax = 1;
+    const int64_t pfb = per_frame_bytes(J, ds, host_input, capmax, kcapmax);
+    const int64_t budget = J->d.memory_budget > 0 ? J->d.memory_budget : (int64_t)4 << 30;
+    int64_t Fmax = std::max<int64_t>(1, budget / std::max<int64_t>(pfb, 1));
+    if (J->d.max_batch_frames > 0) Fmax = std::min<int64_t>(Fmax, J->d.max_batch_frames);
+    Fmax = std::min<int64_t>(Fmax, 64);
+    *capmax_ = capmax;
+    *kcapmax_ = kcapmax;
+    *Fmax_ = Fmax;
+    *pfb_ = pfb;
+    return CM_OK;
+}
+
 int submit_common(cm_job* J, int32_t nframes, const int64_t* frame_index, const void* xu, const void* xv, int32_t dtype,
                   const double* cells, const double* weights, bool host_input)
 {
@@ -1914,23 +1951,9 @@ int submit_common(cm_job* J, int32_t nframes, const int64_t* frame_index, const 
         if (frame_index[f] < 1 || frame_index[f] > 0xFFFFFFFFll) return fail(CM_EINVAL, "frame_index must be in [1, 2^32)");
     }
     if (nframes == 0) return CM_OK;
-    // cell slots per frame: the largest grid of the submitted frames (with 10% headroom
-    // so that batches of later frames of a fluctuating box reuse the allocation)
-    int64_t capmax = 1, kcapmax = 1;
-    for (int f = 0; f < nframes; f++) {
-        int64_t kk;
-        capmax = std::max(capmax, cells_for(J, cells + 9 * f, &kk));
-        kcapmax = std::max(kcapmax, kk);
-    }
-    if (capmax > (int64_t)1 << 26 || kcapmax > (int64_t)1 << 27)
-        return fail(CM_EUNSUPPORTED, "unit cell too large for the cell grid");
-    capmax = capmax + capmax / 10 + 1;
-    if (!J->atoms_family) kcapmax = 1;
-    int64_t pfb = per_frame_bytes(J, ds, host_input, capmax, kcapmax);
-    int64_t budget = J->d.memory_budget > 0 ? J->d.memory_budget : (int64_t)4 << 30;
-    int64_t Fmax = std::max<int64_t>(1, budget / std::max<int64_t>(pfb, 1));
-    if (J->d.max_batch_frames > 0) Fmax = std::min<int64_t>(Fmax, J->d.max_batch_frames);
-    Fmax = std::min<int64_t>(Fmax, 64);
+    int64_t capmax, kcapmax, Fmax, pfb;
+    int r0 = batch_plan(J, nframes, cells, ds, host_input, &capmax, &kcapmax, &Fmax, &pfb);
+    if (r0) return r0;
     int f0 = 0;
     while (f0 < nframes) {
         // a batch holds frames of one weight (the weight multiplies the integer hits at flush)
@@ -1971,17 +1994,10 @@ int submit_common(cm_job* J, int32_t nframes, const int64_t* frame_index, const 
     return CM_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-const char* cm_last_error(void) { return g_err.c_str(); }
-
-int cm_job_create(const cm_job_desc* desc, cm_job** out)
+// argument checks of a job description (the messages the host maps to
+// ArgumentError)
+int validate_desc(const cm_job_desc& d)
 {
-    if (!desc || !out) return fail(CM_EINVAL, "null argument");
-    *out = nullptr;
-    const cm_job_desc& d = *desc;
     if (d.nmols_u < 1 || d.natoms_per_mol_u < 1 || d.nmols_v < 1 || d.natoms_per_mol_v < 1)
         return fail(CM_EINVAL, "selections must hold at least one molecule of at least one atom");
     if (d.irefatom < 1 || d.irefatom > d.natoms_per_mol_v)
@@ -2001,10 +2017,38 @@ int cm_job_create(const cm_job_desc* desc, cm_job** out)
     if (d.seed == 0) return fail(CM_EINVAL, "seed must be positive");
     int64_t Nu = (int64_t)d.nmols_u * d.natoms_per_mol_u, Nv = (int64_t)d.nmols_v * d.natoms_per_mol_v;
     if (Nu > INT32_MAX / 4 || Nv > INT32_MAX / 4) return fail(CM_EINVAL, "selections too large");
-    cm_job* J = new cm_job();
+    if (d.groups_u_ptr) {
+        if (!d.groups_u_idx) return fail(CM_EINVAL, "groups_u_idx missing");
+        for (int64_t k = 0; k < d.groups_u_ptr[d.natoms_per_mol_u]; k++)
+            if (d.groups_u_idx[k] < 0 || d.groups_u_idx[k] >= d.n_groups_u) return fail(CM_EINVAL, "solute group id out of range");
+    }
+    if (d.groups_v_ptr) {
+        if (!d.groups_v_idx) return fail(CM_EINVAL, "groups_v_idx missing");
+        const int64_t lim = d.autocorrelation ? d.n_groups_u : d.n_groups_v;
+        for (int64_t k = 0; k < d.groups_v_ptr[Nv]; k++)
+            if (d.groups_v_idx[k] < 0 || d.groups_v_idx[k] >= lim) return fail(CM_EINVAL, "solvent group id out of range");
+    }
+    return CM_OK;
+}
+
+// (re)initialises a job on its stream for a validated description; device
+// buffers already held by J are kept and grown as needed
+// host-side fields of a job for a validated description (no device work)
+void job_params(cm_job* J, const cm_job_desc& d)
+{
+    const int64_t Nu = (int64_t)d.nmols_u * d.natoms_per_mol_u, Nv = (int64_t)d.nmols_v * d.natoms_per_mol_v;
     J->d = d;
     J->Nu = Nu;
     J->Nv = Nv;
+    J->pending = false;
+    J->pending_w = 0.0;
+    J->volume_total = 0.0;
+    J->finished = false;
+    J->profiling = false;
+    J->eval_ms = J->total_ms = 0.0;
+    J->eval_launches = J->total_launches = J->frames = 0;
+    J->cell_frac = 0.5;
+    J->eval_grid = 0;
     // tiles family: one solute molecule, solvent molecules that fit a wavefront
     J->atoms_family = (d.nmols_u == 1 && !d.autocorrelation && d.natoms_per_mol_v <= 64);
     J->K = d.coordination_number_only ? 0 : d.n_random_samples;
@@ -2038,49 +2082,112 @@ int cm_job_create(const cm_job_desc* desc, cm_job** out)
             if (hipDeviceGetAttribute(&J->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || J->cus < 1)
                 J->cus = 256;
     }
+    J->acc_len = 4 * (int64_t)d.nbins + 2 * (int64_t)d.n_groups_u * d.nbins + 2 * (int64_t)d.n_groups_v * d.nbins;
+}
+
+int job_setup(cm_job* J, const cm_job_desc& d)
+{
+    job_params(J, d);
+    KP& P = J->kp;
+    const int64_t Nv = J->Nv;
+    const int64_t nb = d.nbins;
     int r;
-    auto bail = [&](int code) { cm_job_destroy(J); return code; };
-    if (hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(CM_EHIP, "hipStreamCreate failed"));
-    int64_t nb = d.nbins;
-    J->acc_len = 4 * nb + 2 * (int64_t)d.n_groups_u * nb + 2 * (int64_t)d.n_groups_v * nb;
-    if ((r = ensure(J->cnt, 8 * J->acc_len))) return bail(r);
-    if ((r = ensure(J->acc, 8 * J->acc_len))) return bail(r);
-    if ((r = ensure(J->hits, 8 * 4 * nb))) return bail(r);
-    if ((r = ensure(J->stats, 64))) return bail(r);
-    if (hipMemset(J->cnt.p, 0, 8 * J->acc_len) != hipSuccess || hipMemset(J->acc.p, 0, 8 * J->acc_len) != hipSuccess ||
-        hipMemset(J->hits.p, 0, 8 * 4 * nb) != hipSuccess || hipMemset(J->stats.p, 0, 64) != hipSuccess)
-        return bail(fail(CM_EHIP, "hipMemset failed"));
+    if ((r = ensure(J->cnt, 8 * J->acc_len))) return r;
+    if ((r = ensure(J->acc, 8 * J->acc_len))) return r;
+    if ((r = ensure(J->hits, 8 * 4 * nb))) return r;
+    if ((r = ensure(J->stats, 64))) return r;
+    HIPCHK(hipMemsetAsync(J->cnt.p, 0, 8 * J->acc_len, J->stream));
+    HIPCHK(hipMemsetAsync(J->acc.p, 0, 8 * J->acc_len, J->stream));
+    HIPCHK(hipMemsetAsync(J->hits.p, 0, 8 * 4 * nb, J->stream));
+    HIPCHK(hipMemsetAsync(J->stats.p, 0, 64, J->stream));
     // group maps
-    std::vector<int32_t> gm;
+    std::vector<int32_t>& gm = J->hgm;
+    gm.clear();
     int64_t ou_p = -1, ou_i = -1, ov_p = -1, ov_i = -1;
     if (d.groups_u_ptr) {
-        if (!d.groups_u_idx) return bail(fail(CM_EINVAL, "groups_u_idx missing"));
-        int64_t n = d.natoms_per_mol_u;
-        int64_t m = d.groups_u_ptr[n];
-        for (int64_t k = 0; k < m; k++)
-            if (d.groups_u_idx[k] < 0 || d.groups_u_idx[k] >= d.n_groups_u) return bail(fail(CM_EINVAL, "solute group id out of range"));
+        const int64_t n = d.natoms_per_mol_u, m = d.groups_u_ptr[n];
         ou_p = gm.size(); gm.insert(gm.end(), d.groups_u_ptr, d.groups_u_ptr + n + 1);
         ou_i = gm.size(); gm.insert(gm.end(), d.groups_u_idx, d.groups_u_idx + m); gm.push_back(0);
     }
     if (d.groups_v_ptr) {
-        if (!d.groups_v_idx) return bail(fail(CM_EINVAL, "groups_v_idx missing"));
-        int64_t n = Nv;
-        int64_t m = d.groups_v_ptr[n];
-        int64_t lim = d.autocorrelation ? d.n_groups_u : d.n_groups_v;
-        for (int64_t k = 0; k < m; k++)
-            if (d.groups_v_idx[k] < 0 || d.groups_v_idx[k] >= lim) return bail(fail(CM_EINVAL, "solvent group id out of range"));
+        const int64_t n = Nv, m = d.groups_v_ptr[n];
         ov_p = gm.size(); gm.insert(gm.end(), d.groups_v_ptr, d.groups_v_ptr + n + 1);
         ov_i = gm.size(); gm.insert(gm.end(), d.groups_v_idx, d.groups_v_idx + m); gm.push_back(0);
     }
     if (!gm.empty()) {
-        if ((r = ensure(J->gmap, 4 * gm.size()))) return bail(r);
-        if (hipMemcpy(J->gmap.p, gm.data(), 4 * gm.size(), hipMemcpyHostToDevice) != hipSuccess)
-            return bail(fail(CM_EHIP, "hipMemcpy of group maps failed"));
+        if ((r = ensure(J->gmap, 4 * gm.size()))) return r;
+        HIPCHK(hipMemcpyAsync(J->gmap.p, gm.data(), 4 * gm.size(), hipMemcpyHostToDevice, J->stream));
+        HIPCHK(hipStreamSynchronize(J->stream));
         int32_t* base = (int32_t*)J->gmap.p;
         if (ou_p >= 0) { P.gu_ptr = base + ou_p; P.gu_idx = base + ou_i; }
         if (ov_p >= 0) { P.gv_ptr = base + ov_p; P.gv_idx = base + ov_i; }
     }
     J->d.groups_u_ptr = J->d.groups_u_idx = J->d.groups_v_ptr = J->d.groups_v_idx = nullptr;
+    return CM_OK;
+}
+
+// flushes the pending weight and reads the accumulators back into J->hacc / J->hhits
+int job_collect(cm_job* J)
+{
+    if (J->pending) {
+        int r = flush(J, J->pending_w);
+        if (r) return r;
+        J->pending = false;
+    }
+    J->hacc.resize(J->acc_len);
+    J->hhits.resize(4 * (size_t)J->d.nbins);
+    HIPCHK(hipMemcpyAsync(J->hacc.data(), J->acc.p, 8 * J->acc_len, hipMemcpyDeviceToHost, J->stream));
+    HIPCHK(hipMemcpyAsync(J->hhits.data(), J->hits.p, 8 * 4 * J->d.nbins, hipMemcpyDeviceToHost, J->stream));
+    HIPCHK(hipStreamSynchronize(J->stream));
+    J->finished = true;
+    return CM_OK;
+}
+
+// raw accumulators of a collected job into the caller's cm_raw
+void fill_raw(const cm_job_desc& d, const std::vector<double>& acc, const std::vector<ull>& hits, double volume,
+              cm_raw* out)
+{
+    const int64_t nb = d.nbins, ngu = d.n_groups_u, ngv = d.n_groups_v;
+    const double* a = acc.data();
+    auto cp = [&](double* dst, const double* src, int64_t n) { if (dst) memcpy(dst, src, 8 * n); };
+    cp(out->md_count, a, nb);
+    cp(out->md_count_random, a + nb, nb);
+    cp(out->rdf_count, a + 2 * nb, nb);
+    cp(out->rdf_count_random, a + 3 * nb, nb);
+    const double* g = a + 4 * nb;
+    cp(out->solute_group_count, g, ngu * nb);
+    cp(out->solvent_group_count, g + ngu * nb, ngv * nb);
+    cp(out->solute_group_count_random, g + (ngu + ngv) * nb, ngu * nb);
+    cp(out->solvent_group_count_random, g + (2 * ngu + ngv) * nb, ngv * nb);
+    out->volume_total = volume;
+    auto cpu = [&](uint64_t* dst, const ull* src) { if (dst) memcpy(dst, src, 8 * nb); };
+    cpu(out->hits_md, hits.data());
+    cpu(out->hits_md_random, hits.data() + nb);
+    cpu(out->hits_rdf, hits.data() + 2 * nb);
+    cpu(out->hits_rdf_random, hits.data() + 3 * nb);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cm_last_error(void) { return g_err.c_str(); }
+
+int cm_job_create(const cm_job_desc* desc, cm_job** out)
+{
+    if (!desc || !out) return fail(CM_EINVAL, "null argument");
+    *out = nullptr;
+    int r = validate_desc(*desc);
+    if (r) return r;
+    cm_job* J = new cm_job();
+    if (hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking) != hipSuccess) {
+        cm_job_destroy(J);
+        return fail(CM_EHIP, "hipStreamCreate failed");
+    }
+    if ((r = job_setup(J, *desc))) {
+        cm_job_destroy(J);
+        return r;
+    }
     *out = J;
     return CM_OK;
 }
@@ -2100,35 +2207,10 @@ int cm_job_submit_device(cm_job* job, int32_t nframes, const int64_t* frame_inde
 int cm_job_finish(cm_job* J, cm_raw* out)
 {
     if (!J || !out) return fail(CM_EINVAL, "null argument");
-    if (J->pending) {
-        int r = flush(J, J->pending_w);
-        if (r) return r;
-        J->pending = false;
-    }
-    HIPCHK(hipStreamSynchronize(J->stream));
-    std::vector<double> acc(J->acc_len);
-    std::vector<ull> hits(4 * J->d.nbins);
-    HIPCHK(hipMemcpy(acc.data(), J->acc.p, 8 * J->acc_len, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(hits.data(), J->hits.p, 8 * 4 * J->d.nbins, hipMemcpyDeviceToHost));
-    int64_t nb = J->d.nbins, ngu = J->d.n_groups_u, ngv = J->d.n_groups_v;
-    const double* a = acc.data();
-    auto cp = [&](double* dst, const double* src, int64_t n) { if (dst) memcpy(dst, src, 8 * n); };
-    cp(out->md_count, a, nb);
-    cp(out->md_count_random, a + nb, nb);
-    cp(out->rdf_count, a + 2 * nb, nb);
-    cp(out->rdf_count_random, a + 3 * nb, nb);
-    const double* g = a + 4 * nb;
-    cp(out->solute_group_count, g, ngu * nb);
-    cp(out->solvent_group_count, g + ngu * nb, ngv * nb);
-    cp(out->solute_group_count_random, g + (ngu + ngv) * nb, ngu * nb);
-    cp(out->solvent_group_count_random, g + (2 * ngu + ngv) * nb, ngv * nb);
-    out->volume_total = J->volume_total;
-    auto cpu = [&](uint64_t* dst, const ull* src) { if (dst) memcpy(dst, src, 8 * nb); };
-    cpu(out->hits_md, hits.data());
-    cpu(out->hits_md_random, hits.data() + nb);
-    cpu(out->hits_rdf, hits.data() + 2 * nb);
-    cpu(out->hits_rdf_random, hits.data() + 3 * nb);
-    J->finished = true;
+    if (J->finished) return fail(CM_ESTATE, "job already finished");
+    int r = job_collect(J);
+    if (r) return r;
+    fill_raw(J->d, J->hacc, J->hhits, J->volume_total, out);
     return CM_OK;
 }
 
@@ -2165,8 +2247,8 @@ int cm_job_kernel_stats(cm_job* J, cm_kernel_stats* out)
 {
     if (!J || !out) return fail(CM_EINVAL, "null argument");
     ull st[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(st, J->stats.p, 16, hipMemcpyDeviceToHost, J->stream));
     HIPCHK(hipStreamSynchronize(J->stream));
-    HIPCHK(hipMemcpy(st, J->stats.p, 16, hipMemcpyDeviceToHost));
     out->eval_ms = J->eval_ms;
     out->eval_launches = J->eval_launches;
     out->total_ms = J->total_ms;
@@ -2184,7 +2266,8 @@ void cm_job_destroy(cm_job* J)
     DevBuf* bufs[] = {&J->cnt, &J->acc, &J->hits, &J->stats, &J->gmap, &J->geo, &J->cells_cnt, &J->cells_start,
                       &J->tmp4, &J->tmpcell, &J->sorted, &J->solw, &J->ucen, &J->vw, &J->vcell, &J->vcen, &J->P,
                       &J->vw_r, &J->vcell_r, &J->vcen_r, &J->bflag, &J->blist, &J->nbulk, &J->uk, &J->drawn_ptr,
-                      &J->drawn_idx, &J->xin_u, &J->xin_v};
+                      &J->drawn_idx, &J->xin_u, &J->xin_v, &J->near, &J->mkey, &J->mcnt, &J->mstart,
+                      &J->msorted, &J->nnear, &J->chunk_base};
     for (DevBuf* b : bufs) if (b->p) hipFree(b->p);
     if (J->e0) hipEventDestroy(J->e0);
     if (J->e1) hipEventDestroy(J->e1);
@@ -2217,6 +2300,260 @@ int cm_device_synchronize(void)
 {
     HIPCHK(hipDeviceSynchronize());
     return CM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// request server: slots (one stream and one pooled job each) take requests in
+// arrival order when the footprint estimates of the admitted ones fit the budget
+// ---------------------------------------------------------------------------
+namespace {
+
+struct SrvReq {
+    uint64_t id = 0;
+    cm_job_desc d{};
+    std::vector<int32_t> gup, gui, gvp, gvi;
+    std::vector<int64_t> fi;
+    std::vector<double> cells, w;
+    const void* xu = nullptr;
+    const void* xv = nullptr;
+    int32_t nframes = 0, dtype = 0;
+    double cf = 0.0;
+    int32_t cols = 0;
+    int64_t foot = 0;
+    bool done = false;
+    int status = CM_OK;
+    std::string err;
+    std::vector<double> acc;
+    std::vector<ull> hits;
+    double volume = 0.0;
+    cm_request_times t{};
+};
+
+int64_t job_footprint(const cm_job_desc& d, int32_t nframes, int32_t dtype, const double* cells, int64_t* bytes)
+{
+    cm_job J;
+    job_params(&J, d);
+    int64_t cap, kcap, Fmax, pfb;
+    int r = batch_plan(&J, nframes, cells, dtype == CM_F32 ? 4 : 8, false, &cap, &kcap, &Fmax, &pfb);
+    if (r) return r;
+    int64_t groups = 0;
+    if (d.groups_u_ptr) groups += 4 * ((int64_t)d.natoms_per_mol_u + 2 + d.groups_u_ptr[d.natoms_per_mol_u]);
+    if (d.groups_v_ptr) groups += 4 * (J.Nv + 2 + d.groups_v_ptr[J.Nv]);
+    *bytes = 16 * J.acc_len + 32 * (int64_t)d.nbins + 64 + groups + std::min<int64_t>(Fmax, nframes) * pfb;
+    return CM_OK;
+}
+
+}  // namespace
+
+struct cm_server {
+    int slots = 4;
+    int64_t budget = 0;
+    int device = 0;
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    std::deque<SrvReq*> queue;
+    std::unordered_map<uint64_t, SrvReq*> reqs;
+    int64_t reserved = 0;
+    bool stop = false;
+    uint64_t next = 1;
+    std::vector<cm_job*> shells;
+    std::vector<std::thread> workers;
+    std::chrono::steady_clock::time_point t0;
+    double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+namespace {
+
+int run_request(cm_job* J, SrvReq* q)
+{
+    cm_job_desc d = q->d;
+    d.groups_u_ptr = q->gup.empty() ? nullptr : q->gup.data();
+    d.groups_u_idx = q->gup.empty() ? nullptr : q->gui.data();
+    d.groups_v_ptr = q->gvp.empty() ? nullptr : q->gvp.data();
+    d.groups_v_idx = q->gvp.empty() ? nullptr : q->gvi.data();
+    int r = job_setup(J, d);
+    if (r) return r;
+    if (q->cf > 0.0 || q->cols > 0)
+        if ((r = cm_job_set_tuning(J, q->cf > 0.0 ? q->cf : 0.5, q->cols, 0))) return r;
+    if ((r = submit_common(J, q->nframes, q->fi.data(), q->xu, q->xv, q->dtype, q->cells.data(), q->w.data(), false)))
+        return r;
+    if ((r = job_collect(J))) return r;
+    q->acc = J->hacc;
+    q->hits = J->hhits;
+    q->volume = J->volume_total;
+    return CM_OK;
+}
+
+void serve_slot(cm_server* S, int k)
+{
+    hipSetDevice(S->device);
+    cm_job* J = S->shells[k];
+    for (;;) {
+        SrvReq* q;
+        {
+            std::unique_lock<std::mutex> lk(S->m);
+            S->cv_work.wait(lk, [&] {
+                if (S->queue.empty()) return S->stop;
+                // arrival order; a request larger than the budget runs alone
+                return S->reserved == 0 || S->reserved + S->queue.front()->foot <= S->budget;
+            });
+            if (S->queue.empty()) return;
+            q = S->queue.front();
+            S->queue.pop_front();
+            S->reserved += q->foot;
+            q->t.started = S->now();
+        }
+        const int r = run_request(J, q);
+        {
+            std::lock_guard<std::mutex> lk(S->m);
+            S->reserved -= q->foot;
+            q->status = r;
+            if (r) q->err = g_err;
+            q->t.finished = S->now();
+            q->done = true;
+        }
+        S->cv_done.notify_all();
+        S->cv_work.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cm_job_footprint(const cm_job_desc* desc, int32_t nframes, int32_t dtype, const double* cells, int64_t* bytes)
+{
+    if (!desc || !bytes || (nframes > 0 && !cells)) return fail(CM_EINVAL, "null argument");
+    if (nframes < 0) return fail(CM_EINVAL, "nframes must be >= 0");
+    if (dtype != CM_F64 && dtype != CM_F32) return fail(CM_EINVAL, "dtype must be CM_F64 or CM_F32");
+    int r = validate_desc(*desc);
+    if (r) return r;
+    return (int)job_footprint(*desc, nframes, dtype, cells, bytes);
+}
+
+int cm_server_create(const cm_server_desc* desc, cm_server** out)
+{
+    if (!out) return fail(CM_EINVAL, "null argument");
+    *out = nullptr;
+    const int slots = desc && desc->slots > 0 ? desc->slots : 4;
+    if (slots > 64) return fail(CM_EINVAL, "at most 64 slots");
+    cm_server* S = new cm_server();
+    S->slots = slots;
+    S->budget = desc && desc->memory_budget > 0 ? desc->memory_budget : (int64_t)200 << 30;
+    S->t0 = std::chrono::steady_clock::now();
+    if (hipGetDevice(&S->device) != hipSuccess) {
+        delete S;
+        return fail(CM_EHIP, "hipGetDevice failed");
+    }
+    for (int k = 0; k < slots; k++) {
+        cm_job* J = new cm_job();
+        if (hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking) != hipSuccess) {
+            cm_job_destroy(J);
+            cm_server_destroy(S);
+            return fail(CM_EHIP, "hipStreamCreate failed");
+        }
+        S->shells.push_back(J);
+    }
+    for (int k = 0; k < slots; k++) S->workers.emplace_back(serve_slot, S, k);
+    *out = S;
+    return CM_OK;
+}
+
+int cm_server_enqueue(cm_server* S, const cm_request* req, uint64_t* ticket)
+{
+    if (!S || !req || !req->job || !ticket) return fail(CM_EINVAL, "null argument");
+    const cm_job_desc& d = *req->job;
+    int r = validate_desc(d);
+    if (r) return r;
+    if (req->nframes < 1 || !req->frame_index || !req->d_x_solute || !req->d_x_solvent || !req->cells || !req->weights)
+        return fail(CM_EINVAL, "invalid frame arrays");
+    if (req->dtype != CM_F64 && req->dtype != CM_F32) return fail(CM_EINVAL, "dtype must be CM_F64 or CM_F32");
+    for (int f = 0; f < req->nframes; f++) {
+        if (!(req->weights[f] > 0.0) || !std::isfinite(req->weights[f])) return fail(CM_EINVAL, "frame weights must be positive and finite");
+        if (req->frame_index[f] < 1 || req->frame_index[f] > 0xFFFFFFFFll) return fail(CM_EINVAL, "frame_index must be in [1, 2^32)");
+    }
+    if (req->cell_fraction != 0.0 && !(req->cell_fraction >= 0.2 && req->cell_fraction <= 1.0))
+        return fail(CM_EINVAL, "cell_fraction must be in [0.2, 1]");
+    if (req->columns != 0 && req->columns != 1 && req->columns != 2 && req->columns != 4 && req->columns != 8)
+        return fail(CM_EINVAL, "columns must be 0 (default), 1, 2, 4 or 8");
+    SrvReq* q = new SrvReq();
+    q->d = d;
+    const int64_t Nv = (int64_t)d.nmols_v * d.natoms_per_mol_v;
+    if (d.groups_u_ptr) {
+        q->gup.assign(d.groups_u_ptr, d.groups_u_ptr + d.natoms_per_mol_u + 1);
+        q->gui.assign(d.groups_u_idx, d.groups_u_idx + d.groups_u_ptr[d.natoms_per_mol_u]);
+    }
+    if (d.groups_v_ptr) {
+        q->gvp.assign(d.groups_v_ptr, d.groups_v_ptr + Nv + 1);
+        q->gvi.assign(d.groups_v_idx, d.groups_v_idx + d.groups_v_ptr[Nv]);
+    }
+    q->d.groups_u_ptr = q->d.groups_u_idx = q->d.groups_v_ptr = q->d.groups_v_idx = nullptr;
+    q->fi.assign(req->frame_index, req->frame_index + req->nframes);
+    q->cells.assign(req->cells, req->cells + 9 * (size_t)req->nframes);
+    q->w.assign(req->weights, req->weights + req->nframes);
+    q->xu = req->d_x_solute;
+    q->xv = req->d_x_solvent;
+    q->nframes = req->nframes;
+    q->dtype = req->dtype;
+    q->cf = req->cell_fraction;
+    q->cols = req->columns;
+    if ((r = (int)job_footprint(d, req->nframes, req->dtype, req->cells, &q->foot))) {
+        delete q;
+        return r;
+    }
+    {
+        std::lock_guard<std::mutex> lk(S->m);
+        if (S->stop) {
+            delete q;
+            return fail(CM_ESTATE, "server is shutting down");
+        }
+        q->id = S->next++;
+        q->t.enqueued = S->now();
+        S->reqs[q->id] = q;
+        S->queue.push_back(q);
+        *ticket = q->id;
+    }
+    S->cv_work.notify_all();
+    return CM_OK;
+}
+
+int cm_server_wait(cm_server* S, uint64_t ticket, cm_raw* out, cm_request_times* times)
+{
+    if (!S) return fail(CM_EINVAL, "null server");
+    SrvReq* q;
+    {
+        std::unique_lock<std::mutex> lk(S->m);
+        auto it = S->reqs.find(ticket);
+        if (it == S->reqs.end()) return fail(CM_EINVAL, "unknown ticket");
+        q = it->second;
+        S->cv_done.wait(lk, [&] { return q->done; });
+        S->reqs.erase(it);
+    }
+    if (times) *times = q->t;
+    const int st = q->status;
+    if (st == CM_OK) {
+        if (out) fill_raw(q->d, q->acc, q->hits, q->volume, out);
+    } else {
+        g_err = q->err;
+    }
+    delete q;
+    return st;
+}
+
+void cm_server_destroy(cm_server* S)
+{
+    if (!S) return;
+    {
+        std::lock_guard<std::mutex> lk(S->m);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    for (auto& t : S->workers) t.join();
+    for (cm_job* J : S->shells) cm_job_destroy(J);
+    for (auto& kv : S->reqs) delete kv.second;
+    delete S;
 }
 
 }  // extern "C"

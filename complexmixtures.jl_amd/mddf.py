@@ -47,7 +47,8 @@ class TrajectoryMetaData:
         self.n_groups_solvent = trajectory.solvent.ngroups
 
 
-def _job_for(R, options, tmeta, coordination_number_only, **kw):
+def job_kwargs(R, options, tmeta, coordination_number_only):
+    """Arguments of a device job (cm_job_desc) for Result R of this trajectory."""
     solute, solvent = R.solute, R.solvent
     nu = solute.natomspermol
     Nv = solvent.nmols * solvent.natomspermol
@@ -55,14 +56,18 @@ def _job_for(R, options, tmeta, coordination_number_only, **kw):
     # self: the solvent-side atom j counts into the solute groups (update_counters.jl:48-53)
     groups_v = group_csr(solvent, list(range(Nv)), solute if R.autocorrelation else solvent)
     seed = options.seed if options.seed > 0 else secrets.randbits(63) + 1
-    return _lib.Job(nmols_u=solute.nmols, natoms_per_mol_u=nu, nmols_v=solvent.nmols,
-                    natoms_per_mol_v=solvent.natomspermol, autocorrelation=R.autocorrelation,
-                    irefatom=tmeta.irefatom, nbins=R.nbins, binstep=options.binstep, cutoff=options.cutoff,
-                    dbulk=options.dbulk, usecutoff=options.usecutoff, n_random_samples=options.n_random_samples,
-                    coordination_number_only=coordination_number_only, triclinic=not tmeta.orthorhombic, seed=seed,
-                    n_groups_u=tmeta.n_groups_solute,
-                    n_groups_v=tmeta.n_groups_solute if R.autocorrelation else tmeta.n_groups_solvent,
-                    groups_u=groups_u, groups_v=groups_v, lcell=options.lcell, **kw)
+    return dict(nmols_u=solute.nmols, natoms_per_mol_u=nu, nmols_v=solvent.nmols,
+                natoms_per_mol_v=solvent.natomspermol, autocorrelation=R.autocorrelation,
+                irefatom=tmeta.irefatom, nbins=R.nbins, binstep=options.binstep, cutoff=options.cutoff,
+                dbulk=options.dbulk, usecutoff=options.usecutoff, n_random_samples=options.n_random_samples,
+                coordination_number_only=coordination_number_only, triclinic=not tmeta.orthorhombic, seed=seed,
+                n_groups_u=tmeta.n_groups_solute,
+                n_groups_v=tmeta.n_groups_solute if R.autocorrelation else tmeta.n_groups_solvent,
+                groups_u=groups_u, groups_v=groups_v, lcell=options.lcell)
+
+
+def _job_for(R, options, tmeta, coordination_number_only, **kw):
+    return _lib.Job(**job_kwargs(R, options, tmeta, coordination_number_only), **kw)
 
 
 def _store_raw(R, raw):

@@ -1,0 +1,229 @@
+"""Serving MDDF requests on one GPU.
+
+Server: continuous batching through the request server of libcmgpu
+(cm_server_*): requests are admitted in arrival order as slots and memory free
+up, each slot drives its own stream, so the work of admitted requests
+overlaps on the device.  Plain path: one job at a time (cm_job_create,
+cm_job_submit_device, cm_job_finish, cm_job_destroy), the way a single mddf
+call runs.  Both read the same device-resident systems; each request's counts
+are the same either way.
+
+The request stream is the uneven mixed-size workload: Poisson arrivals,
+systems whose solute size is log-uniform over 200-20,000 atoms in cubic water
+boxes of 30-150 A, 10-200 frames per request, 30% with the ideal-gas reference
+passes (n_random_samples 10; the others are coordination-number requests) and
+20% self-MDDF (water vs water).  The autotuner picks the cell size and the
+tile (columns per atom row) of the single-solute kernels per system by timing
+them on a few frames.
+"""
+import hashlib
+import queue
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .results import setbin
+from .synthetic import served_system
+
+CUTOFF = 10.0
+BINSTEP = 0.02
+N_RANDOM = 10
+
+
+@dataclass
+class System:
+    n_solute: int
+    n_water: int
+    box: float
+    nframes: int
+    d_solute: object
+    d_water: object
+    cells: np.ndarray
+    irefatom: int
+    tuning: dict = field(default_factory=dict)
+    host: dict = None
+
+
+def default_irefatom(x_water_frame0):
+    """The reference's default irefatom (src/Trajectory.jl:206-211): the atom of
+    the first solvent molecule closest to that molecule's centre, frame 1."""
+    m = np.asarray(x_water_frame0[:3], dtype=np.float64)
+    return int(np.argmin(np.linalg.norm(m - m.mean(axis=0), axis=1))) + 1
+
+
+def make_pool(n_systems, seed, nframes, n_solute_range=(200, 20000), box_range=(30.0, 150.0), dtype=np.float32,
+              keep_host=False):
+    """Systems of the uneven distribution, uploaded once to device memory
+    (keep_host: also keep the host coordinates)."""
+    rng = np.random.default_rng(seed)
+    pool = []
+    for k in range(n_systems):
+        lo, hi = np.log(n_solute_range[0]), np.log(n_solute_range[1])
+        nu = int(round(np.exp(rng.uniform(lo, hi))))
+        box = float(rng.uniform(*box_range))
+        s = served_system(seed * 1000 + k, nu, box, nframes, dtype=dtype)
+        pool.append(System(n_solute=nu, n_water=s["n_water"], box=box, nframes=nframes,
+                           d_solute=_lib.DeviceArray(s["x_solute"]), d_water=_lib.DeviceArray(s["x_solvent"]),
+                           cells=s["cells"], irefatom=default_irefatom(s["x_solvent"][0]),
+                           host=s if keep_host else None))
+    return pool
+
+
+@dataclass
+class Request:
+    arrival: float
+    system: int
+    nframes: int
+    mddf: bool
+    self_: bool
+    seed: int
+
+
+def request_stream(rng, n, n_systems, max_frames, rate=20.0, frames=(10, 200), p_mddf=0.3, p_self=0.2,
+                   saturate=False):
+    """n requests; arrivals are a Poisson process of the given rate (all at 0
+    when saturate)."""
+    t = 0.0
+    out = []
+    for i in range(n):
+        if not saturate:
+            t += rng.exponential(1.0 / rate)
+        out.append(Request(arrival=t, system=int(rng.integers(n_systems)),
+                           nframes=int(min(max_frames, rng.integers(frames[0], frames[1] + 1))),
+                           mddf=bool(rng.random() < p_mddf), self_=bool(rng.random() < p_self),
+                           seed=int(rng.integers(1, 2 ** 62))))
+    return out
+
+
+def job_kwargs(system, req):
+    """Job arguments and device inputs of one request (default Options with
+    cutoff = dbulk = 10 A, binstep 0.02 A; default groups)."""
+    common = dict(nbins=setbin(CUTOFF, BINSTEP), binstep=BINSTEP, cutoff=CUTOFF, dbulk=CUTOFF, usecutoff=False,
+                  n_random_samples=N_RANDOM, coordination_number_only=not req.mddf, triclinic=False, seed=req.seed,
+                  irefatom=system.irefatom)
+    if req.self_:
+        kw = dict(nmols_u=system.n_water, natoms_per_mol_u=3, nmols_v=system.n_water, natoms_per_mol_v=3,
+                  autocorrelation=True, n_groups_u=3, n_groups_v=3, **common)
+        return kw, system.d_water, system.d_water, {}
+    kw = dict(nmols_u=1, natoms_per_mol_u=system.n_solute, nmols_v=system.n_water, natoms_per_mol_v=3,
+              autocorrelation=False, n_groups_u=system.n_solute, n_groups_v=3, **common)
+    return kw, system.d_solute, system.d_water, system.tuning
+
+
+def digest(raw):
+    """SHA-1 over every raw accumulator (bit patterns) and the volume."""
+    h = hashlib.sha1()
+    for k in sorted(raw):
+        if k == "volume_total":
+            h.update(np.float64(raw[k]).tobytes())
+        else:
+            h.update(np.ascontiguousarray(raw[k]).tobytes())
+    return h.hexdigest()
+
+
+def _summary(lat, t_first, t_last, n):
+    lat = np.sort(np.asarray(lat))
+    span = max(t_last - t_first, 1e-9)
+    return {"jobs_per_s": n / span, "makespan_s": span, "p50_s": float(np.percentile(lat, 50)),
+            "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean())}
+
+
+def run_served(pool, reqs, slots=4, memory_budget=0, keep_raw=False):
+    """Requests enqueued at their arrival times on a cm_server; latency is
+    completion minus arrival."""
+    srv = _lib.Server(slots, memory_budget)
+    handoff = queue.Queue()
+    res = [None] * len(reqs)
+
+    def collect():
+        while True:
+            item = handoff.get()
+            if item is None:
+                return
+            i, tk = item
+            raw, tm = srv.wait(tk)
+            res[i] = (tm, digest(raw), raw if keep_raw else None)
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t0 = time.perf_counter()
+    try:
+        for i, r in enumerate(reqs):
+            dt = t0 + r.arrival - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            s = pool[r.system]
+            kw, du, dv, tuning = job_kwargs(s, r)
+            tk = srv.enqueue(np.arange(1, r.nframes + 1), du, dv, 0, s.cells[:r.nframes], np.ones(r.nframes),
+                             tuning=tuning, **kw)
+            handoff.put((i, tk))
+    finally:
+        handoff.put(None)
+        th.join()
+        srv.close()
+    lat = [tm["finished"] - tm["enqueued"] for tm, _, _ in res]
+    out = _summary(lat, min(tm["enqueued"] for tm, _, _ in res), max(tm["finished"] for tm, _, _ in res), len(reqs))
+    out["digests"] = [d for _, d, _ in res]
+    if keep_raw:
+        out["raw"] = [x for _, _, x in res]
+    return out
+
+
+def run_plain(pool, reqs, keep_raw=False):
+    """The same requests one at a time (each starts when it has arrived and the
+    previous one is done)."""
+    t0 = time.perf_counter()
+    lat, digests, raws, fin = [], [], [], []
+    for r in reqs:
+        dt = t0 + r.arrival - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        s = pool[r.system]
+        kw, du, dv, tuning = job_kwargs(s, r)
+        job = _lib.Job(tuning=tuning or None, **kw)
+        job.submit_device(np.arange(1, r.nframes + 1), du, dv, 0, s.cells[:r.nframes], np.ones(r.nframes))
+        raw = job.finish()
+        job.close()
+        t = time.perf_counter() - t0
+        fin.append(t)
+        lat.append(t - r.arrival)
+        digests.append(digest(raw))
+        if keep_raw:
+            raws.append(raw)
+    out = _summary(lat, min(r.arrival for r in reqs), max(fin), len(reqs))
+    out["digests"] = digests
+    if keep_raw:
+        out["raw"] = raws
+    return out
+
+
+TUNING_CANDIDATES = [dict(cell_fraction=cf, columns=c) for cf in (0.35, 0.5, 0.75, 1.0) for c in (1, 2)]
+
+
+def autotune(system, frames=4, candidates=TUNING_CANDIDATES):
+    """Times the single-solute kernels of this system (ideal-gas passes
+    included) for each candidate tuning on the first frames and keeps the
+    fastest; every candidate must give the same counts."""
+    req = Request(arrival=0.0, system=0, nframes=min(frames, system.nframes), mddf=True, self_=False, seed=12345)
+    kw, du, dv, _ = job_kwargs(system, req)
+    best, ref, timing = None, None, []
+    for cand in candidates:
+        job = _lib.Job(tuning=cand, **kw)
+        job.set_profiling(True)
+        job.submit_device(np.arange(1, req.nframes + 1), du, dv, 0, system.cells[:req.nframes], np.ones(req.nframes))
+        raw = job.finish()
+        ms = job.stats()["total_ms"]
+        job.close()
+        d = digest(raw)
+        if ref is None:
+            ref = d
+        elif d != ref:
+            raise RuntimeError(f"tuning {cand} changed the counts")
+        timing.append((ms, cand))
+        if best is None or ms < best[0]:
+            best = (ms, cand)
+    system.tuning = dict(best[1])
+    return system.tuning, timing

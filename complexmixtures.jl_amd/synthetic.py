@@ -183,3 +183,27 @@ def water_box(nframes, n_solute_atoms, box, seed, density=0.0334, dtype=np.float
     cent = rng.uniform(0, box, size=(n_water, 3))
     water = place_molecules(rng, water_template(), cent)
     return _pack(sol, water, np.diag([box] * 3), nframes, 1, n_water, rng, dtype)
+
+
+def served_system(seed, n_solute, box, nframes, density=0.0334, sigma=0.3, dtype=np.float32, noise_frames=8):
+    """A system of the uneven batched distribution: a random-walk solute of
+    n_solute atoms started at the box centre and water at the given molecular
+    density in a cubic box.  Frames are the base coordinates plus N(0, sigma)
+    jitter; the jitter of frame f is one of noise_frames draws, rolled over the
+    atoms by a frame-dependent offset (so frames differ without drawing
+    nframes x natoms normals)."""
+    rng = np.random.default_rng(seed)
+    centre = np.full(3, box / 2)
+    sol = random_walk(rng, n_solute, bond=1.5, rmin=2.0, start=centre)
+    n_water = max(2, int(density * box ** 3))
+    water = place_molecules(rng, water_template(), rng.uniform(0, box, size=(n_water, 3)))
+    base = np.concatenate([sol, water]).astype(dtype)
+    k = min(noise_frames, nframes)
+    noise = rng.normal(scale=sigma, size=(k,) + base.shape).astype(dtype)
+    x = np.empty((nframes,) + base.shape, dtype=dtype)
+    for f in range(nframes):
+        x[f] = base + np.roll(noise[f % k], (f // k) * 7919, axis=0)
+    nu = len(sol)
+    return {"x_solute": np.ascontiguousarray(x[:, :nu]), "x_solvent": np.ascontiguousarray(x[:, nu:]),
+            "cells": np.repeat(np.diag([float(box)] * 3)[None], nframes, axis=0), "n_solute": nu, "n_water": n_water,
+            "box": float(box)}

@@ -152,6 +152,62 @@ int cm_device_upload(void *dptr, const void *host, size_t bytes);
 int cm_device_free(void *dptr);
 int cm_device_synchronize(void);
 
+/* Device bytes a job would hold while it runs these frames (the admission
+ * estimate of the request server).  No counterpart in the reference. */
+int cm_job_footprint(const cm_job_desc *desc, int32_t nframes, int32_t dtype, const double *cells, int64_t *bytes);
+
+/* ------------------------------------------------------------------------
+ * Request server: continuous batching of MDDF jobs on one GPU.  No
+ * counterpart in the reference (whose unit of work is one mddf call).
+ *
+ * Requests are admitted in arrival order as soon as a slot is free and the
+ * footprint estimates of the admitted requests fit the memory budget.  Each
+ * slot drives its own HIP stream and keeps its device buffers between
+ * requests, so admitted requests overlap on the device and steady-state
+ * admission allocates nothing.  Results do not depend on what else runs.
+ * ------------------------------------------------------------------------ */
+typedef struct cm_server cm_server;
+
+typedef struct cm_server_desc {
+    int32_t slots;           /* requests on the device at once (0 = 4) */
+    int32_t pad;
+    int64_t memory_budget;   /* bytes of device memory for admitted requests (0 = 200 GiB) */
+} cm_server_desc;
+
+/* One request: a job description plus device-resident frames, as
+ * cm_job_create + cm_job_submit_device + cm_job_finish.  The description,
+ * its group maps, frame_index, cells and weights are copied at enqueue; the
+ * device coordinates must stay valid until the request completes. */
+typedef struct cm_request {
+    const cm_job_desc *job;
+    int32_t nframes;
+    int32_t dtype;
+    const int64_t *frame_index;
+    const void *d_x_solute;
+    const void *d_x_solvent;
+    const double *cells;
+    const double *weights;
+    double cell_fraction;    /* tuning as cm_job_set_tuning (0 = default) */
+    int32_t columns;         /* 0 = default */
+    int32_t pad;
+} cm_request;
+
+/* Seconds since the server was created (steady clock). */
+typedef struct cm_request_times {
+    double enqueued;
+    double started;
+    double finished;
+} cm_request_times;
+
+int cm_server_create(const cm_server_desc *desc, cm_server **out);
+/* Checks the request as cm_job_create would (same errors) and queues it. */
+int cm_server_enqueue(cm_server *srv, const cm_request *req, uint64_t *ticket);
+/* Blocks until the request completes; returns the request's own status and,
+ * on success, its raw accumulators.  A ticket can be waited for once. */
+int cm_server_wait(cm_server *srv, uint64_t ticket, cm_raw *out, cm_request_times *times);
+/* Completes the queued requests, then releases the slots. */
+void cm_server_destroy(cm_server *srv);
+
 #ifdef __cplusplus
 }
 #endif
