@@ -59,6 +59,46 @@ def cpu_baseline(pkg, O, s, opts, tm, nframes_sample, threads):
     return f / dt, dt
 
 
+def served_stream(args):
+    """Served mixed-size stream (Poisson arrivals at 20 requests/s) through the
+    request server vs the plain one-job-at-a-time path, plus both paths'
+    capacity with every request queued at once; each request's counts must be
+    the same on both paths."""
+    import complexmixtures_amd.serve as SV
+    t0 = time.perf_counter()
+    pool = SV.make_pool(args.serve_systems, 20261206, args.serve_frames)
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tunings = [SV.autotune(s, frames=4)[0] for s in pool]
+    tune_s = time.perf_counter() - t0
+    rng = np.random.default_rng(20261207)
+    reqs = SV.request_stream(rng, args.serve_requests, len(pool), args.serve_frames, rate=20.0)
+    sat = [SV.Request(0.0, r.system, r.nframes, r.mddf, r.self_, r.seed) for r in reqs]
+    SV.run_plain(pool, [r for r in sat if r.self_][:1] + sat[:1])          # first use of every kernel
+    plain = SV.run_plain(pool, reqs)
+    served = SV.run_served(pool, reqs, slots=args.slots)
+    plain_cap = SV.run_plain(pool, sat)
+    served_cap = SV.run_served(pool, sat, slots=args.slots)
+    same = plain["digests"] == served["digests"] == plain_cap["digests"] == served_cap["digests"]
+    keep = ("jobs_per_s", "p50_s", "p99_s", "mean_s", "makespan_s")
+    r3 = lambda d: {k: round(d[k], 4) for k in keep}
+    frames = sum(r.nframes for r in reqs)
+    out = {"workload": f"{len(reqs)} requests over {len(pool)} systems (solute 200-20,000 atoms log-uniform, cubic "
+                       "water boxes 30-150 A), 10-200 frames each, 30% with 10 ideal-gas passes, 20% self-MDDF, "
+                       "Poisson arrivals 20/s",
+           "frames_total": frames, "slots": args.slots, "counts_identical": bool(same),
+           "served": r3(served), "plain": r3(plain),
+           "capacity": {"served_jobs_per_s": round(served_cap["jobs_per_s"], 3),
+                        "plain_jobs_per_s": round(plain_cap["jobs_per_s"], 3),
+                        "ratio": round(served_cap["jobs_per_s"] / plain_cap["jobs_per_s"], 3),
+                        "served_p99_s": round(served_cap["p99_s"], 4), "plain_p99_s": round(plain_cap["p99_s"], 4)},
+           "tunings": tunings, "pool_gen_s": round(gen_s, 2), "autotune_s": round(tune_s, 2)}
+    for s in pool:
+        s.d_solute.free()
+        s.d_water.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +111,12 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="only time the evaluation kernels per tuning, then exit")
     ap.add_argument("--cell-fraction", type=float, default=0.0, help="tuning: fine cell side / cutoff (0: default)")
     ap.add_argument("--columns", type=int, default=0, help="tuning: columns per solvent atom row (0: default)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--serve-only", action="store_true", help="only the served-stream measurement")
+    ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--serve-systems", type=int, default=12)
+    ap.add_argument("--serve-frames", type=int, default=200)
+    ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain read-N/write-N elementwise kernel over MIB MiB three times (the "
                          "reference for scaling FETCH_SIZE/WRITE_SIZE counters), then exit")
@@ -91,6 +137,9 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 3}))
         return
     pkg = G.load_package()
+    if args.serve_only:
+        print(json.dumps({"served_stream": served_stream(args)}), flush=True)
+        return
     import complexmixtures_amd.synthetic as S
     from complexmixtures_amd import _lib
     from complexmixtures_amd.mddf import TrajectoryMetaData, _job_for
@@ -187,9 +236,11 @@ def main():
         out["cpu_baseline"] = {"value": round(v, 4), "unit": "frames/s", "cores": threads, "kind": "port",
                                "sample": f"first {f} frames of the same trajectory, {dt:.1f} s",
                                "gpu_over_cpu": round(fps / v, 1)}
-    print(json.dumps(out), flush=True)
     du.free()
     dv.free()
+    if not args.no_serve:
+        out["served_stream"] = served_stream(args)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
