@@ -85,6 +85,7 @@ struct Geo {                 // one frame of a batch
     float rstage;            // staging radius around a chunk (cutoff + margin), tiles family
     float rnear;             // near-cell radius around a solute atom (cutoff + radius + margin)
     int kbits;               // bits per dimension of the Morton keys of the fine cells
+    int ortho;               // diagonal cell matrix: cells are boxes (exact near-cell test)
 };
 
 struct KP {                  // job parameters, by value
@@ -553,10 +554,15 @@ struct Best {
     float best, second;
     int bi, ba;
     __device__ __forceinline__ void init() { best = INFINITY; second = INFINITY; bi = -1; ba = -1; }
+    // branch-free: second smallest = median of (best, second, d2) while
+    // best <= second; selects rather than a branch keep the loop straight
     __device__ __forceinline__ void offer(float d2, int i, int a)
     {
-        second = fminf(second, fmaxf(best, d2));
-        if (d2 < best) { best = d2; bi = i; ba = a; }
+        const bool lt = d2 < best;
+        second = __builtin_amdgcn_fmed3f(best, second, d2);
+        best = lt ? d2 : best;
+        bi = lt ? i : bi;
+        ba = lt ? a : ba;
     }
 };
 
@@ -920,12 +926,22 @@ __global__ void k_near_mark(KP P, Bufs B)
         int ilo[3], ihi[3];
         cell_range(g, s.x, s.y, s.z, s.x, s.y, s.z, g.rnear, ilo, ihi);
         unsigned char* nr = B.near + (int64_t)f * P.cap;
+        // boxes of a diagonal cell: only the cells whose box is within rnear
+        // (in the image of index i); otherwise the whole index box
+        const float rn2 = g.rnear * g.rnear;
+        const float hx = g.H32[0] / g.nc[0], hy = g.H32[4] / g.nc[1], hz = g.H32[8] / g.nc[2];
+        auto gap = [](float v, float lo, float hi) { return fmaxf(fmaxf(lo - v, v - hi), 0.f); };
         for (int iz = ilo[2]; iz <= ihi[2]; iz++) {
             int mz; int cz = wrap_index(iz, g.nc[2], mz);
+            const float dz = g.ortho ? gap(s.z, iz * hz, (iz + 1) * hz) : 0.f;
             for (int iy = ilo[1]; iy <= ihi[1]; iy++) {
                 int my; int cy = wrap_index(iy, g.nc[1], my);
+                const float dy = g.ortho ? gap(s.y, iy * hy, (iy + 1) * hy) : 0.f;
+                if (dz * dz + dy * dy > rn2) continue;
                 for (int ix = ilo[0]; ix <= ihi[0]; ix++) {
                     int mx; int cx = wrap_index(ix, g.nc[0], mx);
+                    const float dx = g.ortho ? gap(s.x, ix * hx, (ix + 1) * hx) : 0.f;
+                    if (dz * dz + dy * dy + dx * dx > rn2) continue;
                     nr[(cz * g.nc[1] + cy) * g.nc[0] + cx] = 1;
                 }
             }
@@ -1133,23 +1149,52 @@ __global__ void k_place_near(KP P, Bufs B, const TV* xv)
     }
 }
 
+// Wavefront scans and reductions with DPP row shifts and row broadcasts
+// (row_shr:1,2,4,8 within rows of 16 lanes, then row_bcast:15 into rows 1
+// and 3 and row_bcast:31 into rows 2 and 3); lanes without a source read the
+// identity.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i(int identity, int x)
+{
+    return __builtin_amdgcn_update_dpp(identity, x, CTRL, ROWS, 0xf, false);
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    v += dpp_i<0x111, 0xf>(0, v);
+    v += dpp_i<0x112, 0xf>(0, v);
+    v += dpp_i<0x114, 0xf>(0, v);
+    v += dpp_i<0x118, 0xf>(0, v);
+    v += dpp_i<0x142, 0xa>(0, v);
+    v += dpp_i<0x143, 0xc>(0, v);
+    return v;
+}
+
+__device__ __forceinline__ float fmin_sel(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float fmax_sel(float a, float b) { return b > a ? b : a; }
+
+// minimum / maximum over the wavefront, the same in every lane
 __device__ __forceinline__ float wave_min(float x)
 {
-    for (int o = 32; o >= 1; o >>= 1) x = fminf(x, __shfl_xor(x, o, 64));
-    return x;
+    const int id = __float_as_int(INFINITY);
+    x = fmin_sel(x, __int_as_float(dpp_i<0x111, 0xf>(id, __float_as_int(x))));
+    x = fmin_sel(x, __int_as_float(dpp_i<0x112, 0xf>(id, __float_as_int(x))));
+    x = fmin_sel(x, __int_as_float(dpp_i<0x114, 0xf>(id, __float_as_int(x))));
+    x = fmin_sel(x, __int_as_float(dpp_i<0x118, 0xf>(id, __float_as_int(x))));
+    x = fmin_sel(x, __int_as_float(dpp_i<0x142, 0xa>(id, __float_as_int(x))));
+    x = fmin_sel(x, __int_as_float(dpp_i<0x143, 0xc>(id, __float_as_int(x))));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
 }
 __device__ __forceinline__ float wave_max(float x)
 {
-    for (int o = 32; o >= 1; o >>= 1) x = fmaxf(x, __shfl_xor(x, o, 64));
-    return x;
-}
-__device__ __forceinline__ int wave_incl_scan(int v, int lane)
-{
-    for (int o = 1; o < 64; o <<= 1) {
-        int t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
+    const int id = __float_as_int(-INFINITY);
+    x = fmax_sel(x, __int_as_float(dpp_i<0x111, 0xf>(id, __float_as_int(x))));
+    x = fmax_sel(x, __int_as_float(dpp_i<0x112, 0xf>(id, __float_as_int(x))));
+    x = fmax_sel(x, __int_as_float(dpp_i<0x114, 0xf>(id, __float_as_int(x))));
+    x = fmax_sel(x, __int_as_float(dpp_i<0x118, 0xf>(id, __float_as_int(x))));
+    x = fmax_sel(x, __int_as_float(dpp_i<0x142, 0xa>(id, __float_as_int(x))));
+    x = fmax_sel(x, __int_as_float(dpp_i<0x143, 0xc>(id, __float_as_int(x))));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
 }
 
 // lexicographic minimum of two exact records, OR of the reference flags
@@ -1273,27 +1318,34 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
                 cnt = st[cell + 1] - cstart;
                 shift_of(g, mx, my, mz, tx, ty, tz);
             }
-            const int incl = wave_incl_scan(cnt, lane);
+            const int incl = wave_incl_scan(cnt);
             const int excl = incl - cnt;
-            const int T = __shfl(incl, 63, 64);
+            const int T = __builtin_amdgcn_readlane(incl, 63);
             for (int t0 = 0; t0 < T; t0 += 64 * SU) {
-                float4 q[SU];
+                // owners of the ranks (largest j with excl[j] <= t), then all
+                // loads in flight before any is used
+                int sidx[SU];
+                float sx[SU], sy[SU], sz[SU];
 #pragma unroll
                 for (int u = 0; u < SU; u++) {
                     const int t = t0 + 64 * u + lane;
-                    int j = 0;   // owner cell: largest j with excl[j] <= t
+                    int j = 0;
+#pragma unroll
                     for (int step = 32; step >= 1; step >>= 1) {
                         const int e = __shfl(excl, (j + step) & 63, 64);
-                        if (j + step < 64 && e <= t) j += step;
+                        j = (j + step < 64 && e <= t) ? j + step : j;
                     }
                     const int sj = __shfl(cstart, j, 64), ej = __shfl(excl, j, 64);
-                    const float txj = __shfl(tx, j, 64), tyj = __shfl(ty, j, 64), tzj = __shfl(tz, j, 64);
-                    q[u] = mk4(INFINITY, INFINITY, INFINITY, 0.f);
-                    if (t < T) {
-                        const float4 sa = srt[sj + (t - ej)];
-                        q[u] = mk4(sa.x + txj, sa.y + tyj, sa.z + tzj, sa.w);
-                    }
+                    sx[u] = __shfl(tx, j, 64);
+                    sy[u] = __shfl(ty, j, 64);
+                    sz[u] = __shfl(tz, j, 64);
+                    sidx[u] = t < T ? sj + (t - ej) : 0;
                 }
+                float4 q[SU];
+#pragma unroll
+                for (int u = 0; u < SU; u++) q[u] = srt[sidx[u]];
+#pragma unroll
+                for (int u = 0; u < SU; u++) q[u] = mk4(q[u].x + sx[u], q[u].y + sy[u], q[u].z + sz[u], q[u].w);
 #pragma unroll
                 for (int u = 0; u < SU; u++) {
                     if (t0 + 64 * u >= T) break;
@@ -1640,6 +1692,8 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
             g.Hi32[k] = (float)g.cell.Hi[k];
         }
         plane_spacings(g.cell.H, g.sp);
+        g.ortho = (g.cell.H[1] == 0.0 && g.cell.H[2] == 0.0 && g.cell.H[3] == 0.0 && g.cell.H[5] == 0.0 &&
+                   g.cell.H[6] == 0.0 && g.cell.H[7] == 0.0) ? 1 : 0;
         double spmin = std::min(g.sp[0], std::min(g.sp[1], g.sp[2]));
         if (!(cutsys < 0.5 * spmin * (1.0 - 1e-9)))
             return fail(CM_EUNSUPPORTED, "cutoff must be smaller than half of the smallest plane spacing of the unit cell");
