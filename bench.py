@@ -92,6 +92,7 @@ def served_stream(args):
                         "plain_jobs_per_s": round(plain_cap["jobs_per_s"], 3),
                         "ratio": round(served_cap["jobs_per_s"] / plain_cap["jobs_per_s"], 3),
                         "served_p99_s": round(served_cap["p99_s"], 4), "plain_p99_s": round(plain_cap["p99_s"], 4)},
+           "plain_service_by_class": SV.service_breakdown(sat, plain_cap["service_s"]),
            "tunings": tunings, "pool_gen_s": round(gen_s, 2), "autotune_s": round(tune_s, 2)}
     for s in pool:
         s.d_solute.free()
@@ -204,6 +205,13 @@ def main():
     el = time.perf_counter() - t
     fps = nf * args.steps / el
     _, st = one_job(profile=True)
+    # the same job from host buffers (cm_job_submit: coordinates cross PCIe per batch)
+    job = _job_for(R, opts, tm, False, tuning=tuning)
+    t1 = time.perf_counter()
+    job.submit(frames, s["x_solute"], s["x_solvent"], cells, weights)
+    job.finish()
+    host_fps = nf / (time.perf_counter() - t1)
+    job.close()
 
     n_u = s["solute"].natomspermol
     n_v = s["solvent"].nmols * s["solvent"].natomspermol
@@ -226,6 +234,7 @@ def main():
                      "eval_ms_per_frame": round(st["eval_ms"] / nf, 5),
                      "hbm_GBps_algorithmic": round(bpf * nf / eval_s / 1e9, 2),
                      "fp64_exhaustive_resolves": st["fp64_rechecks"]},
+        "host_input_frames_per_s": round(host_fps, 2),
         "gen_s": round(gen_s, 2),
     }
     if not args.no_cpu:

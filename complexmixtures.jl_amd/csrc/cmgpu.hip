@@ -86,6 +86,7 @@ struct Geo {                 // one frame of a batch
     float rnear;             // near-cell radius around a solute atom (cutoff + radius + margin)
     int kbits;               // bits per dimension of the Morton keys of the fine cells
     int ortho;               // diagonal cell matrix: cells are boxes (exact near-cell test)
+    double guard;            // bound on |d2(FP32) - d2(FP64)| for pairs up to the cutoff
 };
 
 struct KP {                  // job parameters, by value
@@ -326,6 +327,7 @@ __global__ void k_grid(KP P, Bufs B, int mols_family)
     double E = 10.0 * halfulp32(2.0 * xm) + 1e-7;
     double dmax = P.cutsys + 0.5;
     double G = E * (2.0 * 1.7320508075688772 * dmax + 3.0 * E) + 8.0 * 0x1p-24 * dmax * dmax;
+    g.guard = G;
     float hi = (float)(P.cut2 + G);
     g.cut2_hi = nextafterf(hi, INFINITY);
     float ulpc = (float)(halfulp32((double)g.cut2_hi) * 2.0);
@@ -549,6 +551,24 @@ __device__ __forceinline__ bool inbulk(const KP& P, const Rec& r)
     return P.usecutoff ? (r.within && r.d > P.dbulk) : !r.within;
 }
 
+// A record from the FP32 minimum b of a unique argmin, without the FP64
+// recompute, when every FP64 d2 in [b - G, b + G] gives the same cutoff
+// test, bin and dbulk test (all monotone in d2): then *within is set and *d
+// is a distance with the bin and dbulk test of the FP64 one.  Returns false
+// when the interval straddles a decision and the pair must be recomputed.
+__device__ __forceinline__ bool decide_fp32(const KP& P, double G, float b, bool& within, double& d)
+{
+    const double lo = fmax((double)b - G, 0.0), hi = (double)b + G;
+    if (lo > P.cut2) { within = false; return true; }
+    if (!(hi <= P.cut2)) return false;
+    const double dlo = sqrt(lo), dhi = sqrt(hi);
+    if (cm::setbin(dlo, P.binstep) != cm::setbin(dhi, P.binstep)) return false;
+    if ((dlo > P.dbulk) != (dhi > P.dbulk)) return false;
+    within = true;
+    d = dlo;
+    return true;
+}
+
 // FP32 running minimum with second best
 struct Best {
     float best, second;
@@ -606,20 +626,32 @@ __device__ Rec resolve_mols(const KP& P, const Geo& g, const TU* xuf, int u, int
     int64_t ibase = (int64_t)u * P.nu, jbase = (int64_t)v * P.nv;
     if (bm.best <= g.cut2_hi) {
         if (bm.second - bm.best > g.amb) {
-            double x0, x1, x2, y0, y1, y2;
-            load3(xuf, ibase + bm.bi, x0, x1, x2);
-            ysrc(bm.ba, y0, y1, y2);
-            double d2 = cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
-            if (d2 <= P.cut2) { r.within = 1; r.i = bm.bi; r.j = jbase + bm.ba; r.d = sqrt(d2); }
+            bool w;
+            double dd;
+            if (decide_fp32(P, g.guard, bm.best, w, dd)) {
+                if (w) { r.within = 1; r.i = bm.bi; r.j = jbase + bm.ba; r.d = dd; }
+            } else {
+                double x0, x1, x2, y0, y1, y2;
+                load3(xuf, ibase + bm.bi, x0, x1, x2);
+                ysrc(bm.ba, y0, y1, y2);
+                double d2 = cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
+                if (d2 <= P.cut2) { r.within = 1; r.i = bm.bi; r.j = jbase + bm.ba; r.d = sqrt(d2); }
+            }
         } else exhaustive = true;
     }
     if (br.best <= g.cut2_hi) {
         if (br.second - br.best > g.amb) {
-            double x0, x1, x2, y0, y1, y2;
-            load3(xuf, ibase + br.bi, x0, x1, x2);
-            ysrc(P.iref, y0, y1, y2);
-            double d2 = cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
-            if (d2 <= P.cut2) { r.ref = 1; r.dref = sqrt(d2); }
+            bool w;
+            double dd;
+            if (decide_fp32(P, g.guard, br.best, w, dd)) {
+                if (w) { r.ref = 1; r.dref = dd; }
+            } else {
+                double x0, x1, x2, y0, y1, y2;
+                load3(xuf, ibase + br.bi, x0, x1, x2);
+                ysrc(P.iref, y0, y1, y2);
+                double d2 = cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
+                if (d2 <= P.cut2) { r.ref = 1; r.dref = sqrt(d2); }
+            }
         } else exhaustive = true;
     }
     if (exhaustive) {
@@ -1406,20 +1438,32 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
         if (leader) {
             if (mb <= g.cut2_hi) {
                 if (ms - mb > g.amb) {
-                    double x0, x1, x2, y0, y1, y2;
-                    load3(xuf, mi, x0, x1, x2);
-                    ysrc(ma, y0, y1, y2);
-                    double d2 = cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
-                    if (d2 <= P.cut2) { r.within = 1; r.i = mi; r.j = jbase + ma; r.d = sqrt(d2); }
+                    bool w;
+                    double dd;
+                    if (decide_fp32(P, g.guard, mb, w, dd)) {
+                        if (w) { r.within = 1; r.i = mi; r.j = jbase + ma; r.d = dd; }
+                    } else {
+                        double x0, x1, x2, y0, y1, y2;
+                        load3(xuf, mi, x0, x1, x2);
+                        ysrc(ma, y0, y1, y2);
+                        double d2 = cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
+                        if (d2 <= P.cut2) { r.within = 1; r.i = mi; r.j = jbase + ma; r.d = sqrt(d2); }
+                    }
                 } else exh = true;
             }
             if (rb <= g.cut2_hi) {
                 if (rsec - rb > g.amb) {
-                    double x0, x1, x2, y0, y1, y2;
-                    load3(xuf, ri, x0, x1, x2);
-                    ysrc(P.iref, y0, y1, y2);
-                    double d2 = cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
-                    if (d2 <= P.cut2) { r.ref = 1; r.dref = sqrt(d2); }
+                    bool w;
+                    double dd;
+                    if (decide_fp32(P, g.guard, rb, w, dd)) {
+                        if (w) { r.ref = 1; r.dref = dd; }
+                    } else {
+                        double x0, x1, x2, y0, y1, y2;
+                        load3(xuf, ri, x0, x1, x2);
+                        ysrc(P.iref, y0, y1, y2);
+                        double d2 = cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
+                        if (d2 <= P.cut2) { r.ref = 1; r.dref = sqrt(d2); }
+                    }
                 } else exh = true;
             }
         }

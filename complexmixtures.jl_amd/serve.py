@@ -176,11 +176,12 @@ def run_plain(pool, reqs, keep_raw=False):
     """The same requests one at a time (each starts when it has arrived and the
     previous one is done)."""
     t0 = time.perf_counter()
-    lat, digests, raws, fin = [], [], [], []
+    lat, digests, raws, fin, service = [], [], [], [], []
     for r in reqs:
         dt = t0 + r.arrival - time.perf_counter()
         if dt > 0:
             time.sleep(dt)
+        t_start = time.perf_counter()
         s = pool[r.system]
         kw, du, dv, tuning = job_kwargs(s, r)
         job = _lib.Job(tuning=tuning or None, **kw)
@@ -188,6 +189,7 @@ def run_plain(pool, reqs, keep_raw=False):
         raw = job.finish()
         job.close()
         t = time.perf_counter() - t0
+        service.append(time.perf_counter() - t_start)
         fin.append(t)
         lat.append(t - r.arrival)
         digests.append(digest(raw))
@@ -195,8 +197,23 @@ def run_plain(pool, reqs, keep_raw=False):
             raws.append(raw)
     out = _summary(lat, min(r.arrival for r in reqs), max(fin), len(reqs))
     out["digests"] = digests
+    out["service_s"] = service
     if keep_raw:
         out["raw"] = raws
+    return out
+
+
+def service_breakdown(reqs, service):
+    """Plain-path service time per request class: self-MDDF, cross with
+    ideal-gas passes, cross coordination numbers."""
+    out = {}
+    for name, sel in (("self", lambda r: r.self_), ("cross_mddf", lambda r: not r.self_ and r.mddf),
+                      ("cross_cn", lambda r: not r.self_ and not r.mddf)):
+        idx = [i for i, r in enumerate(reqs) if sel(r)]
+        if idx:
+            tot = sum(service[i] for i in idx)
+            out[name] = {"requests": len(idx), "seconds": round(tot, 3),
+                         "ms_per_frame": round(1e3 * tot / sum(reqs[i].nframes for i in idx), 3)}
     return out
 
 

@@ -335,3 +335,64 @@ def test_dense_solute_staging_rounds(pkg, oracle):
                    dict(cell_fraction=0.5, columns=8)):
         b = gpu_raw(pkg, traj, opts, job_kw={"tuning": tuning})
         assert_raw_equal(a, b)
+
+
+def _edge_distances():
+    # bin edges (multiples of binstep), dbulk, the cutoff, and values one ulp-ish around them
+    base = [2.0, 2.02, 3.0, 4.46, 6.0, 8.0, 9.98, 10.0]
+    out = []
+    for d in base:
+        out += [d, np.nextafter(d, 0.0), np.nextafter(d, 20.0), d + 1e-9, d - 1e-9]
+    return out
+
+
+def test_bin_edges_cutoff_and_dbulk(pkg, oracle):
+    """Minimum distances exactly on and next to bin edges, dbulk and the
+    cutoff (single-atom solute, waters along the axes, FP64 input): the bins,
+    cutoff and bulk decisions taken from FP32 bounds or FP64 recomputation
+    match the oracle."""
+    c = np.array([15.0, 15.0, 15.0])
+    axes = [np.array(v, float) for v in ([1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1])]
+    ds = _edge_distances()
+    nframes = (len(ds) + 5) // 6
+    xv = np.zeros((nframes, 6 * 3, 3))
+    for f in range(nframes):
+        for k, e in enumerate(axes):
+            d = ds[(f * 6 + k) % len(ds)]
+            p = np.roll(e, 1)
+            o = c + d * e
+            xv[f, 3 * k] = o
+            xv[f, 3 * k + 1] = c + (d + 0.8) * e + 0.6 * p
+            xv[f, 3 * k + 2] = c + (d + 0.8) * e - 0.6 * p
+    xu = np.repeat(c[None, None], nframes, axis=0)
+    su = pkg.AtomSelection([1], nmols=1)
+    sv = pkg.AtomSelection(list(range(2, 20)), natomspermol=3)
+    cells = np.repeat(np.diag([40.0] * 3)[None], nframes, axis=0)
+    traj = pkg.MemoryTrajectory(xu, xv, cells, su, sv)
+    for opts in (pkg.Options(dbulk=8.0, cutoff=10.0, usecutoff=True, n_random_samples=3, seed=3, silent=True),
+                 pkg.Options(dbulk=10.0, n_random_samples=2, seed=4, silent=True)):
+        a, _ = oracle_raw(pkg, oracle, traj, opts)
+        b = gpu_raw(pkg, traj, opts)
+        assert_raw_equal(a, b)
+
+
+def test_bin_edges_self(pkg, oracle):
+    """The same edge distances between single-atom molecules of a self-MDDF
+    (many-solute-molecule kernels): per frame a centre atom and six atoms at
+    the listed distances along the axes."""
+    ds = _edge_distances()
+    c = np.array([20.0, 20.0, 20.0])
+    axes = [np.array(v, float) for v in ([1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1])]
+    nframes = (len(ds) + 5) // 6
+    xv = np.zeros((nframes, 7, 3))
+    for f in range(nframes):
+        xv[f, 0] = c
+        for k, e in enumerate(axes):
+            xv[f, 1 + k] = c + ds[(f * 6 + k) % len(ds)] * e
+    sel = pkg.AtomSelection(list(range(1, 8)), natomspermol=1)
+    cells = np.repeat(np.diag([40.0] * 3)[None], nframes, axis=0)
+    traj = pkg.MemoryTrajectory(xv, xv, cells, sel, sel)
+    opts = pkg.Options(dbulk=8.0, cutoff=10.0, usecutoff=True, n_random_samples=2, seed=5, silent=True)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts)
+    assert_raw_equal(a, b)
