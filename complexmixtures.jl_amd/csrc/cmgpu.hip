@@ -679,12 +679,15 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
     int64_t total = (int64_t)P.F * P.Mv;
     for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
         int f = (int)(it / P.Mv);
-        int v = (int)(it - (int64_t)f * P.Mv);
+        const int q0 = (int)(it - (int64_t)f * P.Mv);
+        const float4* srt = B.sorted + (int64_t)f * B.nitems;
+        // self: solvent molecules in cell order (the solute list), so that the
+        // lanes of a wavefront share neighbour cells
+        const int v = P.self_ ? __float_as_int(srt[q0].w) : q0;
         const Geo& g = B.geo[f];
         const float4* yw = B.vw + (int64_t)f * P.Nv + (int64_t)v * P.nv;
         float4 cv = B.vcen[(int64_t)f * P.Mv + v];
         int vcell = B.vcell[(int64_t)f * P.Mv + v];
-        const float4* srt = B.sorted + (int64_t)f * B.nitems;
         const int* st = B.start + (int64_t)f * (P.cap + 1);
         const float4* solw = B.solw + (int64_t)f * P.Nu;
         const TU* xuf = xu + (int64_t)f * P.Nu * 3;
@@ -693,6 +696,45 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
         auto ysrc = [&](int a, double& y0, double& y1, double& y2) { load3(xvf, jbase + a, y0, y1, y2); };
         int nc[3] = {g.nc[0], g.nc[1], g.nc[2]};
         int cc[3] = {vcell % nc[0], (vcell / nc[0]) % nc[1], vcell / (nc[0] * nc[1])};
+        auto record = [&](int u, const Best& bm, const Best& br) {
+            Rec r = resolve_mols(P, g, xuf, u, v, ysrc, bm, br, &B.stats[0]);
+            hist_record(P, sh, B.cnt_gu, B.cnt_gv, r);
+            if (P.K > 0 && r.within) {
+                const int* dp = B.drawn_ptr + (int64_t)f * (P.Mu + 1);
+                const int* di = B.drawn_idx + (int64_t)f * P.K;
+                for (int q = dp[u]; q < dp[u + 1]; q++) {
+                    int k = di[q];
+                    B.bflag[((int64_t)f * P.kb + k) * P.Mv + v] = inbulk(P, r) ? 1 : 0;
+                }
+            }
+        };
+        if (nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3) {
+            // 27 distinct neighbour cells, one lattice shift each
+            for (int oz = -1; oz <= 1; oz++)
+            for (int oy = -1; oy <= 1; oy++)
+            for (int ox = -1; ox <= 1; ox++) {
+                int mx, my, mz;
+                const int cx = nb_cell(cc[0], ox, nc[0], mx), cy = nb_cell(cc[1], oy, nc[1], my),
+                          cz = nb_cell(cc[2], oz, nc[2], mz);
+                const float fx = (float)mx, fy = (float)my, fz = (float)mz;
+                const float tx = g.H32[0] * fx + g.H32[3] * fy + g.H32[6] * fz;
+                const float ty = g.H32[1] * fx + g.H32[4] * fy + g.H32[7] * fz;
+                const float tz = g.H32[2] * fx + g.H32[5] * fy + g.H32[8] * fz;
+                const int cell = (cz * nc[1] + cy) * nc[0] + cx;
+                for (int p = st[cell]; p < st[cell + 1]; p++) {
+                    const float4 cu = srt[p];
+                    const int u = __float_as_int(cu.w);
+                    if (P.self_ && u == v) continue;
+                    const float dx = cv.x - (cu.x + tx), dy = cv.y - (cu.y + ty), dz = cv.z - (cu.z + tz);
+                    if (dx * dx + dy * dy + dz * dz > g.rpre2) continue;
+                    Best bm, br;
+                    bm.init(); br.init();
+                    pairs_mol(P, solw + (int64_t)u * P.nu, yw, tx, ty, tz, bm, br);
+                    record(u, bm, br);
+                }
+            }
+            continue;
+        }
         // distinct neighbour cells per dimension with all the shifts that map onto them
         int dcell[3][3], dns[3], dsh[3][3][3];
         for (int k = 0; k < 3; k++) {
@@ -738,16 +780,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
                     }
                 }
                 if (!any) continue;
-                Rec r = resolve_mols(P, g, xuf, u, v, ysrc, bm, br, &B.stats[0]);
-                hist_record(P, sh, B.cnt_gu, B.cnt_gv, r);
-                if (P.K > 0 && r.within) {
-                    const int* dp = B.drawn_ptr + (int64_t)f * (P.Mu + 1);
-                    const int* di = B.drawn_idx + (int64_t)f * P.K;
-                    for (int q = dp[u]; q < dp[u + 1]; q++) {
-                        int k = di[q];
-                        B.bflag[((int64_t)f * P.kb + k) * P.Mv + v] = inbulk(P, r) ? 1 : 0;
-                    }
-                }
+                record(u, bm, br);
             }
         }
     }
