@@ -1370,18 +1370,52 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
         // atoms of those cells are ranked (wave scan of the counts), each lane
         // loads the atoms of its ranks, and the ones within rs of the AABB are
         // appended at ballot/popcount offsets.
-        for (int cb = 0; cb < ncl; cb += 64) {
-            const int ci = cb + lane;
+        // Staging walks rows of cells (x fastest in memory): a slot is the part
+        // of one (y, z) row of the index box that lies in one periodic image
+        // along x, so its atoms are one contiguous range of the sorted list.
+        // With a diagonal cell, rows farther than rs from the AABB are
+        // dropped and each row is trimmed to the x range within rs (plus a
+        // slack over FP32 rounding); the per-atom AABB test below is exact.
+        int q0;
+        wrap_index(ilo[0], g.nc[0], q0);
+        int q1;
+        wrap_index(ihi[0], g.nc[0], q1);
+        const int npc = q1 - q0 + 1;
+        const int nslots = ny * nz * npc;
+        const float rsl = rs + 1e-3f, rsl2 = rsl * rsl;
+        const float hcx = g.H32[0] / g.nc[0], hcy = g.H32[4] / g.nc[1], hcz = g.H32[8] / g.nc[2];
+        for (int sb = 0; sb < nslots; sb += 64) {
+            const int si = sb + lane;
             int cstart = 0, cnt = 0;
             float tx = 0.f, ty = 0.f, tz = 0.f;
-            if (ci < ncl) {
-                int ix = ilo[0] + ci % nx, iy = ilo[1] + (ci / nx) % ny, iz = ilo[2] + ci / (nx * ny);
-                int mx, my, mz;
-                int cx = wrap_index(ix, g.nc[0], mx), cy = wrap_index(iy, g.nc[1], my), cz = wrap_index(iz, g.nc[2], mz);
-                int cell = (cz * g.nc[1] + cy) * g.nc[0] + cx;
-                cstart = st[cell];
-                cnt = st[cell + 1] - cstart;
-                shift_of(g, mx, my, mz, tx, ty, tz);
+            if (si < nslots) {
+                const int r = si / npc, pc = si - r * npc;
+                const int iy = ilo[1] + r % ny, iz = ilo[2] + r / ny;
+                int x0 = ilo[0], x1 = ihi[0];
+                bool keep = true;
+                if (g.ortho) {
+                    const float dy = fmaxf(fmaxf(iy * hcy - hy, ly - (iy + 1) * hcy), 0.f);
+                    const float dz = fmaxf(fmaxf(iz * hcz - hz, lz - (iz + 1) * hcz), 0.f);
+                    const float r2 = rsl2 - dy * dy - dz * dz;
+                    if (r2 < 0.f) {
+                        keep = false;
+                    } else {
+                        const float rx = sqrtf(r2) + 1e-3f;
+                        x0 = max(x0, (int)floorf((lx - rx) / hcx));
+                        x1 = min(x1, (int)floorf((hx + rx) / hcx));
+                    }
+                }
+                const int mx = q0 + pc;
+                x0 = max(x0, mx * g.nc[0]);
+                x1 = min(x1, mx * g.nc[0] + g.nc[0] - 1);
+                if (keep && x0 <= x1) {
+                    int my, mz;
+                    const int cy = wrap_index(iy, g.nc[1], my), cz = wrap_index(iz, g.nc[2], mz);
+                    const int row = (cz * g.nc[1] + cy) * g.nc[0] - mx * g.nc[0];
+                    cstart = st[row + x0];
+                    cnt = st[row + x1 + 1] - cstart;
+                    shift_of(g, mx, my, mz, tx, ty, tz);
+                }
             }
             const int incl = wave_incl_scan(cnt);
             const int excl = incl - cnt;
