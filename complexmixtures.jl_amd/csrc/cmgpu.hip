@@ -1569,9 +1569,13 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
                         }
                     }
                 } else {
-                    // staged buffer recycled: scan the cells around the atom
+                    // staged buffer recycled: scan the cells around the atom out
+                    // to the band (an FP32 d2 within the band means an FP64 d2
+                    // within band + guard), every pair there in FP64
                     int jlo[3], jhi[3];
-                    cell_range(g, yx, yy, yz, yx, yy, yz, rs, jlo, jhi);
+                    const float band = isref ? fmaxf(bandm, bandr) : bandm;
+                    const float rband = fminf(sqrtf(band + (float)g.guard) + 1e-3f, rs);
+                    cell_range(g, yx, yy, yz, yx, yy, yz, rband, jlo, jhi);
                     const int mxn = jhi[0] - jlo[0] + 1, myn = jhi[1] - jlo[1] + 1, mzn = jhi[2] - jlo[2] + 1;
                     const int ncq = mxn * myn * mzn;
                     for (int ci = lane; ci < ncq; ci += 64) {
