@@ -45,6 +45,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
@@ -594,6 +595,62 @@ __device__ __forceinline__ int nb_cell(int c, int o, int n, int& m)
     return t - m * n;
 }
 
+// Wavefront scans and reductions with DPP row shifts and row broadcasts
+// (row_shr:1,2,4,8 within rows of 16 lanes, then row_bcast:15 into rows 1
+// and 3 and row_bcast:31 into rows 2 and 3); lanes without a source read the
+// identity.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i(int identity, int x)
+{
+    return __builtin_amdgcn_update_dpp(identity, x, CTRL, ROWS, 0xf, false);
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    v += dpp_i<0x111, 0xf>(0, v);
+    v += dpp_i<0x112, 0xf>(0, v);
+    v += dpp_i<0x114, 0xf>(0, v);
+    v += dpp_i<0x118, 0xf>(0, v);
+    v += dpp_i<0x142, 0xa>(0, v);
+    v += dpp_i<0x143, 0xc>(0, v);
+    return v;
+}
+
+__device__ __forceinline__ float fmin_sel(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float fmax_sel(float a, float b) { return b > a ? b : a; }
+
+// minimum / maximum over the wavefront, the same in every lane
+__device__ __forceinline__ float wave_min(float x)
+{
+    const int id = __float_as_int(INFINITY);
+    x = fmin_sel(x, __int_as_float(dpp_i<0x111, 0xf>(id, __float_as_int(x))));
+    x = fmin_sel(x, __int_as_float(dpp_i<0x112, 0xf>(id, __float_as_int(x))));
+    x = fmin_sel(x, __int_as_float(dpp_i<0x114, 0xf>(id, __float_as_int(x))));
+    x = fmin_sel(x, __int_as_float(dpp_i<0x118, 0xf>(id, __float_as_int(x))));
+    x = fmin_sel(x, __int_as_float(dpp_i<0x142, 0xa>(id, __float_as_int(x))));
+    x = fmin_sel(x, __int_as_float(dpp_i<0x143, 0xc>(id, __float_as_int(x))));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+}
+__device__ __forceinline__ float wave_max(float x)
+{
+    const int id = __float_as_int(-INFINITY);
+    x = fmax_sel(x, __int_as_float(dpp_i<0x111, 0xf>(id, __float_as_int(x))));
+    x = fmax_sel(x, __int_as_float(dpp_i<0x112, 0xf>(id, __float_as_int(x))));
+    x = fmax_sel(x, __int_as_float(dpp_i<0x114, 0xf>(id, __float_as_int(x))));
+    x = fmax_sel(x, __int_as_float(dpp_i<0x118, 0xf>(id, __float_as_int(x))));
+    x = fmax_sel(x, __int_as_float(dpp_i<0x142, 0xa>(id, __float_as_int(x))));
+    x = fmax_sel(x, __int_as_float(dpp_i<0x143, 0xc>(id, __float_as_int(x))));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+}
+
+__device__ __forceinline__ void shift_of(const Geo& g, int mx, int my, int mz, float& tx, float& ty, float& tz)
+{
+    float fx = (float)mx, fy = (float)my, fz = (float)mz;
+    tx = g.H32[0] * fx + g.H32[3] * fy + g.H32[6] * fz;
+    ty = g.H32[1] * fx + g.H32[4] * fy + g.H32[7] * fz;
+    tz = g.H32[2] * fx + g.H32[5] * fy + g.H32[8] * fz;
+}
+
 // ---------------------------------------------------------------------------
 // minimum distances, mols family (many solute molecules)
 // ---------------------------------------------------------------------------
@@ -671,70 +728,142 @@ __device__ Rec resolve_mols(const KP& P, const Geo& g, const TU* xuf, int u, int
     return r;
 }
 
+// LDS pair queue of the molecule-pair kernel: entries per wavefront
+constexpr int QCAP = 128;
+
+__device__ __forceinline__ int wave_max_i(int x)
+{
+    x = max(x, dpp_i<0x111, 0xf>(INT_MIN, x));
+    x = max(x, dpp_i<0x112, 0xf>(INT_MIN, x));
+    x = max(x, dpp_i<0x114, 0xf>(INT_MIN, x));
+    x = max(x, dpp_i<0x118, 0xf>(INT_MIN, x));
+    x = max(x, dpp_i<0x142, 0xa>(INT_MIN, x));
+    x = max(x, dpp_i<0x143, 0xc>(INT_MIN, x));
+    return __builtin_amdgcn_readlane(x, 63);
+}
+
+// Real pass of the many-solute-molecule family.  blockIdx.y is the frame;
+// each lane owns one solvent molecule v (self: in cell order).  When every
+// dimension has at least three molecule cells, the 27 neighbour cells are
+// distinct and each close pair has one lattice shift: the lanes walk the cells
+// together (trip count = the wavefront's largest cell), the molecule pairs
+// that pass the centre test are appended to an LDS queue (ballot offsets),
+// and the queue is evaluated 64 pairs at a time, one pair per lane, so the
+// nu x nv atom loops run without divergence.  Self-MDDF evaluates each
+// unordered pair once (u < v) and emits the records of both orientations
+// (same distance and argmin pair; reference atoms and tie order per
+// orientation).  Smaller grids fall back to a per-lane walk with all shifts of
+// a cell grouped per molecule pair.
 template <class TU, class TV>
 __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU* xu, const TV* xv)
 {
     extern __shared__ unsigned sh[];
     hist_clear(sh, 2 * P.nbins);
-    int64_t total = (int64_t)P.F * P.Mv;
-    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
-        int f = (int)(it / P.Mv);
-        const int q0 = (int)(it - (int64_t)f * P.Mv);
-        const float4* srt = B.sorted + (int64_t)f * B.nitems;
-        // self: solvent molecules in cell order (the solute list), so that the
-        // lanes of a wavefront share neighbour cells
-        const int v = P.self_ ? __float_as_int(srt[q0].w) : q0;
-        const Geo& g = B.geo[f];
-        const float4* yw = B.vw + (int64_t)f * P.Nv + (int64_t)v * P.nv;
-        float4 cv = B.vcen[(int64_t)f * P.Mv + v];
-        int vcell = B.vcell[(int64_t)f * P.Mv + v];
-        const int* st = B.start + (int64_t)f * (P.cap + 1);
-        const float4* solw = B.solw + (int64_t)f * P.Nu;
-        const TU* xuf = xu + (int64_t)f * P.Nu * 3;
-        const TV* xvf = xv + (int64_t)f * P.Nv * 3;
-        int64_t jbase = (int64_t)v * P.nv;
-        auto ysrc = [&](int a, double& y0, double& y1, double& y2) { load3(xvf, jbase + a, y0, y1, y2); };
-        int nc[3] = {g.nc[0], g.nc[1], g.nc[2]};
-        int cc[3] = {vcell % nc[0], (vcell / nc[0]) % nc[1], vcell / (nc[0] * nc[1])};
-        auto record = [&](int u, const Best& bm, const Best& br) {
-            Rec r = resolve_mols(P, g, xuf, u, v, ysrc, bm, br, &B.stats[0]);
-            hist_record(P, sh, B.cnt_gu, B.cnt_gv, r);
-            if (P.K > 0 && r.within) {
-                const int* dp = B.drawn_ptr + (int64_t)f * (P.Mu + 1);
-                const int* di = B.drawn_idx + (int64_t)f * P.K;
-                for (int q = dp[u]; q < dp[u + 1]; q++) {
-                    int k = di[q];
-                    B.bflag[((int64_t)f * P.kb + k) * P.Mv + v] = inbulk(P, r) ? 1 : 0;
-                }
+    int4* qall = (int4*)(sh + ((2 * P.nbins + 3) & ~3));
+    int4* q = qall + (threadIdx.x >> 6) * QCAP;
+    const int lane = threadIdx.x & 63;
+    const int f = blockIdx.y;
+    const Geo& g = B.geo[f];
+    const float4* srt = B.sorted + (int64_t)f * B.nitems;
+    const int* st = B.start + (int64_t)f * (P.cap + 1);
+    const float4* solw = B.solw + (int64_t)f * P.Nu;
+    const float4* vwf = B.vw + (int64_t)f * P.Nv;
+    const TU* xuf = xu + (int64_t)f * P.Nu * 3;
+    const TV* xvf = xv + (int64_t)f * P.Nv * 3;
+    const int nc[3] = {g.nc[0], g.nc[1], g.nc[2]};
+    const bool simple = nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3;
+    const int* dp = B.drawn_ptr + (int64_t)f * (P.Mu + 1);
+    const int* di = B.drawn_idx + (int64_t)f * P.K;
+    // record of solute molecule uu and solvent molecule vv from their FP32 bests
+    auto emit = [&](int uu, int vv, const Best& bm, const Best& br) {
+        const int64_t jb = (int64_t)vv * P.nv;
+        auto ysrc = [&](int a, double& y0, double& y1, double& y2) { load3(xvf, jb + a, y0, y1, y2); };
+        Rec r = resolve_mols(P, g, xuf, uu, vv, ysrc, bm, br, &B.stats[0]);
+        hist_record(P, sh, B.cnt_gu, B.cnt_gv, r);
+        if (P.K > 0 && r.within)
+            for (int qq = dp[uu]; qq < dp[uu + 1]; qq++)
+                B.bflag[((int64_t)f * P.kb + di[qq]) * P.Mv + vv] = inbulk(P, r) ? 1 : 0;
+    };
+    // one queued pair: solute u shifted by lattice vector code, solvent v
+    auto eval_pair = [&](int4 e) {
+        const int u = e.x, v = e.y, code = e.z;
+        float tx, ty, tz;
+        shift_of(g, code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1, tx, ty, tz);
+        const float4* xs = solw + (int64_t)u * P.nu;
+        const float4* yw = vwf + (int64_t)v * P.nv;
+        Best bm, brv, bru;
+        bm.init(); brv.init(); bru.init();
+        for (int a = 0; a < P.nv; a++) {
+            const float4 y = yw[a];
+            const float yx = y.x - tx, yy = y.y - ty, yz = y.z - tz;
+            for (int i = 0; i < P.nu; i++) {
+                const float4 s4 = xs[i];
+                const float dx = yx - s4.x, dy = yy - s4.y, dz = yz - s4.z;
+                const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                bm.offer(d2, i, a);
+                if (a == P.iref) brv.offer(d2, i, a);
+                if (P.self_ && i == P.iref) bru.offer(d2, a, i);
             }
-        };
-        if (nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3) {
-            // 27 distinct neighbour cells, one lattice shift each
-            for (int oz = -1; oz <= 1; oz++)
-            for (int oy = -1; oy <= 1; oy++)
-            for (int ox = -1; ox <= 1; ox++) {
+        }
+        emit(u, v, bm, brv);
+        if (P.self_) {
+            Best bs = bm;
+            bs.bi = bm.ba;
+            bs.ba = bm.bi;
+            emit(v, u, bs, bru);
+        }
+    };
+    const int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < P.Mv; base += stride) {
+        const int q0 = base + lane;
+        const bool act = q0 < P.Mv;
+        const int v = act ? (P.self_ ? __float_as_int(srt[q0].w) : q0) : 0;
+        const float4 cv = B.vcen[(int64_t)f * P.Mv + v];
+        const int vcell = B.vcell[(int64_t)f * P.Mv + v];
+        const int cc[3] = {vcell % nc[0], (vcell / nc[0]) % nc[1], vcell / (nc[0] * nc[1])};
+        if (simple) {
+            int qn = 0;
+            const unsigned long long lt = (1ull << lane) - 1ull;
+            for (int o = 0; o < 27; o++) {
+                const int ox = o % 3 - 1, oy = (o / 3) % 3 - 1, oz = o / 9 - 1;
                 int mx, my, mz;
                 const int cx = nb_cell(cc[0], ox, nc[0], mx), cy = nb_cell(cc[1], oy, nc[1], my),
                           cz = nb_cell(cc[2], oz, nc[2], mz);
-                const float fx = (float)mx, fy = (float)my, fz = (float)mz;
-                const float tx = g.H32[0] * fx + g.H32[3] * fy + g.H32[6] * fz;
-                const float ty = g.H32[1] * fx + g.H32[4] * fy + g.H32[7] * fz;
-                const float tz = g.H32[2] * fx + g.H32[5] * fy + g.H32[8] * fz;
+                float tx, ty, tz;
+                shift_of(g, mx, my, mz, tx, ty, tz);
+                const int code = (mx + 1) + 3 * (my + 1) + 9 * (mz + 1);
                 const int cell = (cz * nc[1] + cy) * nc[0] + cx;
-                for (int p = st[cell]; p < st[cell + 1]; p++) {
-                    const float4 cu = srt[p];
-                    const int u = __float_as_int(cu.w);
-                    if (P.self_ && u == v) continue;
-                    const float dx = cv.x - (cu.x + tx), dy = cv.y - (cu.y + ty), dz = cv.z - (cu.z + tz);
-                    if (dx * dx + dy * dy + dz * dz > g.rpre2) continue;
-                    Best bm, br;
-                    bm.init(); br.init();
-                    pairs_mol(P, solw + (int64_t)u * P.nu, yw, tx, ty, tz, bm, br);
-                    record(u, bm, br);
+                const int cs = st[cell];
+                const int cnt = act ? st[cell + 1] - cs : 0;
+                const int kmax = wave_max_i(cnt);
+                for (int k = 0; k < kmax; k++) {
+                    bool pass = false;
+                    int u = 0;
+                    if (k < cnt) {
+                        const float4 cu = srt[cs + k];
+                        u = __float_as_int(cu.w);
+                        const float dx = cv.x - (cu.x + tx), dy = cv.y - (cu.y + ty), dz = cv.z - (cu.z + tz);
+                        pass = (P.self_ ? u < v : true) && dx * dx + dy * dy + dz * dz <= g.rpre2;
+                    }
+                    const unsigned long long m = __ballot(pass);
+                    if (pass) q[qn + __popcll(m & lt)] = make_int4(u, v, code, 0);
+                    qn += __popcll(m);
+                    if (qn >= 64) {
+                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                        const int4 e = q[qn - 64 + lane];
+                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                        qn -= 64;
+                        eval_pair(e);
+                    }
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (lane < qn) eval_pair(q[lane]);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             continue;
         }
+        if (!act) continue;
+        const float4* yw = vwf + (int64_t)v * P.nv;
         // distinct neighbour cells per dimension with all the shifts that map onto them
         int dcell[3][3], dns[3], dsh[3][3][3];
         for (int k = 0; k < 3; k++) {
@@ -743,10 +872,10 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
             for (int o = -1; o <= 1; o++) {
                 int m;
                 int c = nb_cell(cc[k], o, nc[k], m);
-                int q = -1;
-                for (int e = 0; e < dns[k]; e++) if (dcell[k][e] == c) q = e;
-                if (q < 0) { q = dns[k]++; dcell[k][q] = c; }
-                dsh[k][q][cnts[q]++] = m;
+                int qq = -1;
+                for (int e = 0; e < dns[k]; e++) if (dcell[k][e] == c) qq = e;
+                if (qq < 0) { qq = dns[k]++; dcell[k][qq] = c; }
+                dsh[k][qq][cnts[qq]++] = m;
             }
             for (int e = 0; e < dns[k]; e++) dsh[k][e][2] = cnts[e] > 2 ? dsh[k][e][2] : 99;
             for (int e = 0; e < dns[k]; e++) if (cnts[e] < 2) dsh[k][e][1] = 99;
@@ -768,10 +897,8 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
                         int my = dsh[1][ey][sy]; if (my == 99) break;
                         for (int sx = 0; sx < 3; sx++) {
                             int mx = dsh[0][ex][sx]; if (mx == 99) break;
-                            float fx = (float)mx, fy = (float)my, fz = (float)mz;
-                            float tx = g.H32[0] * fx + g.H32[3] * fy + g.H32[6] * fz;
-                            float ty = g.H32[1] * fx + g.H32[4] * fy + g.H32[7] * fz;
-                            float tz = g.H32[2] * fx + g.H32[5] * fy + g.H32[8] * fz;
+                            float tx, ty, tz;
+                            shift_of(g, mx, my, mz, tx, ty, tz);
                             float dx = cv.x - (cu.x + tx), dy = cv.y - (cu.y + ty), dz = cv.z - (cu.z + tz);
                             if (dx * dx + dy * dy + dz * dz > g.rpre2) continue;
                             any = true;
@@ -780,7 +907,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
                     }
                 }
                 if (!any) continue;
-                record(u, bm, br);
+                emit(u, v, bm, br);
             }
         }
     }
@@ -972,13 +1099,6 @@ __device__ __forceinline__ void cell_range(const Geo& g, float lx, float ly, flo
     }
 }
 
-__device__ __forceinline__ void shift_of(const Geo& g, int mx, int my, int mz, float& tx, float& ty, float& tz)
-{
-    float fx = (float)mx, fy = (float)my, fz = (float)mz;
-    tx = g.H32[0] * fx + g.H32[3] * fy + g.H32[6] * fz;
-    ty = g.H32[1] * fx + g.H32[4] * fy + g.H32[7] * fz;
-    tz = g.H32[2] * fx + g.H32[5] * fy + g.H32[8] * fz;
-}
 
 // cells near the solute: every cell within rnear of a solute atom (index box)
 __global__ void k_near_mark(KP P, Bufs B)
@@ -1212,54 +1332,6 @@ __global__ void k_place_near(KP P, Bufs B, const TV* xv)
         double ua[3] = {cm::u01(r2.x), cm::u01(r2.y), cm::u01(r2.z)};
         cm::random_move(src, dst, P.nv, P.iref, g.cell, uc, ua);
     }
-}
-
-// Wavefront scans and reductions with DPP row shifts and row broadcasts
-// (row_shr:1,2,4,8 within rows of 16 lanes, then row_bcast:15 into rows 1
-// and 3 and row_bcast:31 into rows 2 and 3); lanes without a source read the
-// identity.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ int dpp_i(int identity, int x)
-{
-    return __builtin_amdgcn_update_dpp(identity, x, CTRL, ROWS, 0xf, false);
-}
-
-__device__ __forceinline__ int wave_incl_scan(int v)
-{
-    v += dpp_i<0x111, 0xf>(0, v);
-    v += dpp_i<0x112, 0xf>(0, v);
-    v += dpp_i<0x114, 0xf>(0, v);
-    v += dpp_i<0x118, 0xf>(0, v);
-    v += dpp_i<0x142, 0xa>(0, v);
-    v += dpp_i<0x143, 0xc>(0, v);
-    return v;
-}
-
-__device__ __forceinline__ float fmin_sel(float a, float b) { return b < a ? b : a; }
-__device__ __forceinline__ float fmax_sel(float a, float b) { return b > a ? b : a; }
-
-// minimum / maximum over the wavefront, the same in every lane
-__device__ __forceinline__ float wave_min(float x)
-{
-    const int id = __float_as_int(INFINITY);
-    x = fmin_sel(x, __int_as_float(dpp_i<0x111, 0xf>(id, __float_as_int(x))));
-    x = fmin_sel(x, __int_as_float(dpp_i<0x112, 0xf>(id, __float_as_int(x))));
-    x = fmin_sel(x, __int_as_float(dpp_i<0x114, 0xf>(id, __float_as_int(x))));
-    x = fmin_sel(x, __int_as_float(dpp_i<0x118, 0xf>(id, __float_as_int(x))));
-    x = fmin_sel(x, __int_as_float(dpp_i<0x142, 0xa>(id, __float_as_int(x))));
-    x = fmin_sel(x, __int_as_float(dpp_i<0x143, 0xc>(id, __float_as_int(x))));
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
-}
-__device__ __forceinline__ float wave_max(float x)
-{
-    const int id = __float_as_int(-INFINITY);
-    x = fmax_sel(x, __int_as_float(dpp_i<0x111, 0xf>(id, __float_as_int(x))));
-    x = fmax_sel(x, __int_as_float(dpp_i<0x112, 0xf>(id, __float_as_int(x))));
-    x = fmax_sel(x, __int_as_float(dpp_i<0x114, 0xf>(id, __float_as_int(x))));
-    x = fmax_sel(x, __int_as_float(dpp_i<0x118, 0xf>(id, __float_as_int(x))));
-    x = fmax_sel(x, __int_as_float(dpp_i<0x142, 0xa>(id, __float_as_int(x))));
-    x = fmax_sel(x, __int_as_float(dpp_i<0x143, 0xc>(id, __float_as_int(x))));
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
 }
 
 // lexicographic minimum of two exact records, OR of the reference flags
@@ -1955,8 +2027,11 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         launch_tiles_c(J->tile_c, f32, false, s, J->eval_grid, J->cus, P, B, xu_dev, xv_dev);
     } else {
         int64_t n = (int64_t)F * P.Mv;
-        if (f32) hipLaunchKernelGGL((k_eval_mols_real<float, float>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, uf, vf);
-        else hipLaunchKernelGGL((k_eval_mols_real<double, double>), dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, ud, vd);
+        const size_t shq = sizeof(unsigned) * ((2 * P.nbins + 3) & ~3) + sizeof(int4) * QCAP * (BLOCK / 64);
+        const dim3 gr((unsigned)std::max<int64_t>(1, std::min<int64_t>((P.Mv + BLOCK - 1) / BLOCK, 4096 / F + 1)),
+                      (unsigned)F);
+        if (f32) hipLaunchKernelGGL((k_eval_mols_real<float, float>), gr, dim3(BLOCK), shq, s, P, B, uf, vf);
+        else hipLaunchKernelGGL((k_eval_mols_real<double, double>), gr, dim3(BLOCK), shq, s, P, B, ud, vd);
     }
     if (J->profiling) HIPCHK(hipEventRecord(ev[1], s));
     HIPCHK(hipGetLastError());
