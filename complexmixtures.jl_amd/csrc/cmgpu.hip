@@ -103,6 +103,8 @@ struct KP {                  // job parameters, by value
     int64_t Nu, Nv;
     int mpt;                 // molecules per chunk (tiles family)
     int tcap;                // staged solute atoms per round (tiles family)
+    int glds;                // molecule-pair kernels: group counts privatised in LDS
+    int gnu, gnv;            // group counts of the solute / solvent side
     int NP;                  // passes per frame, 1 + K (tiles family)
     int kcap;                // molecule sort keys per (frame, pass): Morton codes of the fine cells
 };
@@ -532,6 +534,52 @@ __device__ __forceinline__ void hist_record_global(const KP& P, ull* md, ull* rd
     }
 }
 
+// Group counts privatised in LDS (small group arrays): the same updates as
+// group_hit into a block-local copy, flushed once per block.
+__device__ __forceinline__ void group_hit_lds(unsigned* gc, int nbins, int b, int64_t atom, const int* ptr, const int* idx,
+                                              int natomspermol)
+{
+    if (!ptr) {
+        const int64_t t = atom % natomspermol;
+        atomicAdd(&gc[t * nbins + b], 1u);
+    } else {
+        for (int k = ptr[atom]; k < ptr[atom + 1]; k++) atomicAdd(&gc[(int64_t)idx[k] * nbins + b], 1u);
+    }
+}
+
+// md/rdf into the block histogram sh; groups into LDS (shg) when privatised,
+// else straight into the global counters
+__device__ __forceinline__ void hist_record_mols(const KP& P, unsigned* sh, unsigned* shg, ull* gu, ull* gv, const Rec& r)
+{
+    if (!shg) {
+        hist_record(P, sh, gu, gv, r);
+        return;
+    }
+    if (!r.within) return;
+    const int b = (int)cm::setbin(r.d, P.binstep) - 1;
+    if (b >= P.nbins) return;
+    atomicAdd(&sh[b], 1u);
+    unsigned* shu = shg;
+    unsigned* shv = shg + (int64_t)P.nbins * P.gnu;
+    group_hit_lds(shu, P.nbins, b, r.i, P.gu_ptr, P.gu_idx, P.nu);
+    if (P.self_) group_hit_lds(shu, P.nbins, b, r.j, P.gv_ptr, P.gv_idx, P.nu);
+    else group_hit_lds(shv, P.nbins, b, r.j, P.gv_ptr, P.gv_idx, P.nv);
+    if (r.ref) {
+        const int rb = (int)cm::setbin(r.dref, P.binstep) - 1;
+        if (rb < P.nbins) atomicAdd(&sh[P.nbins + rb], 1u);
+    }
+}
+
+__device__ __forceinline__ void groups_flush(const KP& P, const unsigned* shg, ull* gu, ull* gv)
+{
+    __syncthreads();
+    const int nu_ = P.gnu * P.nbins, nv_ = P.self_ ? 0 : P.gnv * P.nbins;
+    for (int k = threadIdx.x; k < nu_; k += blockDim.x)
+        if (shg[k]) atomicAdd(&gu[k], (ull)shg[k]);
+    for (int k = threadIdx.x; k < nv_; k += blockDim.x)
+        if (shg[nu_ + k]) atomicAdd(&gv[k], (ull)shg[nu_ + k]);
+}
+
 __device__ __forceinline__ void hist_clear(unsigned* sh, int n)
 {
     for (int k = threadIdx.x; k < n; k += blockDim.x) sh[k] = 0u;
@@ -730,6 +778,7 @@ __device__ Rec resolve_mols(const KP& P, const Geo& g, const TU* xuf, int u, int
 
 // LDS pair queue of the molecule-pair kernel: entries per wavefront
 constexpr int QCAP = 128;
+constexpr int GLDS_MAX = 32 * 1024;   // bytes of LDS group counts per block
 
 __device__ __forceinline__ int wave_max_i(int x)
 {
@@ -758,8 +807,10 @@ template <class TU, class TV>
 __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU* xu, const TV* xv)
 {
     extern __shared__ unsigned sh[];
-    hist_clear(sh, 2 * P.nbins);
-    int4* qall = (int4*)(sh + ((2 * P.nbins + 3) & ~3));
+    const int gwords = P.glds ? P.nbins * (P.gnu + (P.self_ ? 0 : P.gnv)) : 0;
+    hist_clear(sh, 2 * P.nbins + gwords);
+    unsigned* shg = P.glds ? sh + 2 * P.nbins : nullptr;
+    int4* qall = (int4*)(sh + ((2 * P.nbins + gwords + 3) & ~3));
     int4* q = qall + (threadIdx.x >> 6) * QCAP;
     const int lane = threadIdx.x & 63;
     const int f = blockIdx.y;
@@ -779,7 +830,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
         const int64_t jb = (int64_t)vv * P.nv;
         auto ysrc = [&](int a, double& y0, double& y1, double& y2) { load3(xvf, jb + a, y0, y1, y2); };
         Rec r = resolve_mols(P, g, xuf, uu, vv, ysrc, bm, br, &B.stats[0]);
-        hist_record(P, sh, B.cnt_gu, B.cnt_gv, r);
+        hist_record_mols(P, sh, shg, B.cnt_gu, B.cnt_gv, r);
         if (P.K > 0 && r.within)
             for (int qq = dp[uu]; qq < dp[uu + 1]; qq++)
                 B.bflag[((int64_t)f * P.kb + di[qq]) * P.Mv + vv] = inbulk(P, r) ? 1 : 0;
@@ -912,13 +963,16 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
         }
     }
     hist_flush(sh, P.nbins, B.cnt_md, B.cnt_rdf);
+    if (shg) groups_flush(P, shg, B.cnt_gu, B.cnt_gv);
 }
 
 template <class TU>
 __global__ void __launch_bounds__(BLOCK) k_eval_mols_rand(KP P, Bufs B, const TU* xu)
 {
     extern __shared__ unsigned sh[];
-    hist_clear(sh, 2 * P.nbins);
+    const int gwords = P.glds ? P.nbins * (P.gnu + (P.self_ ? 0 : P.gnv)) : 0;
+    hist_clear(sh, 2 * P.nbins + gwords);
+    unsigned* shg = P.glds ? sh + 2 * P.nbins : nullptr;
     int64_t total = (int64_t)P.F * P.K * P.Mv;
     for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
         int64_t fk = it / P.Mv;
@@ -953,9 +1007,10 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_rand(KP P, Bufs B, const TU
             const double* pp = B.P + (fk * P.Nv + jbase + a) * 3; y0 = pp[0]; y1 = pp[1]; y2 = pp[2];
         };
         Rec r = resolve_mols(P, g, xuf, u, v, ysrc, bm, br, &B.stats[0]);
-        hist_record(P, sh, B.cnt_gu_r, B.cnt_gv_r, r);
+        hist_record_mols(P, sh, shg, B.cnt_gu_r, B.cnt_gv_r, r);
     }
     hist_flush(sh, P.nbins, B.cnt_md_r, B.cnt_rdf_r);
+    if (shg) groups_flush(P, shg, B.cnt_gu_r, B.cnt_gv_r);
 }
 
 // ---------------------------------------------------------------------------
@@ -2027,7 +2082,8 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         launch_tiles_c(J->tile_c, f32, false, s, J->eval_grid, J->cus, P, B, xu_dev, xv_dev);
     } else {
         int64_t n = (int64_t)F * P.Mv;
-        const size_t shq = sizeof(unsigned) * ((2 * P.nbins + 3) & ~3) + sizeof(int4) * QCAP * (BLOCK / 64);
+        const int gwords = P.glds ? P.nbins * (P.gnu + (P.self_ ? 0 : P.gnv)) : 0;
+        const size_t shq = sizeof(unsigned) * ((2 * P.nbins + gwords + 3) & ~3) + sizeof(int4) * QCAP * (BLOCK / 64);
         const dim3 gr((unsigned)std::max<int64_t>(1, std::min<int64_t>((P.Mv + BLOCK - 1) / BLOCK, 4096 / F + 1)),
                       (unsigned)F);
         if (f32) hipLaunchKernelGGL((k_eval_mols_real<float, float>), gr, dim3(BLOCK), shq, s, P, B, uf, vf);
@@ -2055,8 +2111,10 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         if (tiles) {
             launch_tiles_c(J->tile_c, f32, true, s, J->eval_grid, J->cus, P, B, xu_dev, nullptr);
         } else {
-            if (f32) hipLaunchKernelGGL(k_eval_mols_rand<float>, dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, uf);
-            else hipLaunchKernelGGL(k_eval_mols_rand<double>, dim3(grid_for(n)), dim3(BLOCK), shm, s, P, B, ud);
+            const int gwords = P.glds ? P.nbins * (P.gnu + (P.self_ ? 0 : P.gnv)) : 0;
+            const size_t shr = sizeof(unsigned) * (2 * P.nbins + gwords);
+            if (f32) hipLaunchKernelGGL(k_eval_mols_rand<float>, dim3(grid_for(n)), dim3(BLOCK), shr, s, P, B, uf);
+            else hipLaunchKernelGGL(k_eval_mols_rand<double>, dim3(grid_for(n)), dim3(BLOCK), shr, s, P, B, ud);
         }
         if (J->profiling) HIPCHK(hipEventRecord(ev[3], s));
     }
@@ -2320,6 +2378,10 @@ void job_params(cm_job* J, const cm_job_desc& d)
     J->tile_c = 1;
     P.mpt = 64 / P.nv;
     P.tcap = TCAP_DEFAULT;
+    P.gnu = d.n_groups_u;
+    P.gnv = d.n_groups_v;
+    // molecule-pair kernels keep the group counts in LDS when they fit
+    P.glds = (int64_t)d.nbins * (d.n_groups_u + (d.autocorrelation ? 0 : d.n_groups_v)) * 4 <= GLDS_MAX ? 1 : 0;
     {
         int dev = 0;
         if (hipGetDevice(&dev) == hipSuccess)

@@ -396,3 +396,41 @@ def test_bin_edges_self(pkg, oracle):
     a, _ = oracle_raw(pkg, oracle, traj, opts)
     b = gpu_raw(pkg, traj, opts)
     assert_raw_equal(a, b)
+
+
+def test_molecule_pair_queue_cross(pkg, oracle):
+    """Many solute molecules in a box with three molecule cells per side:
+    the queued pair evaluation (cross, with ideal-gas passes)."""
+    S = syn()
+    rng = np.random.default_rng(31)
+    box = 40.0
+    wt = S.water_template()
+    xu = S.place_molecules(rng, wt, rng.uniform(0, box, (60, 3)))
+    xv = S.place_molecules(rng, wt, rng.uniform(0, box, (1500, 3)))
+    xu = np.stack([xu + rng.normal(scale=0.3, size=xu.shape) for _ in range(2)])
+    xv = np.stack([xv + rng.normal(scale=0.3, size=xv.shape) for _ in range(2)])
+    su = pkg.AtomSelection(list(range(1, 181)), natomspermol=3)
+    sv = pkg.AtomSelection(list(range(181, 181 + 4500)), natomspermol=3)
+    traj = pkg.MemoryTrajectory(xu, xv, np.repeat(np.diag([box] * 3)[None], 2, axis=0), su, sv)
+    opts = pkg.Options(dbulk=10.0, n_random_samples=5, seed=13, silent=True)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts)
+    assert_raw_equal(a, b)
+
+
+def test_molecule_pair_queue_self_many_groups(pkg, oracle):
+    """Self-MDDF of 24-atom rigid molecules (24 groups: counts too large for
+    the LDS copy, so they go straight to global memory) in a box with three
+    molecule cells per side, and the same molecules as water-sized groups."""
+    S = syn()
+    rng = np.random.default_rng(32)
+    mol = S.random_walk(rng, 24, bond=1.5, rmin=3.0, start=np.zeros(3))
+    box = 66.0
+    x = S.place_molecules(rng, mol, rng.uniform(0, box, (260, 3)))
+    xv = np.stack([x + rng.normal(scale=0.2, size=x.shape) for _ in range(2)])
+    sel = pkg.AtomSelection(list(range(1, 24 * 260 + 1)), natomspermol=24)
+    traj = pkg.MemoryTrajectory(xv, xv, np.repeat(np.diag([box] * 3)[None], 2, axis=0), sel, sel)
+    opts = pkg.Options(dbulk=10.0, n_random_samples=3, seed=14, silent=True)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts)
+    assert_raw_equal(a, b)
