@@ -161,9 +161,13 @@ def main():
     frames = np.arange(1, nf + 1)
     weights = np.ones(nf)
 
-    tuning = None
     if args.cell_fraction or args.columns:
         tuning = dict(cell_fraction=args.cell_fraction or 0.5, columns=args.columns)
+    else:
+        # the autotuner's choice on the first 8 frames (untimed)
+        import complexmixtures_amd.serve as SV
+        from complexmixtures_amd.mddf import job_kwargs
+        tuning, _ = SV.autotune_job(job_kwargs(R, opts, tm, False), du, dv, cells, frames=8)
 
     def one_job(profile=False):
         job = _job_for(R, opts, tm, False, tuning=tuning)
@@ -234,6 +238,7 @@ def main():
                      "eval_ms_per_frame": round(st["eval_ms"] / nf, 5),
                      "hbm_GBps_algorithmic": round(bpf * nf / eval_s / 1e9, 2),
                      "fp64_exhaustive_resolves": st["fp64_rechecks"]},
+        "tuning": tuning,
         "host_input_frames_per_s": round(host_fps, 2),
         "gen_s": round(gen_s, 2),
     }

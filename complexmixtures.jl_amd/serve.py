@@ -220,17 +220,16 @@ def service_breakdown(reqs, service):
 TUNING_CANDIDATES = [dict(cell_fraction=cf, columns=c) for cf in (0.35, 0.5, 0.75, 1.0) for c in (1, 2)]
 
 
-def autotune(system, frames=4, candidates=TUNING_CANDIDATES):
-    """Times the single-solute kernels of this system (ideal-gas passes
-    included) for each candidate tuning on the first frames and keeps the
-    fastest; every candidate must give the same counts."""
-    req = Request(arrival=0.0, system=0, nframes=min(frames, system.nframes), mddf=True, self_=False, seed=12345)
-    kw, du, dv, _ = job_kwargs(system, req)
+def autotune_job(job_kw, d_solute, d_solvent, cells, frames=4, candidates=TUNING_CANDIDATES):
+    """Times the single-solute kernels of one job description on its first
+    frames for each candidate tuning (kernel time from HIP events) and returns
+    (best tuning, [(ms, tuning)]); every candidate must give the same counts."""
+    nf = min(frames, d_solute.shape[0])
     best, ref, timing = None, None, []
     for cand in candidates:
-        job = _lib.Job(tuning=cand, **kw)
+        job = _lib.Job(tuning=cand, **job_kw)
         job.set_profiling(True)
-        job.submit_device(np.arange(1, req.nframes + 1), du, dv, 0, system.cells[:req.nframes], np.ones(req.nframes))
+        job.submit_device(np.arange(1, nf + 1), d_solute, d_solvent, 0, cells[:nf], np.ones(nf))
         raw = job.finish()
         ms = job.stats()["total_ms"]
         job.close()
@@ -242,5 +241,13 @@ def autotune(system, frames=4, candidates=TUNING_CANDIDATES):
         timing.append((ms, cand))
         if best is None or ms < best[0]:
             best = (ms, cand)
-    system.tuning = dict(best[1])
+    return dict(best[1]), timing
+
+
+def autotune(system, frames=4, candidates=TUNING_CANDIDATES):
+    """Autotunes the cross (single-solute) requests of a system with the
+    ideal-gas passes on and keeps the result in system.tuning."""
+    req = Request(arrival=0.0, system=0, nframes=min(frames, system.nframes), mddf=True, self_=False, seed=12345)
+    kw, du, dv, _ = job_kwargs(system, req)
+    system.tuning, timing = autotune_job(kw, du, dv, system.cells, frames, candidates)
     return system.tuning, timing
