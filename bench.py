@@ -59,6 +59,46 @@ def cpu_baseline(pkg, O, s, opts, tm, nframes_sample, threads):
     return f / dt, dt
 
 
+def config3_rates(args):
+    """Frames/s of the triclinic mixed case (a = b = c = 90 A, 80 degree angles,
+    FP64 input): 3,000-atom polymer vs 12,000 urea and vs 15,000 waters, and
+    urea vs urea (self-MDDF), n_random_samples 10."""
+    import complexmixtures_amd as cm
+    import complexmixtures_amd.synthetic as S
+    from complexmixtures_amd import _lib
+    from complexmixtures_amd.mddf import TrajectoryMetaData, _job_for
+    from complexmixtures_amd.results import make_result
+    from complexmixtures_amd.trajectory import convert_unitcell
+    out = {}
+    nf = args.config3_frames
+    for name in ("urea", "water", "self"):
+        s = S.config3(nframes=nf, solvent=name)
+        opts = cm.Options(dbulk=10.0, binstep=0.02, n_random_samples=10, seed=20261205, silent=True)
+        traj = cm.MemoryTrajectory(s["x_solute"], s["x_solvent"], s["cells"], s["solute"], s["solvent"])
+        tm = TrajectoryMetaData(traj, opts)
+        R = make_result(traj, opts, tm)
+        du = _lib.DeviceArray(s["x_solute"])
+        dv = du if name == "self" else _lib.DeviceArray(s["x_solvent"])
+        cells = np.stack([convert_unitcell(c, tm.orthorhombic) for c in s["cells"]])
+        rate = None
+        for rep in range(2):
+            job = _job_for(R, opts, tm, False)
+            job.set_profiling(True)
+            t = time.perf_counter()
+            job.submit_device(np.arange(1, nf + 1), du, dv, 0, cells, np.ones(nf))
+            job.finish()
+            el = time.perf_counter() - t
+            st = job.stats()
+            job.close()
+            rate = {"frames_per_s": round(nf / el, 2), "eval_ms_per_frame": round(st["eval_ms"] / nf, 4),
+                    "fp64_exhaustive_resolves": st["fp64_rechecks"]}
+        out["polymer_vs_" + name if name != "self" else "urea_self"] = rate
+        du.free()
+        if dv is not du:
+            dv.free()
+    return {"frames": nf, "triclinic": True, "input": "f64", **out}
+
+
 def served_stream(args):
     """Served mixed-size stream (Poisson arrivals at 20 requests/s) through the
     request server vs the plain one-job-at-a-time path, plus both paths'
@@ -118,6 +158,8 @@ def main():
     ap.add_argument("--serve-systems", type=int, default=12)
     ap.add_argument("--serve-frames", type=int, default=200)
     ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--no-config3", action="store_true", help="skip the triclinic mixed-case rates")
+    ap.add_argument("--config3-frames", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain read-N/write-N elementwise kernel over MIB MiB three times (the "
                          "reference for scaling FETCH_SIZE/WRITE_SIZE counters), then exit")
@@ -254,6 +296,8 @@ def main():
     dv.free()
     if not args.no_serve:
         out["served_stream"] = served_stream(args)
+    if not args.no_config3:
+        out["triclinic_mixed"] = config3_rates(args)
     print(json.dumps(out), flush=True)
 
 
