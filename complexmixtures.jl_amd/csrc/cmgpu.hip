@@ -791,18 +791,35 @@ __device__ __forceinline__ int wave_max_i(int x)
     return __builtin_amdgcn_readlane(x, 63);
 }
 
+// Lattice shifts m (per dimension, at most three) that map the cell index cv
+// of one molecule onto the neighbour cell cu of another: the offsets o in
+// {-1, 0, 1} with wrap(cv + o) == cu.  With three or more cells per dimension
+// there is exactly one; small grids map several offsets onto one cell.
+__device__ __forceinline__ int shifts_onto(int cv, int cu, int n, int m[3])
+{
+    int k = 0;
+    for (int o = -1; o <= 1; o++) {
+        int mm;
+        if (nb_cell(cv, o, n, mm) == cu) {
+            bool dup = false;
+            for (int e = 0; e < k; e++) dup |= (m[e] == mm);
+            if (!dup) m[k++] = mm;
+        }
+    }
+    return k;
+}
+
 // Real pass of the many-solute-molecule family.  blockIdx.y is the frame;
-// each lane owns one solvent molecule v (self: in cell order).  When every
-// dimension has at least three molecule cells, the 27 neighbour cells are
-// distinct and each close pair has one lattice shift: the lanes walk the cells
-// together (trip count = the wavefront's largest cell), the molecule pairs
-// that pass the centre test are appended to an LDS queue (ballot offsets),
-// and the queue is evaluated 64 pairs at a time, one pair per lane, so the
-// nu x nv atom loops run without divergence.  Self-MDDF evaluates each
-// unordered pair once (u < v) and emits the records of both orientations
-// (same distance and argmin pair; reference atoms and tie order per
-// orientation).  Smaller grids fall back to a per-lane walk with all shifts of
-// a cell grouped per molecule pair.
+// each lane owns one solvent molecule v (self: in cell order).  The lanes walk
+// their distinct neighbour cells together (trip count = the wavefront's
+// largest cell, four candidates per step with their loads in flight), the
+// molecule pairs with a centre within reach under some lattice shift are
+// appended to an LDS queue (ballot offsets), and the queue is evaluated 64
+// pairs at a time, one pair per lane, so the nu x nv atom loops (over every
+// shift that maps the two cells onto each other) run without divergence.
+// Self-MDDF evaluates each unordered pair once (u < v) and emits the records
+// of both orientations (same distance and argmin pair; reference atoms and tie
+// order per orientation).
 template <class TU, class TV>
 __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU* xu, const TV* xv)
 {
@@ -819,10 +836,12 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
     const int* st = B.start + (int64_t)f * (P.cap + 1);
     const float4* solw = B.solw + (int64_t)f * P.Nu;
     const float4* vwf = B.vw + (int64_t)f * P.Nv;
+    const float4* vcen = B.vcen + (int64_t)f * P.Mv;
+    const int* vcellf = B.vcell + (int64_t)f * P.Mv;
     const TU* xuf = xu + (int64_t)f * P.Nu * 3;
     const TV* xvf = xv + (int64_t)f * P.Nv * 3;
     const int nc[3] = {g.nc[0], g.nc[1], g.nc[2]};
-    const bool simple = nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3;
+    const int nd[3] = {min(nc[0], 3), min(nc[1], 3), min(nc[2], 3)};
     const int* dp = B.drawn_ptr + (int64_t)f * (P.Mu + 1);
     const int* di = B.drawn_idx + (int64_t)f * P.K;
     // record of solute molecule uu and solvent molecule vv from their FP32 bests
@@ -835,25 +854,38 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
             for (int qq = dp[uu]; qq < dp[uu + 1]; qq++)
                 B.bflag[((int64_t)f * P.kb + di[qq]) * P.Mv + vv] = inbulk(P, r) ? 1 : 0;
     };
-    // one queued pair: solute u shifted by lattice vector code, solvent v
+    // one queued pair: solute u in cell cu, solvent v; every shift of the pair
     auto eval_pair = [&](int4 e) {
-        const int u = e.x, v = e.y, code = e.z;
-        float tx, ty, tz;
-        shift_of(g, code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1, tx, ty, tz);
+        const int u = e.x, v = e.y, cu = e.z;
+        const int vc = vcellf[v];
+        int mx[3], my[3], mz[3];
+        const int kx = shifts_onto(vc % nc[0], cu % nc[0], nc[0], mx);
+        const int ky = shifts_onto((vc / nc[0]) % nc[1], (cu / nc[0]) % nc[1], nc[1], my);
+        const int kz = shifts_onto(vc / (nc[0] * nc[1]), cu / (nc[0] * nc[1]), nc[2], mz);
         const float4* xs = solw + (int64_t)u * P.nu;
         const float4* yw = vwf + (int64_t)v * P.nv;
+        const float4 cv = vcen[v];
+        const float4 cuc = B.ucen[(int64_t)f * P.Mu + u];
         Best bm, brv, bru;
         bm.init(); brv.init(); bru.init();
-        for (int a = 0; a < P.nv; a++) {
-            const float4 y = yw[a];
-            const float yx = y.x - tx, yy = y.y - ty, yz = y.z - tz;
-            for (int i = 0; i < P.nu; i++) {
-                const float4 s4 = xs[i];
-                const float dx = yx - s4.x, dy = yy - s4.y, dz = yz - s4.z;
-                const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                bm.offer(d2, i, a);
-                if (a == P.iref) brv.offer(d2, i, a);
-                if (P.self_ && i == P.iref) bru.offer(d2, a, i);
+        for (int iz = 0; iz < kz; iz++)
+        for (int iy = 0; iy < ky; iy++)
+        for (int ix = 0; ix < kx; ix++) {
+            float tx, ty, tz;
+            shift_of(g, mx[ix], my[iy], mz[iz], tx, ty, tz);
+            const float dx = cv.x - (cuc.x + tx), dy = cv.y - (cuc.y + ty), dz = cv.z - (cuc.z + tz);
+            if (dx * dx + dy * dy + dz * dz > g.rpre2) continue;
+            for (int a = 0; a < P.nv; a++) {
+                const float4 y = yw[a];
+                const float yx = y.x - tx, yy = y.y - ty, yz = y.z - tz;
+                for (int i = 0; i < P.nu; i++) {
+                    const float4 s4 = xs[i];
+                    const float ex = yx - s4.x, ey = yy - s4.y, ez = yz - s4.z;
+                    const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+                    bm.offer(d2, i, a);
+                    if (a == P.iref) brv.offer(d2, i, a);
+                    if (P.self_ && i == P.iref) bru.offer(d2, a, i);
+                }
             }
         }
         emit(u, v, bm, brv);
@@ -864,40 +896,58 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
             emit(v, u, bs, bru);
         }
     };
+    constexpr int CU = 4;   // candidates per lane per step
     const int stride = gridDim.x * blockDim.x;
+    const unsigned long long lt = (1ull << lane) - 1ull;
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < P.Mv; base += stride) {
         const int q0 = base + lane;
         const bool act = q0 < P.Mv;
         const int v = act ? (P.self_ ? __float_as_int(srt[q0].w) : q0) : 0;
-        const float4 cv = B.vcen[(int64_t)f * P.Mv + v];
-        const int vcell = B.vcell[(int64_t)f * P.Mv + v];
+        const float4 cv = vcen[v];
+        const int vcell = vcellf[v];
         const int cc[3] = {vcell % nc[0], (vcell / nc[0]) % nc[1], vcell / (nc[0] * nc[1])};
-        if (simple) {
-            int qn = 0;
-            const unsigned long long lt = (1ull << lane) - 1ull;
-            for (int o = 0; o < 27; o++) {
-                const int ox = o % 3 - 1, oy = (o / 3) % 3 - 1, oz = o / 9 - 1;
-                int mx, my, mz;
-                const int cx = nb_cell(cc[0], ox, nc[0], mx), cy = nb_cell(cc[1], oy, nc[1], my),
-                          cz = nb_cell(cc[2], oz, nc[2], mz);
-                float tx, ty, tz;
-                shift_of(g, mx, my, mz, tx, ty, tz);
-                const int code = (mx + 1) + 3 * (my + 1) + 9 * (mz + 1);
-                const int cell = (cz * nc[1] + cy) * nc[0] + cx;
-                const int cs = st[cell];
-                const int cnt = act ? st[cell + 1] - cs : 0;
-                const int kmax = wave_max_i(cnt);
-                for (int k = 0; k < kmax; k++) {
-                    bool pass = false;
-                    int u = 0;
-                    if (k < cnt) {
-                        const float4 cu = srt[cs + k];
-                        u = __float_as_int(cu.w);
-                        const float dx = cv.x - (cu.x + tx), dy = cv.y - (cu.y + ty), dz = cv.z - (cu.z + tz);
-                        pass = (P.self_ ? u < v : true) && dx * dx + dy * dy + dz * dz <= g.rpre2;
+        int qn = 0;
+        const int ncells = nd[0] * nd[1] * nd[2];
+        for (int o = 0; o < ncells; o++) {
+            // distinct neighbour cells: offsets -1..1, or 0..nc-1 when nc < 3
+            const int ox = o % nd[0], oy = (o / nd[0]) % nd[1], oz = o / (nd[0] * nd[1]);
+            int m0;
+            const int cx = nb_cell(cc[0], nc[0] >= 3 ? ox - 1 : ox, nc[0], m0);
+            const int cy = nb_cell(cc[1], nc[1] >= 3 ? oy - 1 : oy, nc[1], m0);
+            const int cz = nb_cell(cc[2], nc[2] >= 3 ? oz - 1 : oz, nc[2], m0);
+            const int cell = (cz * nc[1] + cy) * nc[0] + cx;
+            // the shifts under which this cell neighbours v's cell
+            int sx[3], sy[3], sz[3];
+            const int kx = shifts_onto(cc[0], cx, nc[0], sx);
+            const int ky = shifts_onto(cc[1], cy, nc[1], sy);
+            const int kz = shifts_onto(cc[2], cz, nc[2], sz);
+            const int cs = st[cell];
+            const int cnt = act ? st[cell + 1] - cs : 0;
+            const int kmax = wave_max_i(cnt);
+            for (int k0 = 0; k0 < kmax; k0 += CU) {
+                float4 cu4[CU];
+#pragma unroll
+                for (int w = 0; w < CU; w++) cu4[w] = srt[min(cs + min(k0 + w, max(cnt - 1, 0)), P.Mu - 1)];
+#pragma unroll
+                for (int w = 0; w < CU; w++) {
+                    if (k0 + w >= kmax) break;
+                    const int u = __float_as_int(cu4[w].w);
+                    bool pass = k0 + w < cnt && (P.self_ ? u < v : true);
+                    if (pass) {
+                        bool nearby = false;
+                        for (int iz = 0; iz < kz; iz++)
+                        for (int iy = 0; iy < ky; iy++)
+                        for (int ix = 0; ix < kx; ix++) {
+                            float tx, ty, tz;
+                            shift_of(g, sx[ix], sy[iy], sz[iz], tx, ty, tz);
+                            const float dx = cv.x - (cu4[w].x + tx), dy = cv.y - (cu4[w].y + ty),
+                                        dz = cv.z - (cu4[w].z + tz);
+                            nearby |= dx * dx + dy * dy + dz * dz <= g.rpre2;
+                        }
+                        pass = nearby;
                     }
                     const unsigned long long m = __ballot(pass);
-                    if (pass) q[qn + __popcll(m & lt)] = make_int4(u, v, code, 0);
+                    if (pass) q[qn + __popcll(m & lt)] = make_int4(u, v, cell, 0);
                     qn += __popcll(m);
                     if (qn >= 64) {
                         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -908,59 +958,10 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
                     }
                 }
             }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            if (lane < qn) eval_pair(q[lane]);
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            continue;
         }
-        if (!act) continue;
-        const float4* yw = vwf + (int64_t)v * P.nv;
-        // distinct neighbour cells per dimension with all the shifts that map onto them
-        int dcell[3][3], dns[3], dsh[3][3][3];
-        for (int k = 0; k < 3; k++) {
-            dns[k] = 0;
-            int cnts[3] = {0, 0, 0};
-            for (int o = -1; o <= 1; o++) {
-                int m;
-                int c = nb_cell(cc[k], o, nc[k], m);
-                int qq = -1;
-                for (int e = 0; e < dns[k]; e++) if (dcell[k][e] == c) qq = e;
-                if (qq < 0) { qq = dns[k]++; dcell[k][qq] = c; }
-                dsh[k][qq][cnts[qq]++] = m;
-            }
-            for (int e = 0; e < dns[k]; e++) dsh[k][e][2] = cnts[e] > 2 ? dsh[k][e][2] : 99;
-            for (int e = 0; e < dns[k]; e++) if (cnts[e] < 2) dsh[k][e][1] = 99;
-        }
-        for (int ez = 0; ez < dns[2]; ez++)
-        for (int ey = 0; ey < dns[1]; ey++)
-        for (int ex = 0; ex < dns[0]; ex++) {
-            int cell = (dcell[2][ez] * nc[1] + dcell[1][ey]) * nc[0] + dcell[0][ex];
-            for (int p = st[cell]; p < st[cell + 1]; p++) {
-                float4 cu = srt[p];
-                int u = __float_as_int(cu.w);
-                if (P.self_ && u == v) continue;
-                Best bm, br;
-                bm.init(); br.init();
-                bool any = false;
-                for (int sz = 0; sz < 3; sz++) {
-                    int mz = dsh[2][ez][sz]; if (mz == 99) break;
-                    for (int sy = 0; sy < 3; sy++) {
-                        int my = dsh[1][ey][sy]; if (my == 99) break;
-                        for (int sx = 0; sx < 3; sx++) {
-                            int mx = dsh[0][ex][sx]; if (mx == 99) break;
-                            float tx, ty, tz;
-                            shift_of(g, mx, my, mz, tx, ty, tz);
-                            float dx = cv.x - (cu.x + tx), dy = cv.y - (cu.y + ty), dz = cv.z - (cu.z + tz);
-                            if (dx * dx + dy * dy + dz * dz > g.rpre2) continue;
-                            any = true;
-                            pairs_mol(P, solw + (int64_t)u * P.nu, yw, tx, ty, tz, bm, br);
-                        }
-                    }
-                }
-                if (!any) continue;
-                emit(u, v, bm, br);
-            }
-        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (lane < qn) eval_pair(q[lane]);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
     hist_flush(sh, P.nbins, B.cnt_md, B.cnt_rdf);
     if (shg) groups_flush(P, shg, B.cnt_gu, B.cnt_gv);
