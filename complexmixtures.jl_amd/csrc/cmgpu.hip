@@ -842,6 +842,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
     const TV* xvf = xv + (int64_t)f * P.Nv * 3;
     const int nc[3] = {g.nc[0], g.nc[1], g.nc[2]};
     const int nd[3] = {min(nc[0], 3), min(nc[1], 3), min(nc[2], 3)};
+    const bool simple = nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3;
     const int* dp = B.drawn_ptr + (int64_t)f * (P.Mu + 1);
     const int* di = B.drawn_idx + (int64_t)f * P.K;
     // record of solute molecule uu and solvent molecule vv from their FP32 bests
@@ -857,11 +858,19 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
     // one queued pair: solute u in cell cu, solvent v; every shift of the pair
     auto eval_pair = [&](int4 e) {
         const int u = e.x, v = e.y, cu = e.z;
-        const int vc = vcellf[v];
         int mx[3], my[3], mz[3];
-        const int kx = shifts_onto(vc % nc[0], cu % nc[0], nc[0], mx);
-        const int ky = shifts_onto((vc / nc[0]) % nc[1], (cu / nc[0]) % nc[1], nc[1], my);
-        const int kz = shifts_onto(vc / (nc[0] * nc[1]), cu / (nc[0] * nc[1]), nc[2], mz);
+        int kx = 1, ky = 1, kz = 1;
+        if (e.w >= 0) {
+            // the single shift of a grid with three or more cells per dimension
+            mx[0] = e.w % 3 - 1;
+            my[0] = (e.w / 3) % 3 - 1;
+            mz[0] = e.w / 9 - 1;
+        } else {
+            const int vc = vcellf[v];
+            kx = shifts_onto(vc % nc[0], cu % nc[0], nc[0], mx);
+            ky = shifts_onto((vc / nc[0]) % nc[1], (cu / nc[0]) % nc[1], nc[1], my);
+            kz = shifts_onto(vc / (nc[0] * nc[1]), cu / (nc[0] * nc[1]), nc[2], mz);
+        }
         const float4* xs = solw + (int64_t)u * P.nu;
         const float4* yw = vwf + (int64_t)v * P.nv;
         const float4 cv = vcen[v];
@@ -916,11 +925,23 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
             const int cy = nb_cell(cc[1], nc[1] >= 3 ? oy - 1 : oy, nc[1], m0);
             const int cz = nb_cell(cc[2], nc[2] >= 3 ? oz - 1 : oz, nc[2], m0);
             const int cell = (cz * nc[1] + cy) * nc[0] + cx;
-            // the shifts under which this cell neighbours v's cell
+            // the shifts under which this cell neighbours v's cell (one when
+            // every dimension has three or more cells)
             int sx[3], sy[3], sz[3];
-            const int kx = shifts_onto(cc[0], cx, nc[0], sx);
-            const int ky = shifts_onto(cc[1], cy, nc[1], sy);
-            const int kz = shifts_onto(cc[2], cz, nc[2], sz);
+            int kx = 1, ky = 1, kz = 1, code = -1;
+            float t1x = 0.f, t1y = 0.f, t1z = 0.f;
+            if (simple) {
+                int mx, my, mz;
+                nb_cell(cc[0], ox - 1, nc[0], mx);
+                nb_cell(cc[1], oy - 1, nc[1], my);
+                nb_cell(cc[2], oz - 1, nc[2], mz);
+                code = (mx + 1) + 3 * (my + 1) + 9 * (mz + 1);
+                shift_of(g, mx, my, mz, t1x, t1y, t1z);
+            } else {
+                kx = shifts_onto(cc[0], cx, nc[0], sx);
+                ky = shifts_onto(cc[1], cy, nc[1], sy);
+                kz = shifts_onto(cc[2], cz, nc[2], sz);
+            }
             const int cs = st[cell];
             const int cnt = act ? st[cell + 1] - cs : 0;
             const int kmax = wave_max_i(cnt);
@@ -933,7 +954,11 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
                     if (k0 + w >= kmax) break;
                     const int u = __float_as_int(cu4[w].w);
                     bool pass = k0 + w < cnt && (P.self_ ? u < v : true);
-                    if (pass) {
+                    if (simple) {
+                        const float dx = cv.x - (cu4[w].x + t1x), dy = cv.y - (cu4[w].y + t1y),
+                                    dz = cv.z - (cu4[w].z + t1z);
+                        pass = pass && dx * dx + dy * dy + dz * dz <= g.rpre2;
+                    } else if (pass) {
                         bool nearby = false;
                         for (int iz = 0; iz < kz; iz++)
                         for (int iy = 0; iy < ky; iy++)
@@ -947,7 +972,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
                         pass = nearby;
                     }
                     const unsigned long long m = __ballot(pass);
-                    if (pass) q[qn + __popcll(m & lt)] = make_int4(u, v, cell, 0);
+                    if (pass) q[qn + __popcll(m & lt)] = make_int4(u, v, cell, code);
                     qn += __popcll(m);
                     if (qn >= 64) {
                         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
