@@ -434,3 +434,57 @@ def test_molecule_pair_queue_self_many_groups(pkg, oracle):
     a, _ = oracle_raw(pkg, oracle, traj, opts)
     b = gpu_raw(pkg, traj, opts)
     assert_raw_equal(a, b)
+
+
+@pytest.mark.parametrize("nv", [64, 65])
+def test_large_solvent_molecules(pkg, oracle, nv):
+    """Solvent molecules of 64 atoms (the largest the single-solute tile
+    kernels take) and of 65 atoms (molecule-pair kernels with one solute
+    molecule)."""
+    S = syn()
+    rng = np.random.default_rng(40 + nv)
+    mol = S.random_walk(rng, nv, bond=1.2, rmin=1.5, start=np.zeros(3))
+    box = 60.0
+    xv = S.place_molecules(rng, mol, rng.uniform(0, box, (40, 3)))
+    xu = S.random_walk(rng, 120, bond=1.5, rmin=3.0, start=np.full(3, box / 2))
+    xu = np.stack([xu + rng.normal(scale=0.2, size=xu.shape) for _ in range(2)])
+    xv = np.stack([xv + rng.normal(scale=0.2, size=xv.shape) for _ in range(2)])
+    su = pkg.AtomSelection(list(range(1, 121)), nmols=1)
+    sv = pkg.AtomSelection(list(range(121, 121 + 40 * nv)), natomspermol=nv)
+    traj = pkg.MemoryTrajectory(xu, xv, np.repeat(np.diag([box] * 3)[None], 2, axis=0), su, sv)
+    opts = pkg.Options(dbulk=10.0, n_random_samples=4, seed=41, silent=True)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts)
+    assert_raw_equal(a, b)
+
+
+def test_ions_and_empty_frames(pkg, oracle):
+    """Single-atom solute and solvent, with a frame where no solvent atom is
+    near the solute and one where the only solvent atom sits on the solute."""
+    xu = np.array([[[10.0, 10.0, 10.0]], [[10.0, 10.0, 10.0]], [[10.0, 10.0, 10.0]]])
+    xv = np.array([[[35.0, 35.0, 35.0], [36.0, 35.0, 35.0]],
+                   [[10.0, 10.0, 10.0], [35.0, 35.0, 35.0]],
+                   [[13.0, 10.0, 10.0], [10.0, 18.5, 10.0]]])
+    su = pkg.AtomSelection([1], nmols=1)
+    sv = pkg.AtomSelection([2, 3], natomspermol=1)
+    traj = pkg.MemoryTrajectory(xu, xv, np.repeat(np.diag([50.0] * 3)[None], 3, axis=0), su, sv)
+    for opts in (pkg.Options(dbulk=10.0, n_random_samples=3, seed=42, silent=True),
+                 pkg.Options(dbulk=6.0, cutoff=10.0, usecutoff=True, n_random_samples=2, seed=43, silent=True)):
+        a, _ = oracle_raw(pkg, oracle, traj, opts)
+        b = gpu_raw(pkg, traj, opts)
+        assert_raw_equal(a, b)
+
+
+def test_self_coordination_numbers(pkg, oracle):
+    """Self coordination numbers (no ideal-gas passes) of waters."""
+    S = syn()
+    rng = np.random.default_rng(44)
+    box = 34.0
+    x = S.place_molecules(rng, S.water_template(), rng.uniform(0, box, (1200, 3)))
+    xv = np.stack([x + rng.normal(scale=0.2, size=x.shape) for _ in range(2)])
+    sel = pkg.AtomSelection(list(range(1, 3601)), natomspermol=3)
+    traj = pkg.MemoryTrajectory(xv, xv, np.repeat(np.diag([box] * 3)[None], 2, axis=0), sel, sel)
+    opts = pkg.Options(dbulk=10.0, seed=45, silent=True)
+    a, _ = oracle_raw(pkg, oracle, traj, opts, coordination_number_only=True)
+    b = gpu_raw(pkg, traj, opts, coordination_number_only=True)
+    assert_raw_equal(a, b)
