@@ -928,82 +928,6 @@ __device__ __forceinline__ void queue_drain(const CTX& ctx, int4* q, int qn, int
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
-// Self-MDDF real pass, one wavefront per molecule cell (blockIdx.y = frame):
-// the lanes own the cell's molecules (64 at a time); for each distinct
-// neighbour cell the wavefront loads that cell's molecule centres into LDS
-// with one load per lane and every lane tests them from LDS (no global
-// latency in the candidate loop); pairs within reach (u < v, each unordered
-// pair once) go through the pair queue.
-template <class TU, class TV>
-__global__ void __launch_bounds__(BLOCK) k_eval_self_cells(KP P, Bufs B, const TU* xu, const TV* xv)
-{
-    extern __shared__ unsigned sh[];
-    const int gwords = P.glds ? P.nbins * P.gnu : 0;
-    hist_clear(sh, 2 * P.nbins + gwords);
-    unsigned* shg = P.glds ? sh + 2 * P.nbins : nullptr;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int4* q = (int4*)(sh + ((2 * P.nbins + gwords + 3) & ~3)) + wv * QCAP;
-    float4* cb = (float4*)(sh + ((2 * P.nbins + gwords + 3) & ~3)) + (BLOCK / 64) * QCAP + wv * 64;
-    const int f = blockIdx.y;
-    const PairCtx<TU, TV> ctx(P, B, f, xu, xv, sh, shg);
-    const Geo& g = ctx.g;
-    const float4* srt = B.sorted + (int64_t)f * B.nitems;
-    const int* st = B.start + (int64_t)f * (P.cap + 1);
-    const int nc[3] = {g.nc[0], g.nc[1], g.nc[2]};
-    const int nd[3] = {min(nc[0], 3), min(nc[1], 3), min(nc[2], 3)};
-    for (int h = blockIdx.x * (BLOCK / 64) + wv; h < g.ncell; h += gridDim.x * (BLOCK / 64)) {
-        const int hc[3] = {h % nc[0], (h / nc[0]) % nc[1], h / (nc[0] * nc[1])};
-        const int hs = st[h], hn = st[h + 1] - hs;
-        for (int seg = 0; seg < hn; seg += 64) {
-            const bool act = seg + lane < hn;
-            const int v = act ? __float_as_int(srt[hs + seg + lane].w) : 0;
-            const float4 cv = ctx.vcen[v];
-            int qn = 0;
-            for (int o = 0; o < nd[0] * nd[1] * nd[2]; o++) {
-                const int ox = o % nd[0], oy = (o / nd[0]) % nd[1], oz = o / (nd[0] * nd[1]);
-                int m0;
-                const int cx = nb_cell(hc[0], nc[0] >= 3 ? ox - 1 : ox, nc[0], m0);
-                const int cy = nb_cell(hc[1], nc[1] >= 3 ? oy - 1 : oy, nc[1], m0);
-                const int cz = nb_cell(hc[2], nc[2] >= 3 ? oz - 1 : oz, nc[2], m0);
-                const int n = (cz * nc[1] + cy) * nc[0] + cx;
-                int sx[3], sy[3], sz[3];
-                const int kx = shifts_onto(hc[0], cx, nc[0], sx);
-                const int ky = shifts_onto(hc[1], cy, nc[1], sy);
-                const int kz = shifts_onto(hc[2], cz, nc[2], sz);
-                const int code = (kx == 1 && ky == 1 && kz == 1) ? (sx[0] + 1) + 3 * (sy[0] + 1) + 9 * (sz[0] + 1) : -1;
-                const int ns = st[n], nn = st[n + 1] - ns;
-                for (int b0 = 0; b0 < nn; b0 += 64) {
-                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                    if (b0 + lane < nn) cb[lane] = srt[ns + b0 + lane];
-                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                    const int m = min(64, nn - b0);
-                    for (int j = 0; j < m; j++) {
-                        const float4 cu = cb[j];
-                        const int u = __float_as_int(cu.w);
-                        bool pass = act && u < v;
-                        if (pass) {
-                            bool nearby = false;
-                            for (int iz = 0; iz < kz; iz++)
-                            for (int iy = 0; iy < ky; iy++)
-                            for (int ix = 0; ix < kx; ix++) {
-                                float tx, ty, tz;
-                                shift_of(g, sx[ix], sy[iy], sz[iz], tx, ty, tz);
-                                const float dx = cv.x - (cu.x + tx), dy = cv.y - (cu.y + ty), dz = cv.z - (cu.z + tz);
-                                nearby |= dx * dx + dy * dy + dz * dz <= g.rpre2;
-                            }
-                            pass = nearby;
-                        }
-                        queue_push(ctx, q, qn, pass, make_int4(u, v, n, code), lane);
-                    }
-                }
-            }
-            queue_drain(ctx, q, qn, lane);
-        }
-    }
-    hist_flush(sh, P.nbins, B.cnt_md, B.cnt_rdf);
-    if (shg) groups_flush(P, shg, B.cnt_gu, B.cnt_gv);
-}
-
 // Real pass of the many-solute-molecule family.  blockIdx.y is the frame;
 // each lane owns one solvent molecule v (self: in cell order).  The lanes walk
 // their distinct neighbour cells together (trip count = the wavefront's
@@ -1022,93 +946,24 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
     const int gwords = P.glds ? P.nbins * (P.gnu + (P.self_ ? 0 : P.gnv)) : 0;
     hist_clear(sh, 2 * P.nbins + gwords);
     unsigned* shg = P.glds ? sh + 2 * P.nbins : nullptr;
-    int4* qall = (int4*)(sh + ((2 * P.nbins + gwords + 3) & ~3));
-    int4* q = qall + (threadIdx.x >> 6) * QCAP;
+    int4* q = (int4*)(sh + ((2 * P.nbins + gwords + 3) & ~3)) + (threadIdx.x >> 6) * QCAP;
     const int lane = threadIdx.x & 63;
     const int f = blockIdx.y;
-    const Geo& g = B.geo[f];
+    const PairCtx<TU, TV> ctx(P, B, f, xu, xv, sh, shg);
+    const Geo& g = ctx.g;
     const float4* srt = B.sorted + (int64_t)f * B.nitems;
     const int* st = B.start + (int64_t)f * (P.cap + 1);
-    const float4* solw = B.solw + (int64_t)f * P.Nu;
-    const float4* vwf = B.vw + (int64_t)f * P.Nv;
-    const float4* vcen = B.vcen + (int64_t)f * P.Mv;
-    const int* vcellf = B.vcell + (int64_t)f * P.Mv;
-    const TU* xuf = xu + (int64_t)f * P.Nu * 3;
-    const TV* xvf = xv + (int64_t)f * P.Nv * 3;
     const int nc[3] = {g.nc[0], g.nc[1], g.nc[2]};
     const int nd[3] = {min(nc[0], 3), min(nc[1], 3), min(nc[2], 3)};
     const bool simple = nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3;
-    const int* dp = B.drawn_ptr + (int64_t)f * (P.Mu + 1);
-    const int* di = B.drawn_idx + (int64_t)f * P.K;
-    // record of solute molecule uu and solvent molecule vv from their FP32 bests
-    auto emit = [&](int uu, int vv, const Best& bm, const Best& br) {
-        const int64_t jb = (int64_t)vv * P.nv;
-        auto ysrc = [&](int a, double& y0, double& y1, double& y2) { load3(xvf, jb + a, y0, y1, y2); };
-        Rec r = resolve_mols(P, g, xuf, uu, vv, ysrc, bm, br, &B.stats[0]);
-        hist_record_mols(P, sh, shg, B.cnt_gu, B.cnt_gv, r);
-        if (P.K > 0 && r.within)
-            for (int qq = dp[uu]; qq < dp[uu + 1]; qq++)
-                B.bflag[((int64_t)f * P.kb + di[qq]) * P.Mv + vv] = inbulk(P, r) ? 1 : 0;
-    };
-    // one queued pair: solute u in cell cu, solvent v; every shift of the pair
-    auto eval_pair = [&](int4 e) {
-        const int u = e.x, v = e.y, cu = e.z;
-        int mx[3], my[3], mz[3];
-        int kx = 1, ky = 1, kz = 1;
-        if (e.w >= 0) {
-            // the single shift of a grid with three or more cells per dimension
-            mx[0] = e.w % 3 - 1;
-            my[0] = (e.w / 3) % 3 - 1;
-            mz[0] = e.w / 9 - 1;
-        } else {
-            const int vc = vcellf[v];
-            kx = shifts_onto(vc % nc[0], cu % nc[0], nc[0], mx);
-            ky = shifts_onto((vc / nc[0]) % nc[1], (cu / nc[0]) % nc[1], nc[1], my);
-            kz = shifts_onto(vc / (nc[0] * nc[1]), cu / (nc[0] * nc[1]), nc[2], mz);
-        }
-        const float4* xs = solw + (int64_t)u * P.nu;
-        const float4* yw = vwf + (int64_t)v * P.nv;
-        const float4 cv = vcen[v];
-        const float4 cuc = B.ucen[(int64_t)f * P.Mu + u];
-        Best bm, brv, bru;
-        bm.init(); brv.init(); bru.init();
-        for (int iz = 0; iz < kz; iz++)
-        for (int iy = 0; iy < ky; iy++)
-        for (int ix = 0; ix < kx; ix++) {
-            float tx, ty, tz;
-            shift_of(g, mx[ix], my[iy], mz[iz], tx, ty, tz);
-            const float dx = cv.x - (cuc.x + tx), dy = cv.y - (cuc.y + ty), dz = cv.z - (cuc.z + tz);
-            if (dx * dx + dy * dy + dz * dz > g.rpre2) continue;
-            for (int a = 0; a < P.nv; a++) {
-                const float4 y = yw[a];
-                const float yx = y.x - tx, yy = y.y - ty, yz = y.z - tz;
-                for (int i = 0; i < P.nu; i++) {
-                    const float4 s4 = xs[i];
-                    const float ex = yx - s4.x, ey = yy - s4.y, ez = yz - s4.z;
-                    const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-                    bm.offer(d2, i, a);
-                    if (a == P.iref) brv.offer(d2, i, a);
-                    if (P.self_ && i == P.iref) bru.offer(d2, a, i);
-                }
-            }
-        }
-        emit(u, v, bm, brv);
-        if (P.self_) {
-            Best bs = bm;
-            bs.bi = bm.ba;
-            bs.ba = bm.bi;
-            emit(v, u, bs, bru);
-        }
-    };
     constexpr int CU = 4;   // candidates per lane per step
     const int stride = gridDim.x * blockDim.x;
-    const unsigned long long lt = (1ull << lane) - 1ull;
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < P.Mv; base += stride) {
         const int q0 = base + lane;
         const bool act = q0 < P.Mv;
         const int v = act ? (P.self_ ? __float_as_int(srt[q0].w) : q0) : 0;
-        const float4 cv = vcen[v];
-        const int vcell = vcellf[v];
+        const float4 cv = ctx.vcen[v];
+        const int vcell = ctx.vcellf[v];
         const int cc[3] = {vcell % nc[0], (vcell / nc[0]) % nc[1], vcell / (nc[0] * nc[1])};
         int qn = 0;
         const int ncells = nd[0] * nd[1] * nd[2];
@@ -1166,22 +1021,11 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
                         }
                         pass = nearby;
                     }
-                    const unsigned long long m = __ballot(pass);
-                    if (pass) q[qn + __popcll(m & lt)] = make_int4(u, v, cell, code);
-                    qn += __popcll(m);
-                    if (qn >= 64) {
-                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                        const int4 e = q[qn - 64 + lane];
-                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                        qn -= 64;
-                        eval_pair(e);
-                    }
+                    queue_push(ctx, q, qn, pass, make_int4(u, v, cell, code), lane);
                 }
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (lane < qn) eval_pair(q[lane]);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        queue_drain(ctx, q, qn, lane);
     }
     hist_flush(sh, P.nbins, B.cnt_md, B.cnt_rdf);
     if (shg) groups_flush(P, shg, B.cnt_gu, B.cnt_gv);
@@ -2307,17 +2151,8 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         const size_t shq = sizeof(unsigned) * ((2 * P.nbins + gwords + 3) & ~3) + sizeof(int4) * QCAP * (BLOCK / 64);
         const dim3 gr((unsigned)std::max<int64_t>(1, std::min<int64_t>((P.Mv + BLOCK - 1) / BLOCK, 4096 / F + 1)),
                       (unsigned)F);
-        if (P.self_) {
-            // one wavefront per molecule cell (cap bounds the cells of every frame)
-            const size_t shc = shq + sizeof(float4) * 64 * (BLOCK / 64);
-            const dim3 gc((unsigned)std::max<int64_t>(1, std::min<int64_t>(((int64_t)P.cap + 3) / 4, 8192 / F + 1)),
-                          (unsigned)F);
-            if (f32) hipLaunchKernelGGL((k_eval_self_cells<float, float>), gc, dim3(BLOCK), shc, s, P, B, uf, vf);
-            else hipLaunchKernelGGL((k_eval_self_cells<double, double>), gc, dim3(BLOCK), shc, s, P, B, ud, vd);
-        } else {
-            if (f32) hipLaunchKernelGGL((k_eval_mols_real<float, float>), gr, dim3(BLOCK), shq, s, P, B, uf, vf);
-            else hipLaunchKernelGGL((k_eval_mols_real<double, double>), gr, dim3(BLOCK), shq, s, P, B, ud, vd);
-        }
+        if (f32) hipLaunchKernelGGL((k_eval_mols_real<float, float>), gr, dim3(BLOCK), shq, s, P, B, uf, vf);
+        else hipLaunchKernelGGL((k_eval_mols_real<double, double>), gr, dim3(BLOCK), shq, s, P, B, ud, vd);
     }
     if (J->profiling) HIPCHK(hipEventRecord(ev[1], s));
     HIPCHK(hipGetLastError());
