@@ -157,12 +157,16 @@ struct Rec {
     double d;
     int ref;
     double dref;
+    int bin;                 // 0-based bin of d when known without d (-1: from d)
+    int rbin;                // the same for dref
+    int bulk;                // d > dbulk when known without d (-1: from d)
 };
 
 __device__ __forceinline__ Rec rec_zero()
 {
     Rec r;
     r.within = 0; r.i = -1; r.j = -1; r.d = INFINITY; r.ref = 0; r.dref = INFINITY;
+    r.bin = -1; r.rbin = -1; r.bulk = -1;
     return r;
 }
 
@@ -503,17 +507,28 @@ __device__ __forceinline__ void group_hit(ull* gc, int nbins, int b, int64_t ato
     }
 }
 
+// bins and the dbulk test of a record from its distances where the fast
+// decision did not set them (setbin, src/results.jl:28)
+__device__ __forceinline__ void rec_complete(const KP& P, Rec& r)
+{
+    if (r.within && r.bin < 0) {
+        r.bin = (int)cm::setbin(r.d, P.binstep) - 1;
+        r.bulk = r.d > P.dbulk ? 1 : 0;
+    }
+    if (r.ref && r.rbin < 0) r.rbin = (int)cm::setbin(r.dref, P.binstep) - 1;
+}
+
 __device__ __forceinline__ void hist_record(const KP& P, unsigned* sh, ull* gu, ull* gv, const Rec& r)
 {
     if (!r.within) return;
-    int b = (int)cm::setbin(r.d, P.binstep) - 1;
+    int b = r.bin;
     if (b >= P.nbins) return;   // unreachable: d <= cutoff (validated at job creation)
     atomicAdd(&sh[b], 1u);
     group_hit(gu, P.nbins, b, r.i, P.gu_ptr, P.gu_idx, P.nu);
     if (P.self_) group_hit(gu, P.nbins, b, r.j, P.gv_ptr, P.gv_idx, P.nu);
     else group_hit(gv, P.nbins, b, r.j, P.gv_ptr, P.gv_idx, P.nv);
     if (r.ref) {
-        int rb = (int)cm::setbin(r.dref, P.binstep) - 1;
+        const int rb = r.rbin;
         if (rb < P.nbins) atomicAdd(&sh[P.nbins + rb], 1u);
     }
 }
@@ -522,14 +537,14 @@ __device__ __forceinline__ void hist_record(const KP& P, unsigned* sh, ull* gu, 
 __device__ __forceinline__ void hist_record_global(const KP& P, ull* md, ull* rdf, ull* gu, ull* gv, const Rec& r)
 {
     if (!r.within) return;
-    int b = (int)cm::setbin(r.d, P.binstep) - 1;
+    const int b = r.bin;
     if (b >= P.nbins) return;   // unreachable: d <= cutoff (validated at job creation)
     atomicAdd(&md[b], 1ull);
     group_hit(gu, P.nbins, b, r.i, P.gu_ptr, P.gu_idx, P.nu);
     if (P.self_) group_hit(gu, P.nbins, b, r.j, P.gv_ptr, P.gv_idx, P.nu);
     else group_hit(gv, P.nbins, b, r.j, P.gv_ptr, P.gv_idx, P.nv);
     if (r.ref) {
-        int rb = (int)cm::setbin(r.dref, P.binstep) - 1;
+        const int rb = r.rbin;
         if (rb < P.nbins) atomicAdd(&rdf[rb], 1ull);
     }
 }
@@ -556,7 +571,7 @@ __device__ __forceinline__ void hist_record_mols(const KP& P, unsigned* sh, unsi
         return;
     }
     if (!r.within) return;
-    const int b = (int)cm::setbin(r.d, P.binstep) - 1;
+    const int b = r.bin;
     if (b >= P.nbins) return;
     atomicAdd(&sh[b], 1u);
     unsigned* shu = shg;
@@ -565,7 +580,7 @@ __device__ __forceinline__ void hist_record_mols(const KP& P, unsigned* sh, unsi
     if (P.self_) group_hit_lds(shu, P.nbins, b, r.j, P.gv_ptr, P.gv_idx, P.nu);
     else group_hit_lds(shv, P.nbins, b, r.j, P.gv_ptr, P.gv_idx, P.nv);
     if (r.ref) {
-        const int rb = (int)cm::setbin(r.dref, P.binstep) - 1;
+        const int rb = r.rbin;
         if (rb < P.nbins) atomicAdd(&sh[P.nbins + rb], 1u);
     }
 }
@@ -597,24 +612,34 @@ __device__ __forceinline__ void hist_flush(unsigned* sh, int nbins, ull* md, ull
 
 __device__ __forceinline__ bool inbulk(const KP& P, const Rec& r)
 {
-    return P.usecutoff ? (r.within && r.d > P.dbulk) : !r.within;
+    return P.usecutoff ? (r.within && r.bulk == 1) : !r.within;
 }
 
 // A record from the FP32 minimum b of a unique argmin, without the FP64
-// recompute, when every FP64 d2 in [b - G, b + G] gives the same cutoff
-// test, bin and dbulk test (all monotone in d2): then *within is set and *d
-// is a distance with the bin and dbulk test of the FP64 one.  Returns false
-// when the interval straddles a decision and the pair must be recomputed.
-__device__ __forceinline__ bool decide_fp32(const KP& P, double G, float b, bool& within, double& d)
+// recompute, when every FP64 d2 in [b - G, b + G] gives the same cutoff test,
+// bin and dbulk test.  The bin of d2 is max(1, ceil(fl(sqrt(d2)) / binstep));
+// between ((k-1) binstep)^2 (1 + 1e-12) and (k binstep)^2 (1 - 1e-12) it is k
+// whatever the FP64 rounding of the square root, the quotient and the edges
+// (all below 1e-15 relative), and likewise d > dbulk is decided outside
+// dbulk^2 (1 -+ 1e-12).  Sets within, the 0-based bin and the dbulk flag;
+// returns false when the interval straddles a decision.
+__device__ __forceinline__ bool decide_fp32(const KP& P, double G, float b, bool& within, int& bin, int& bulk)
 {
     const double lo = fmax((double)b - G, 0.0), hi = (double)b + G;
     if (lo > P.cut2) { within = false; return true; }
     if (!(hi <= P.cut2)) return false;
-    const double dlo = sqrt(lo), dhi = sqrt(hi);
-    if (cm::setbin(dlo, P.binstep) != cm::setbin(dhi, P.binstep)) return false;
-    if ((dlo > P.dbulk) != (dhi > P.dbulk)) return false;
+    int k = (int)ceilf(sqrtf(b) / (float)P.binstep);
+    k = max(k, 1);
+    const double e_hi = (double)k * P.binstep, e_lo = (double)(k - 1) * P.binstep;
+    if (!(hi < e_hi * e_hi * (1.0 - 1e-12))) return false;
+    if (k > 1 && !(lo > e_lo * e_lo * (1.0 + 1e-12))) return false;
+    const double db2 = P.dbulk * P.dbulk;
+    if (!P.usecutoff) bulk = 0;   // the dbulk test only matters with usecutoff
+    else if (lo > db2 * (1.0 + 1e-12)) bulk = 1;
+    else if (hi < db2 * (1.0 - 1e-12)) bulk = 0;
+    else return false;
     within = true;
-    d = dlo;
+    bin = k - 1;
     return true;
 }
 
@@ -732,9 +757,9 @@ __device__ Rec resolve_mols(const KP& P, const Geo& g, const TU* xuf, int u, int
     if (bm.best <= g.cut2_hi) {
         if (bm.second - bm.best > g.amb) {
             bool w;
-            double dd;
-            if (decide_fp32(P, g.guard, bm.best, w, dd)) {
-                if (w) { r.within = 1; r.i = bm.bi; r.j = jbase + bm.ba; r.d = dd; }
+            int bn, bk;
+            if (decide_fp32(P, g.guard, bm.best, w, bn, bk)) {
+                if (w) { r.within = 1; r.i = bm.bi; r.j = jbase + bm.ba; r.d = bm.best; r.bin = bn; r.bulk = bk; }
             } else {
                 double x0, x1, x2, y0, y1, y2;
                 load3(xuf, ibase + bm.bi, x0, x1, x2);
@@ -747,9 +772,9 @@ __device__ Rec resolve_mols(const KP& P, const Geo& g, const TU* xuf, int u, int
     if (br.best <= g.cut2_hi) {
         if (br.second - br.best > g.amb) {
             bool w;
-            double dd;
-            if (decide_fp32(P, g.guard, br.best, w, dd)) {
-                if (w) { r.ref = 1; r.dref = dd; }
+            int bn, bk;
+            if (decide_fp32(P, g.guard, br.best, w, bn, bk)) {
+                if (w) { r.ref = 1; r.dref = br.best; r.rbin = bn; }
             } else {
                 double x0, x1, x2, y0, y1, y2;
                 load3(xuf, ibase + br.bi, x0, x1, x2);
@@ -773,6 +798,7 @@ __device__ Rec resolve_mols(const KP& P, const Geo& g, const TU* xuf, int u, int
             }
         }
     }
+    rec_complete(P, r);
     return r;
 }
 
@@ -1698,9 +1724,9 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
             if (mb <= g.cut2_hi) {
                 if (ms - mb > g.amb) {
                     bool w;
-                    double dd;
-                    if (decide_fp32(P, g.guard, mb, w, dd)) {
-                        if (w) { r.within = 1; r.i = mi; r.j = jbase + ma; r.d = dd; }
+                    int bn, bk;
+                    if (decide_fp32(P, g.guard, mb, w, bn, bk)) {
+                        if (w) { r.within = 1; r.i = mi; r.j = jbase + ma; r.d = mb; r.bin = bn; r.bulk = bk; }
                     } else {
                         double x0, x1, x2, y0, y1, y2;
                         load3(xuf, mi, x0, x1, x2);
@@ -1713,9 +1739,9 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
             if (rb <= g.cut2_hi) {
                 if (rsec - rb > g.amb) {
                     bool w;
-                    double dd;
-                    if (decide_fp32(P, g.guard, rb, w, dd)) {
-                        if (w) { r.ref = 1; r.dref = dd; }
+                    int bn, bk;
+                    if (decide_fp32(P, g.guard, rb, w, bn, bk)) {
+                        if (w) { r.ref = 1; r.dref = rb; r.rbin = bn; }
                     } else {
                         double x0, x1, x2, y0, y1, y2;
                         load3(xuf, ri, x0, x1, x2);
@@ -1789,6 +1815,7 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
             if (lane == L) r = loc;
         }
         if (leader) {
+            rec_complete(P, r);
             hist_record_global(P, gmd, grdf, gu, gv, r);
             if (!RANDOM && P.K > 0) B.bflag[(int64_t)f * P.kb * P.Mv + v] = inbulk(P, r) ? 1 : 0;
         }
