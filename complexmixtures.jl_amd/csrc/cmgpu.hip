@@ -984,6 +984,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
     const bool simple = nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3;
     constexpr int CU = 4;   // candidates per lane per step
     const int stride = gridDim.x * blockDim.x;
+    const unsigned long long lt = (1ull << lane) - 1ull;
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < P.Mv; base += stride) {
         const int q0 = base + lane;
         const bool act = q0 < P.Mv;
@@ -991,48 +992,60 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
         const float4 cv = ctx.vcen[v];
         const int vcell = ctx.vcellf[v];
         const int cc[3] = {vcell % nc[0], (vcell / nc[0]) % nc[1], vcell / (nc[0] * nc[1])};
-        int qn = 0;
         const int ncells = nd[0] * nd[1] * nd[2];
-        for (int o = 0; o < ncells; o++) {
-            // distinct neighbour cells: offsets -1..1, or 0..nc-1 when nc < 3
-            const int ox = o % nd[0], oy = (o / nd[0]) % nd[1], oz = o / (nd[0] * nd[1]);
-            int m0;
-            const int cx = nb_cell(cc[0], nc[0] >= 3 ? ox - 1 : ox, nc[0], m0);
-            const int cy = nb_cell(cc[1], nc[1] >= 3 ? oy - 1 : oy, nc[1], m0);
-            const int cz = nb_cell(cc[2], nc[2] >= 3 ? oz - 1 : oz, nc[2], m0);
-            const int cell = (cz * nc[1] + cy) * nc[0] + cx;
-            // the shifts under which this cell neighbours v's cell (one when
-            // every dimension has three or more cells)
-            int sx[3], sy[3], sz[3];
-            int kx = 1, ky = 1, kz = 1, code = -1;
-            float t1x = 0.f, t1y = 0.f, t1z = 0.f;
-            if (simple) {
-                int mx, my, mz;
-                nb_cell(cc[0], ox - 1, nc[0], mx);
-                nb_cell(cc[1], oy - 1, nc[1], my);
-                nb_cell(cc[2], oz - 1, nc[2], mz);
-                code = (mx + 1) + 3 * (my + 1) + 9 * (mz + 1);
-                shift_of(g, mx, my, mz, t1x, t1y, t1z);
-            } else {
-                kx = shifts_onto(cc[0], cx, nc[0], sx);
-                ky = shifts_onto(cc[1], cy, nc[1], sy);
-                kz = shifts_onto(cc[2], cz, nc[2], sz);
-            }
-            const int cs = st[cell];
-            const int cnt = act ? st[cell + 1] - cs : 0;
-            const int kmax = wave_max_i(cnt);
-            for (int k0 = 0; k0 < kmax; k0 += CU) {
+        // The walk over the neighbour cells produces queue entries until 64
+        // are waiting, the wavefront evaluates them, and the walk resumes where
+        // it stopped: the pair evaluation has one call site (one inlined copy).
+        int qn = 0;
+        int o = 0, kpos = 0, kmax = -1;
+        int cell = 0, cs = 0, cnt = 0, code = -1, kx = 1, ky = 1, kz = 1;
+        int sx[3], sy[3], sz[3];
+        float t1x = 0.f, t1y = 0.f, t1z = 0.f;
+        for (;;) {
+            while (qn < 64 && o < ncells) {
+                if (kmax < 0) {
+                    // enter neighbour cell o: offsets -1..1, or 0..nc-1 when nc < 3
+                    const int ox = o % nd[0], oy = (o / nd[0]) % nd[1], oz = o / (nd[0] * nd[1]);
+                    int m0;
+                    const int cx = nb_cell(cc[0], nc[0] >= 3 ? ox - 1 : ox, nc[0], m0);
+                    const int cy = nb_cell(cc[1], nc[1] >= 3 ? oy - 1 : oy, nc[1], m0);
+                    const int cz = nb_cell(cc[2], nc[2] >= 3 ? oz - 1 : oz, nc[2], m0);
+                    cell = (cz * nc[1] + cy) * nc[0] + cx;
+                    if (simple) {
+                        // three or more cells per dimension: one shift
+                        int mx, my, mz;
+                        nb_cell(cc[0], ox - 1, nc[0], mx);
+                        nb_cell(cc[1], oy - 1, nc[1], my);
+                        nb_cell(cc[2], oz - 1, nc[2], mz);
+                        code = (mx + 1) + 3 * (my + 1) + 9 * (mz + 1);
+                        shift_of(g, mx, my, mz, t1x, t1y, t1z);
+                    } else {
+                        kx = shifts_onto(cc[0], cx, nc[0], sx);
+                        ky = shifts_onto(cc[1], cy, nc[1], sy);
+                        kz = shifts_onto(cc[2], cz, nc[2], sz);
+                    }
+                    cs = st[cell];
+                    cnt = act ? st[cell + 1] - cs : 0;
+                    kmax = wave_max_i(cnt);
+                    kpos = 0;
+                }
+                if (kpos >= kmax) {
+                    o++;
+                    kmax = -1;
+                    continue;
+                }
                 float4 cu4[CU];
 #pragma unroll
-                for (int w = 0; w < CU; w++) cu4[w] = srt[min(cs + min(k0 + w, max(cnt - 1, 0)), P.Mu - 1)];
+                for (int w = 0; w < CU; w++) cu4[w] = srt[min(cs + min(kpos + w, max(cnt - 1, 0)), P.Mu - 1)];
+                int w = 0;
 #pragma unroll
-                for (int w = 0; w < CU; w++) {
-                    if (k0 + w >= kmax) break;
-                    const int u = __float_as_int(cu4[w].w);
-                    bool pass = k0 + w < cnt && (P.self_ ? u < v : true);
+                for (int ww = 0; ww < CU; ww++) {
+                    if (w != ww || kpos + ww >= kmax || qn >= 64) break;
+                    const int u = __float_as_int(cu4[ww].w);
+                    bool pass = kpos + ww < cnt && (P.self_ ? u < v : true);
                     if (simple) {
-                        const float dx = cv.x - (cu4[w].x + t1x), dy = cv.y - (cu4[w].y + t1y),
-                                    dz = cv.z - (cu4[w].z + t1z);
+                        const float dx = cv.x - (cu4[ww].x + t1x), dy = cv.y - (cu4[ww].y + t1y),
+                                    dz = cv.z - (cu4[ww].z + t1z);
                         pass = pass && dx * dx + dy * dy + dz * dz <= g.rpre2;
                     } else if (pass) {
                         bool nearby = false;
@@ -1041,17 +1054,28 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
                         for (int ix = 0; ix < kx; ix++) {
                             float tx, ty, tz;
                             shift_of(g, sx[ix], sy[iy], sz[iz], tx, ty, tz);
-                            const float dx = cv.x - (cu4[w].x + tx), dy = cv.y - (cu4[w].y + ty),
-                                        dz = cv.z - (cu4[w].z + tz);
+                            const float dx = cv.x - (cu4[ww].x + tx), dy = cv.y - (cu4[ww].y + ty),
+                                        dz = cv.z - (cu4[ww].z + tz);
                             nearby |= dx * dx + dy * dy + dz * dz <= g.rpre2;
                         }
                         pass = nearby;
                     }
-                    queue_push(ctx, q, qn, pass, make_int4(u, v, cell, code), lane);
+                    const unsigned long long m = __ballot(pass);
+                    if (pass) q[qn + __popcll(m & lt)] = make_int4(u, v, cell, code);
+                    qn += __popcll(m);
+                    w = ww + 1;
                 }
+                kpos += w;
             }
+            if (qn == 0) break;
+            const int ne = min(qn, 64);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            const bool has = lane < ne;
+            const int4 e = q[has ? qn - ne + lane : 0];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            qn -= ne;
+            if (has) ctx.eval_pair(e);
         }
-        queue_drain(ctx, q, qn, lane);
     }
     hist_flush(sh, P.nbins, B.cnt_md, B.cnt_rdf);
     if (shg) groups_flush(P, shg, B.cnt_gu, B.cnt_gv);
