@@ -838,7 +838,7 @@ __device__ __forceinline__ int shifts_onto(int cv, int cu, int n, int m[3])
 // Per-frame state of the molecule-pair kernels: records of molecule pairs
 // from their FP32 bests (resolve, histograms, bulk flags) and the evaluation
 // of one queued pair.
-template <class TU, class TV>
+template <class TU, class TV, int NU>
 struct PairCtx {
     const KP& P;
     const Bufs& B;
@@ -874,6 +874,100 @@ struct PairCtx {
                 B.bflag[((int64_t)f * P.kb + di[qq]) * P.Mv + vv] = inbulk(P, r) ? 1 : 0;
     }
 
+    // FP64 distance of solute atom ia of molecule uu and solvent atom ja of vv
+    __device__ double pair_d2(int uu, int ia, int vv, int ja) const
+    {
+        double x0, x1, x2, y0, y1, y2;
+        load3(xuf, (int64_t)uu * P.nu + ia, x0, x1, x2);
+        load3(xvf, (int64_t)vv * P.nv + ja, y0, y1, y2);
+        return cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
+    }
+
+    // main or reference record part of a unique FP32 argmin (ia of uu, ja of
+    // vv): within, 0-based bin and dbulk flag, from the FP32 interval or the
+    // FP64 distance of that pair
+    __device__ void unique_part(const Best& b, int uu, int vv, int ja, bool& within, int& bin, int& bulk) const
+    {
+        within = false;
+        if (!(b.best <= g.cut2_hi)) return;
+        bool w;
+        int bn, bk;
+        if (decide_fp32(P, g.guard, b.best, w, bn, bk)) {
+            within = w; bin = bn; bulk = bk;
+            return;
+        }
+        const double d2 = pair_d2(uu, b.bi, vv, ja);
+        if (d2 <= P.cut2) {
+            const double d = sqrt(d2);
+            within = true;
+            bin = (int)cm::setbin(d, P.binstep) - 1;
+            bulk = d > P.dbulk ? 1 : 0;
+        }
+    }
+
+    // Self mode: both orientations of the unordered pair {u, v}.  With a
+    // unique argmin both records share the distance, the atom pair and the
+    // bins (the FP64 distance is symmetric: the minimum image of -delta is
+    // the negated one, rounding included), so the md and default-group
+    // counts get +2 in one update each; the reference-atom parts belong to
+    // their orientation.  Ambiguous cases take the full per-orientation path.
+    __device__ void emit_self(int u, int v, const Best& bm, const Best& brv, const Best& bru) const
+    {
+        const bool amb = (bm.best <= g.cut2_hi && !(bm.second - bm.best > g.amb)) ||
+                         (brv.best <= g.cut2_hi && !(brv.second - brv.best > g.amb)) ||
+                         (bru.best <= g.cut2_hi && !(bru.second - bru.best > g.amb));
+        if (amb) {
+            // one call site for both orientations keeps the kernel small
+            for (int o = 0; o < 2; o++) {
+                Best b = bm;
+                if (o) { b.bi = bm.ba; b.ba = bm.bi; }
+                emit(o ? v : u, o ? u : v, b, o ? bru : brv);
+            }
+            return;
+        }
+        bool mw, rvw, ruw;
+        int mbin = 0, mbulk = 0, rvb = 0, rub = 0, dummy = 0;
+        unique_part(bm, u, v, bm.ba, mw, mbin, mbulk);
+        unique_part(brv, u, v, P.iref, rvw, rvb, dummy);   // (u, v): atoms of u vs the reference atom of v
+        unique_part(bru, v, u, P.iref, ruw, rub, dummy);   // (v, u): atoms of v vs the reference atom of u
+        if (mw && mbin < P.nbins) {
+            atomicAdd(&sh[mbin], 2u);
+            if (!P.gu_ptr && !P.gv_ptr) {
+                if (shg) {
+                    atomicAdd(&shg[(int64_t)bm.bi * P.nbins + mbin], 2u);
+                    atomicAdd(&shg[(int64_t)bm.ba * P.nbins + mbin], 2u);
+                } else {
+                    atomicAdd(&B.cnt_gu[(int64_t)bm.bi * P.nbins + mbin], 2ull);
+                    atomicAdd(&B.cnt_gu[(int64_t)bm.ba * P.nbins + mbin], 2ull);
+                }
+            } else {
+                // custom groups: the solute side is looked up with the
+                // molecule-local index, the solvent side with the global one
+                const int64_t ju = (int64_t)u * P.nv + bm.bi, jv = (int64_t)v * P.nv + bm.ba;
+                if (shg) {
+                    group_hit_lds(shg, P.nbins, mbin, bm.bi, P.gu_ptr, P.gu_idx, P.nu);
+                    group_hit_lds(shg, P.nbins, mbin, jv, P.gv_ptr, P.gv_idx, P.nu);
+                    group_hit_lds(shg, P.nbins, mbin, bm.ba, P.gu_ptr, P.gu_idx, P.nu);
+                    group_hit_lds(shg, P.nbins, mbin, ju, P.gv_ptr, P.gv_idx, P.nu);
+                } else {
+                    group_hit(B.cnt_gu, P.nbins, mbin, bm.bi, P.gu_ptr, P.gu_idx, P.nu);
+                    group_hit(B.cnt_gu, P.nbins, mbin, jv, P.gv_ptr, P.gv_idx, P.nu);
+                    group_hit(B.cnt_gu, P.nbins, mbin, bm.ba, P.gu_ptr, P.gu_idx, P.nu);
+                    group_hit(B.cnt_gu, P.nbins, mbin, ju, P.gv_ptr, P.gv_idx, P.nu);
+                }
+            }
+        }
+        // a reference-atom distance is never below the minimum distance, so
+        // a reference count implies the main one (as in hist_record)
+        if (mw && rvw && rvb < P.nbins) atomicAdd(&sh[P.nbins + rvb], 1u);
+        if (mw && ruw && rub < P.nbins) atomicAdd(&sh[P.nbins + rub], 1u);
+        if (P.K > 0 && mw) {
+            const int bulk = P.usecutoff ? mbulk : 0;
+            for (int qq = dp[u]; qq < dp[u + 1]; qq++) B.bflag[((int64_t)f * P.kb + di[qq]) * P.Mv + v] = bulk;
+            for (int qq = dp[v]; qq < dp[v + 1]; qq++) B.bflag[((int64_t)f * P.kb + di[qq]) * P.Mv + u] = bulk;
+        }
+    }
+
     // one queued pair {u, v, cell of u, shift code or -1}: every shift of the
     // pair, both orientations in self mode
     __device__ void eval_pair(int4 e) const
@@ -898,33 +992,58 @@ struct PairCtx {
         const float4 cuc = B.ucen[(int64_t)f * P.Mu + u];
         Best bm, brv, bru;
         bm.init(); brv.init(); bru.init();
-        for (int iz = 0; iz < kz; iz++)
-        for (int iy = 0; iy < ky; iy++)
-        for (int ix = 0; ix < kx; ix++) {
-            float tx, ty, tz;
-            shift_of(g, mx[ix], my[iy], mz[iz], tx, ty, tz);
-            const float dx = cv.x - (cuc.x + tx), dy = cv.y - (cuc.y + ty), dz = cv.z - (cuc.z + tz);
-            if (dx * dx + dy * dy + dz * dz > g.rpre2) continue;
-            for (int a = 0; a < P.nv; a++) {
-                const float4 y = yw[a];
-                const float yx = y.x - tx, yy = y.y - ty, yz = y.z - tz;
-                for (int i = 0; i < P.nu; i++) {
-                    const float4 s4 = xs[i];
-                    const float ex = yx - s4.x, ey = yy - s4.y, ez = yz - s4.z;
-                    const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-                    bm.offer(d2, i, a);
-                    if (a == P.iref) brv.offer(d2, i, a);
-                    if (P.self_ && i == P.iref) bru.offer(d2, a, i);
+        if constexpr (NU > 0) {
+            // self with NU-atom molecules: both molecules in registers, all
+            // loads in flight together
+            float4 X[NU], Y[NU];
+#pragma unroll
+            for (int i = 0; i < NU; i++) X[i] = xs[i];
+#pragma unroll
+            for (int a = 0; a < NU; a++) Y[a] = yw[a];
+            for (int iz = 0; iz < kz; iz++)
+            for (int iy = 0; iy < ky; iy++)
+            for (int ix = 0; ix < kx; ix++) {
+                float tx, ty, tz;
+                shift_of(g, mx[ix], my[iy], mz[iz], tx, ty, tz);
+                const float dx = cv.x - (cuc.x + tx), dy = cv.y - (cuc.y + ty), dz = cv.z - (cuc.z + tz);
+                if (dx * dx + dy * dy + dz * dz > g.rpre2) continue;
+#pragma unroll
+                for (int a = 0; a < NU; a++) {
+                    const float yx = Y[a].x - tx, yy = Y[a].y - ty, yz = Y[a].z - tz;
+#pragma unroll
+                    for (int i = 0; i < NU; i++) {
+                        const float ex = yx - X[i].x, ey = yy - X[i].y, ez = yz - X[i].z;
+                        const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+                        bm.offer(d2, i, a);
+                        if (a == P.iref) brv.offer(d2, i, a);
+                        if (i == P.iref) bru.offer(d2, a, i);
+                    }
+                }
+            }
+        } else {
+            for (int iz = 0; iz < kz; iz++)
+            for (int iy = 0; iy < ky; iy++)
+            for (int ix = 0; ix < kx; ix++) {
+                float tx, ty, tz;
+                shift_of(g, mx[ix], my[iy], mz[iz], tx, ty, tz);
+                const float dx = cv.x - (cuc.x + tx), dy = cv.y - (cuc.y + ty), dz = cv.z - (cuc.z + tz);
+                if (dx * dx + dy * dy + dz * dz > g.rpre2) continue;
+                for (int a = 0; a < P.nv; a++) {
+                    const float4 y = yw[a];
+                    const float yx = y.x - tx, yy = y.y - ty, yz = y.z - tz;
+                    for (int i = 0; i < P.nu; i++) {
+                        const float4 s4 = xs[i];
+                        const float ex = yx - s4.x, ey = yy - s4.y, ez = yz - s4.z;
+                        const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+                        bm.offer(d2, i, a);
+                        if (a == P.iref) brv.offer(d2, i, a);
+                        if (P.self_ && i == P.iref) bru.offer(d2, a, i);
+                    }
                 }
             }
         }
-        emit(u, v, bm, brv);
-        if (P.self_) {
-            Best bs = bm;
-            bs.bi = bm.ba;
-            bs.ba = bm.bi;
-            emit(v, u, bs, bru);
-        }
+        if (P.self_) emit_self(u, v, bm, brv, bru);
+        else emit(u, v, bm, brv);
     }
 };
 
@@ -965,8 +1084,8 @@ __device__ __forceinline__ void queue_drain(const CTX& ctx, int4* q, int qn, int
 // Self-MDDF evaluates each unordered pair once (u < v) and emits the records
 // of both orientations (same distance and argmin pair; reference atoms and tie
 // order per orientation).
-template <class TU, class TV>
-__global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU* xu, const TV* xv)
+template <class TU, class TV, int NU>
+__global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const TU* xu, const TV* xv)
 {
     extern __shared__ unsigned sh[];
     const int gwords = P.glds ? P.nbins * (P.gnu + (P.self_ ? 0 : P.gnv)) : 0;
@@ -975,7 +1094,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_real(KP P, Bufs B, const TU
     int4* q = (int4*)(sh + ((2 * P.nbins + gwords + 3) & ~3)) + (threadIdx.x >> 6) * QCAP;
     const int lane = threadIdx.x & 63;
     const int f = blockIdx.y;
-    const PairCtx<TU, TV> ctx(P, B, f, xu, xv, sh, shg);
+    const PairCtx<TU, TV, NU> ctx(P, B, f, xu, xv, sh, shg);
     const Geo& g = ctx.g;
     const float4* srt = B.sorted + (int64_t)f * B.nitems;
     const int* st = B.start + (int64_t)f * (P.cap + 1);
@@ -2202,8 +2321,18 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         const size_t shq = sizeof(unsigned) * ((2 * P.nbins + gwords + 3) & ~3) + sizeof(int4) * QCAP * (BLOCK / 64);
         const dim3 gr((unsigned)std::max<int64_t>(1, std::min<int64_t>((P.Mv + BLOCK - 1) / BLOCK, 4096 / F + 1)),
                       (unsigned)F);
-        if (f32) hipLaunchKernelGGL((k_eval_mols_real<float, float>), gr, dim3(BLOCK), shq, s, P, B, uf, vf);
-        else hipLaunchKernelGGL((k_eval_mols_real<double, double>), gr, dim3(BLOCK), shq, s, P, B, ud, vd);
+        // self-MDDF of 3- and 8-atom molecules (water, urea-like) keeps both
+        // molecules of a pair in registers
+        const int nfix = (P.self_ && P.nu == P.nv && (P.nu == 3 || P.nu == 8)) ? P.nu : 0;
+        if (f32) {
+            if (nfix == 3) hipLaunchKernelGGL((k_eval_mols_real<float, float, 3>), gr, dim3(BLOCK), shq, s, P, B, uf, vf);
+            else if (nfix == 8) hipLaunchKernelGGL((k_eval_mols_real<float, float, 8>), gr, dim3(BLOCK), shq, s, P, B, uf, vf);
+            else hipLaunchKernelGGL((k_eval_mols_real<float, float, 0>), gr, dim3(BLOCK), shq, s, P, B, uf, vf);
+        } else {
+            if (nfix == 3) hipLaunchKernelGGL((k_eval_mols_real<double, double, 3>), gr, dim3(BLOCK), shq, s, P, B, ud, vd);
+            else if (nfix == 8) hipLaunchKernelGGL((k_eval_mols_real<double, double, 8>), gr, dim3(BLOCK), shq, s, P, B, ud, vd);
+            else hipLaunchKernelGGL((k_eval_mols_real<double, double, 0>), gr, dim3(BLOCK), shq, s, P, B, ud, vd);
+        }
     }
     if (J->profiling) HIPCHK(hipEventRecord(ev[1], s));
     HIPCHK(hipGetLastError());
