@@ -804,6 +804,7 @@ __device__ Rec resolve_mols(const KP& P, const Geo& g, const TU* xuf, int u, int
 
 // LDS pair queue of the molecule-pair kernel: entries per wavefront
 constexpr int QCAP = 128;
+constexpr int DCAP = 128;   // deferred FP64 entries per wavefront (self)
 constexpr int GLDS_MAX = 32 * 1024;   // bytes of LDS group counts per block
 
 __device__ __forceinline__ int wave_max_i(int x)
@@ -874,103 +875,171 @@ struct PairCtx {
                 B.bflag[((int64_t)f * P.kb + di[qq]) * P.Mv + vv] = inbulk(P, r) ? 1 : 0;
     }
 
-    // FP64 distance of solute atom ia of molecule uu and solvent atom ja of vv
-    __device__ double pair_d2(int uu, int ia, int vv, int ja) const
+    // md (+2), group and dbulk-flag updates of both orientations of the self
+    // pair {u, v} whose minimum distance joins atom bi of u and atom ba of v,
+    // in 0-based bin mbin (mbulk: d > dbulk)
+    __device__ void self_main(int u, int v, int bi, int ba, int mbin, int mbulk) const
     {
-        double x0, x1, x2, y0, y1, y2;
-        load3(xuf, (int64_t)uu * P.nu + ia, x0, x1, x2);
-        load3(xvf, (int64_t)vv * P.nv + ja, y0, y1, y2);
-        return cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
-    }
-
-    // main or reference record part of a unique FP32 argmin (ia of uu, ja of
-    // vv): within, 0-based bin and dbulk flag, from the FP32 interval or the
-    // FP64 distance of that pair
-    __device__ void unique_part(const Best& b, int uu, int vv, int ja, bool& within, int& bin, int& bulk) const
-    {
-        within = false;
-        if (!(b.best <= g.cut2_hi)) return;
-        bool w;
-        int bn, bk;
-        if (decide_fp32(P, g.guard, b.best, w, bn, bk)) {
-            within = w; bin = bn; bulk = bk;
-            return;
-        }
-        const double d2 = pair_d2(uu, b.bi, vv, ja);
-        if (d2 <= P.cut2) {
-            const double d = sqrt(d2);
-            within = true;
-            bin = (int)cm::setbin(d, P.binstep) - 1;
-            bulk = d > P.dbulk ? 1 : 0;
-        }
-    }
-
-    // Self mode: both orientations of the unordered pair {u, v}.  With a
-    // unique argmin both records share the distance, the atom pair and the
-    // bins (the FP64 distance is symmetric: the minimum image of -delta is
-    // the negated one, rounding included), so the md and default-group
-    // counts get +2 in one update each; the reference-atom parts belong to
-    // their orientation.  Ambiguous cases take the full per-orientation path.
-    __device__ void emit_self(int u, int v, const Best& bm, const Best& brv, const Best& bru) const
-    {
-        const bool amb = (bm.best <= g.cut2_hi && !(bm.second - bm.best > g.amb)) ||
-                         (brv.best <= g.cut2_hi && !(brv.second - brv.best > g.amb)) ||
-                         (bru.best <= g.cut2_hi && !(bru.second - bru.best > g.amb));
-        if (amb) {
-            // one call site for both orientations keeps the kernel small
-            for (int o = 0; o < 2; o++) {
-                Best b = bm;
-                if (o) { b.bi = bm.ba; b.ba = bm.bi; }
-                emit(o ? v : u, o ? u : v, b, o ? bru : brv);
-            }
-            return;
-        }
-        bool mw, rvw, ruw;
-        int mbin = 0, mbulk = 0, rvb = 0, rub = 0, dummy = 0;
-        unique_part(bm, u, v, bm.ba, mw, mbin, mbulk);
-        unique_part(brv, u, v, P.iref, rvw, rvb, dummy);   // (u, v): atoms of u vs the reference atom of v
-        unique_part(bru, v, u, P.iref, ruw, rub, dummy);   // (v, u): atoms of v vs the reference atom of u
-        if (mw && mbin < P.nbins) {
-            atomicAdd(&sh[mbin], 2u);
-            if (!P.gu_ptr && !P.gv_ptr) {
-                if (shg) {
-                    atomicAdd(&shg[(int64_t)bm.bi * P.nbins + mbin], 2u);
-                    atomicAdd(&shg[(int64_t)bm.ba * P.nbins + mbin], 2u);
-                } else {
-                    atomicAdd(&B.cnt_gu[(int64_t)bm.bi * P.nbins + mbin], 2ull);
-                    atomicAdd(&B.cnt_gu[(int64_t)bm.ba * P.nbins + mbin], 2ull);
-                }
+        if (mbin >= P.nbins) return;   // unreachable: d <= cutoff
+        atomicAdd(&sh[mbin], 2u);
+        if (!P.gu_ptr && !P.gv_ptr) {
+            if (shg) {
+                atomicAdd(&shg[(int64_t)bi * P.nbins + mbin], 2u);
+                atomicAdd(&shg[(int64_t)ba * P.nbins + mbin], 2u);
             } else {
-                // custom groups: the solute side is looked up with the
-                // molecule-local index, the solvent side with the global one
-                const int64_t ju = (int64_t)u * P.nv + bm.bi, jv = (int64_t)v * P.nv + bm.ba;
-                if (shg) {
-                    group_hit_lds(shg, P.nbins, mbin, bm.bi, P.gu_ptr, P.gu_idx, P.nu);
-                    group_hit_lds(shg, P.nbins, mbin, jv, P.gv_ptr, P.gv_idx, P.nu);
-                    group_hit_lds(shg, P.nbins, mbin, bm.ba, P.gu_ptr, P.gu_idx, P.nu);
-                    group_hit_lds(shg, P.nbins, mbin, ju, P.gv_ptr, P.gv_idx, P.nu);
-                } else {
-                    group_hit(B.cnt_gu, P.nbins, mbin, bm.bi, P.gu_ptr, P.gu_idx, P.nu);
-                    group_hit(B.cnt_gu, P.nbins, mbin, jv, P.gv_ptr, P.gv_idx, P.nu);
-                    group_hit(B.cnt_gu, P.nbins, mbin, bm.ba, P.gu_ptr, P.gu_idx, P.nu);
-                    group_hit(B.cnt_gu, P.nbins, mbin, ju, P.gv_ptr, P.gv_idx, P.nu);
-                }
+                atomicAdd(&B.cnt_gu[(int64_t)bi * P.nbins + mbin], 2ull);
+                atomicAdd(&B.cnt_gu[(int64_t)ba * P.nbins + mbin], 2ull);
+            }
+        } else {
+            // custom groups: the solute side is looked up with the
+            // molecule-local index, the solvent side with the global one
+            const int64_t ju = (int64_t)u * P.nv + bi, jv = (int64_t)v * P.nv + ba;
+            if (shg) {
+                group_hit_lds(shg, P.nbins, mbin, bi, P.gu_ptr, P.gu_idx, P.nu);
+                group_hit_lds(shg, P.nbins, mbin, jv, P.gv_ptr, P.gv_idx, P.nu);
+                group_hit_lds(shg, P.nbins, mbin, ba, P.gu_ptr, P.gu_idx, P.nu);
+                group_hit_lds(shg, P.nbins, mbin, ju, P.gv_ptr, P.gv_idx, P.nu);
+            } else {
+                group_hit(B.cnt_gu, P.nbins, mbin, bi, P.gu_ptr, P.gu_idx, P.nu);
+                group_hit(B.cnt_gu, P.nbins, mbin, jv, P.gv_ptr, P.gv_idx, P.nu);
+                group_hit(B.cnt_gu, P.nbins, mbin, ba, P.gu_ptr, P.gu_idx, P.nu);
+                group_hit(B.cnt_gu, P.nbins, mbin, ju, P.gv_ptr, P.gv_idx, P.nu);
             }
         }
-        // a reference-atom distance is never below the minimum distance, so
-        // a reference count implies the main one (as in hist_record)
-        if (mw && rvw && rvb < P.nbins) atomicAdd(&sh[P.nbins + rvb], 1u);
-        if (mw && ruw && rub < P.nbins) atomicAdd(&sh[P.nbins + rub], 1u);
-        if (P.K > 0 && mw) {
+        if (P.K > 0) {
             const int bulk = P.usecutoff ? mbulk : 0;
             for (int qq = dp[u]; qq < dp[u + 1]; qq++) B.bflag[((int64_t)f * P.kb + di[qq]) * P.Mv + v] = bulk;
             for (int qq = dp[v]; qq < dp[v + 1]; qq++) B.bflag[((int64_t)f * P.kb + di[qq]) * P.Mv + u] = bulk;
         }
     }
 
+    __device__ void rdf_hit(int bin) const
+    {
+        if (bin < P.nbins) atomicAdd(&sh[P.nbins + bin], 1u);
+    }
+
+    // Self mode, FP32 stage of the unordered pair {u, v}: both orientations
+    // share the distance, the atom pair and the bins of the minimum (the FP64
+    // distance is symmetric: the minimum image of -delta is the negated one,
+    // rounding included), so md and default groups get +2 in one update; the
+    // reference-atom parts belong to their orientation (a reference distance
+    // is never below the minimum, so its count needs no test of the main
+    // part).  Parts the FP32 interval cannot decide are returned as a mask
+    // for the deferred FP64 stage: 1 the minimum, 2 the reference atom of v,
+    // 4 the reference atom of u, 8 the whole pair exhaustively (an argmin
+    // that FP32 cannot separate).  d0, d1 carry the pair and its argmins.
+    __device__ int emit_self(int u, int v, const Best& bm, const Best& brv, const Best& bru, int4& d0, int4& d1) const
+    {
+        d0 = make_int4(u, v, bm.bi, bm.ba);
+        d1 = make_int4(brv.bi, bru.bi, 0, 0);
+        const bool amb = (bm.best <= g.cut2_hi && !(bm.second - bm.best > g.amb)) ||
+                         (brv.best <= g.cut2_hi && !(brv.second - brv.best > g.amb)) ||
+                         (bru.best <= g.cut2_hi && !(bru.second - bru.best > g.amb));
+        if (amb) return 8;
+        int mask = 0;
+        bool w;
+        int bn, bk;
+        if (bm.best <= g.cut2_hi) {
+            if (decide_fp32(P, g.guard, bm.best, w, bn, bk)) {
+                if (w) self_main(u, v, bm.bi, bm.ba, bn, bk);
+            } else mask |= 1;
+        }
+        if (brv.best <= g.cut2_hi) {
+            if (decide_fp32(P, g.guard, brv.best, w, bn, bk)) {
+                if (w) rdf_hit(bn);
+            } else mask |= 2;
+        }
+        if (bru.best <= g.cut2_hi) {
+            if (decide_fp32(P, g.guard, bru.best, w, bn, bk)) {
+                if (w) rdf_hit(bn);
+            } else mask |= 4;
+        }
+        return mask;
+    }
+
+    // Self mode, deferred FP64 stage of one pair (mask and entry from
+    // emit_self).  The three distances are formed together so their loads
+    // are in flight at once; parts outside the mask are discarded.
+    __device__ void self_deferred(int4 d0, int4 d1, int mask) const
+    {
+        const int u = d0.x, v = d0.y;
+        if (mask & 8) {
+            self_exhaustive(u, v);
+            return;
+        }
+        const int64_t ub = (int64_t)u * P.nu, vb = (int64_t)v * P.nv;
+        double a0, a1, a2, b0, b1, b2, c0, c1, c2, e0, e1, e2, p0, p1, p2, q0, q1, q2;
+        load3(xuf, ub + max(d0.z, 0), a0, a1, a2);
+        load3(xvf, vb + max(d0.w, 0), b0, b1, b2);
+        load3(xuf, ub + max(d1.x, 0), c0, c1, c2);
+        load3(xvf, vb + P.iref, e0, e1, e2);
+        load3(xuf, (int64_t)v * P.nu + max(d1.y, 0), p0, p1, p2);
+        load3(xvf, (int64_t)u * P.nv + P.iref, q0, q1, q2);
+        const double dm = cm::canon_d2(g.cell, a0, a1, a2, b0, b1, b2);
+        const double dr = cm::canon_d2(g.cell, c0, c1, c2, e0, e1, e2);   // atoms of u vs the reference atom of v
+        const double du = cm::canon_d2(g.cell, p0, p1, p2, q0, q1, q2);   // atoms of v vs the reference atom of u
+        if ((mask & 1) && dm <= P.cut2) {
+            const double d = sqrt(dm);
+            self_main(u, v, d0.z, d0.w, (int)cm::setbin(d, P.binstep) - 1, d > P.dbulk ? 1 : 0);
+        }
+        if ((mask & 2) && dr <= P.cut2) rdf_hit((int)cm::setbin(sqrt(dr), P.binstep) - 1);
+        if ((mask & 4) && du <= P.cut2) rdf_hit((int)cm::setbin(sqrt(du), P.binstep) - 1);
+    }
+
+    // both orientations of a self pair from one set of FP64 distances: the
+    // records of (u, v) and (v, u) with their own tie order (update_md,
+    // src/minimum_distances.jl:30-39) and reference atoms
+    __device__ void self_exhaustive(int u, int v) const
+    {
+        atomicAdd(&B.stats[0], 2ull);
+        Rec r1 = rec_zero(), r2 = rec_zero();
+        const int64_t ub = (int64_t)u * P.nu, vb = (int64_t)v * P.nv;
+        const int n = NU > 0 ? NU : P.nu;
+        auto offer = [&](int i, int a, double x0, double x1, double x2, double y0, double y1, double y2) {
+            const double d2 = cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2);
+            if (d2 <= P.cut2) {
+                const double d = sqrt(d2);
+                rec_offer(r1, i, vb + a, d, a == P.iref);
+                rec_offer(r2, a, ub + i, d, i == P.iref);
+            }
+        };
+#pragma unroll 1
+        for (int a = 0; a < n; a++) {
+            double y0, y1, y2;
+            load3(xvf, vb + a, y0, y1, y2);
+            if constexpr (NU > 0 && NU <= 4) {
+#pragma unroll
+                for (int i = 0; i < NU; i++) {
+                    double x0, x1, x2;
+                    load3(xuf, ub + i, x0, x1, x2);
+                    offer(i, a, x0, x1, x2, y0, y1, y2);
+                }
+            } else {
+#pragma unroll 1
+                for (int i = 0; i < n; i++) {
+                    double x0, x1, x2;
+                    load3(xuf, ub + i, x0, x1, x2);
+                    offer(i, a, x0, x1, x2, y0, y1, y2);
+                }
+            }
+        }
+        rec_complete(P, r1);
+        rec_complete(P, r2);
+        hist_record_mols(P, sh, shg, B.cnt_gu, B.cnt_gv, r1);
+        hist_record_mols(P, sh, shg, B.cnt_gu, B.cnt_gv, r2);
+        if (P.K > 0) {
+            if (r1.within)
+                for (int qq = dp[u]; qq < dp[u + 1]; qq++)
+                    B.bflag[((int64_t)f * P.kb + di[qq]) * P.Mv + v] = inbulk(P, r1) ? 1 : 0;
+            if (r2.within)
+                for (int qq = dp[v]; qq < dp[v + 1]; qq++)
+                    B.bflag[((int64_t)f * P.kb + di[qq]) * P.Mv + u] = inbulk(P, r2) ? 1 : 0;
+        }
+    }
+
     // one queued pair {u, v, cell of u, shift code or -1}: every shift of the
     // pair, both orientations in self mode
-    __device__ void eval_pair(int4 e) const
+    __device__ int eval_pair(int4 e, int4& d0, int4& d1) const
     {
         const int u = e.x, v = e.y, cu = e.z;
         const int nc0 = g.nc[0], nc1 = g.nc[1], nc2 = g.nc[2];
@@ -1042,36 +1111,11 @@ struct PairCtx {
                 }
             }
         }
-        if (P.self_) emit_self(u, v, bm, brv, bru);
-        else emit(u, v, bm, brv);
+        if (P.self_) return emit_self(u, v, bm, brv, bru, d0, d1);
+        emit(u, v, bm, brv);
+        return 0;
     }
 };
-
-// appends the passing candidates of this step to the wavefront's LDS queue
-// and evaluates 64 queued pairs (one per lane) whenever 64 are waiting
-template <class CTX>
-__device__ __forceinline__ void queue_push(const CTX& ctx, int4* q, int& qn, bool pass, int4 ent, int lane)
-{
-    const unsigned long long lt = (1ull << lane) - 1ull;
-    const unsigned long long m = __ballot(pass);
-    if (pass) q[qn + __popcll(m & lt)] = ent;
-    qn += __popcll(m);
-    if (qn >= 64) {
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const int4 e = q[qn - 64 + lane];
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        qn -= 64;
-        ctx.eval_pair(e);
-    }
-}
-
-template <class CTX>
-__device__ __forceinline__ void queue_drain(const CTX& ctx, int4* q, int qn, int lane)
-{
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (lane < qn) ctx.eval_pair(q[lane]);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-}
 
 // Real pass of the many-solute-molecule family.  blockIdx.y is the frame;
 // each lane owns one solvent molecule v (self: in cell order).  The lanes walk
@@ -1092,6 +1136,9 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
     hist_clear(sh, 2 * P.nbins + gwords);
     unsigned* shg = P.glds ? sh + 2 * P.nbins : nullptr;
     int4* q = (int4*)(sh + ((2 * P.nbins + gwords + 3) & ~3)) + (threadIdx.x >> 6) * QCAP;
+    // self: deferred FP64 entries (two int4 each) after the pair queues
+    int4* dq = (int4*)(sh + ((2 * P.nbins + gwords + 3) & ~3)) + (BLOCK / 64) * QCAP + (threadIdx.x >> 6) * 2 * DCAP;
+    int dqn = 0;
     const int lane = threadIdx.x & 63;
     const int f = blockIdx.y;
     const PairCtx<TU, TV, NU> ctx(P, B, f, xu, xv, sh, shg);
@@ -1101,7 +1148,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
     const int nc[3] = {g.nc[0], g.nc[1], g.nc[2]};
     const int nd[3] = {min(nc[0], 3), min(nc[1], 3), min(nc[2], 3)};
     const bool simple = nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3;
-    constexpr int CU = 4;   // candidates per lane per step
+    constexpr int CU = 8;   // candidates per lane per step
     const int stride = gridDim.x * blockDim.x;
     const unsigned long long lt = (1ull << lane) - 1ull;
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < P.Mv; base += stride) {
@@ -1116,7 +1163,13 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
         // are waiting, the wavefront evaluates them, and the walk resumes where
         // it stopped: the pair evaluation has one call site (one inlined copy).
         int qn = 0;
-        int o = 0, kpos = 0, kmax = -1;
+        // Self with three or more cells per dimension: every offset maps to a
+        // distinct cell, so each unordered pair of molecules in different
+        // cells is met exactly once from the half shell of offsets after
+        // (0, 0, 0) (linear index 13 of the 27; z, then y, then x), and pairs
+        // inside one cell once with u < v.  Otherwise all offsets with u < v.
+        const int o_first = (P.self_ && simple) ? 13 : 0;
+        int o = o_first, kpos = 0, kmax = -1;
         int cell = 0, cs = 0, cnt = 0, code = -1, kx = 1, ky = 1, kz = 1;
         int sx[3], sy[3], sz[3];
         float t1x = 0.f, t1y = 0.f, t1z = 0.f;
@@ -1161,7 +1214,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
                 for (int ww = 0; ww < CU; ww++) {
                     if (w != ww || kpos + ww >= kmax || qn >= 64) break;
                     const int u = __float_as_int(cu4[ww].w);
-                    bool pass = kpos + ww < cnt && (P.self_ ? u < v : true);
+                    bool pass = kpos + ww < cnt && (P.self_ ? ((simple && o > 13) || u < v) : true);
                     if (simple) {
                         const float dx = cv.x - (cu4[ww].x + t1x), dy = cv.y - (cu4[ww].y + t1y),
                                     dz = cv.z - (cu4[ww].z + t1z);
@@ -1193,7 +1246,36 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
             const int4 e = q[has ? qn - ne + lane : 0];
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             qn -= ne;
-            if (has) ctx.eval_pair(e);
+            int4 d0 = make_int4(0, 0, 0, 0), d1 = d0;
+            int dm = 0;
+            if (has) dm = ctx.eval_pair(e, d0, d1);
+            if (P.self_) {
+                // parts FP32 could not decide wait in the deferred queue and
+                // are resolved 64 at a time, so the FP64 work runs on full
+                // wavefronts
+                const unsigned long long m = __ballot(dm != 0);
+                if (dm != 0) {
+                    const int k = dqn + __popcll(m & lt);
+                    dq[2 * k] = d0;
+                    dq[2 * k + 1] = make_int4(d1.x, d1.y, dm, 0);
+                }
+                dqn += __popcll(m);
+                if (dqn >= 64) {
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    const int k = dqn - 64 + lane;
+                    const int4 x0 = dq[2 * k], x1 = dq[2 * k + 1];
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    dqn -= 64;
+                    ctx.self_deferred(x0, x1, x1.z);
+                }
+            }
+        }
+    }
+    if (dqn > 0) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (lane < dqn) {
+            const int4 x0 = dq[2 * lane], x1 = dq[2 * lane + 1];
+            ctx.self_deferred(x0, x1, x1.z);
         }
     }
     hist_flush(sh, P.nbins, B.cnt_md, B.cnt_rdf);
@@ -2318,7 +2400,8 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
     } else {
         int64_t n = (int64_t)F * P.Mv;
         const int gwords = P.glds ? P.nbins * (P.gnu + (P.self_ ? 0 : P.gnv)) : 0;
-        const size_t shq = sizeof(unsigned) * ((2 * P.nbins + gwords + 3) & ~3) + sizeof(int4) * QCAP * (BLOCK / 64);
+        const size_t shq = sizeof(unsigned) * ((2 * P.nbins + gwords + 3) & ~3) + sizeof(int4) * QCAP * (BLOCK / 64) +
+                           (P.self_ ? sizeof(int4) * 2 * DCAP * (BLOCK / 64) : 0);
         const dim3 gr((unsigned)std::max<int64_t>(1, std::min<int64_t>((P.Mv + BLOCK - 1) / BLOCK, 4096 / F + 1)),
                       (unsigned)F);
         // self-MDDF of 3- and 8-atom molecules (water, urea-like) keeps both
