@@ -94,6 +94,9 @@ struct KP {                  // job parameters, by value
     int Mu, nu, Mv, nv;
     int self_, iref, nbins, usecutoff;
     double binstep, cutsys, cut2, dbulk;
+    double e2lo, e2hi;       // binstep^2 (1 + 1e-12), binstep^2 (1 - 1e-12): squared bin edges with margins
+    double db2lo, db2hi;     // dbulk^2 (1 - 1e-12), dbulk^2 (1 + 1e-12)
+    float inv_binstep;       // 1 / binstep in FP32 (bin estimates only)
     int K;                   // random passes per frame (n_random_samples, 0 for coordination numbers)
     int kb;                  // bulk lists per frame (1 in the atoms family, K in the mols family)
     uint32_t key0, key1;
@@ -620,23 +623,25 @@ __device__ __forceinline__ bool inbulk(const KP& P, const Rec& r)
 // bin and dbulk test.  The bin of d2 is max(1, ceil(fl(sqrt(d2)) / binstep));
 // between ((k-1) binstep)^2 (1 + 1e-12) and (k binstep)^2 (1 - 1e-12) it is k
 // whatever the FP64 rounding of the square root, the quotient and the edges
-// (all below 1e-15 relative), and likewise d > dbulk is decided outside
-// dbulk^2 (1 -+ 1e-12).  Sets within, the 0-based bin and the dbulk flag;
+// (all below 1e-15 relative; the edges are formed as k^2 * binstep^2 (1 -+
+// 1e-12), a few ulp from the exact products), and likewise d > dbulk is
+// decided outside dbulk^2 (1 -+ 1e-12).  Sets within, the 0-based bin and the dbulk flag;
 // returns false when the interval straddles a decision.
 __device__ __forceinline__ bool decide_fp32(const KP& P, double G, float b, bool& within, int& bin, int& bulk)
 {
     const double lo = fmax((double)b - G, 0.0), hi = (double)b + G;
     if (lo > P.cut2) { within = false; return true; }
     if (!(hi <= P.cut2)) return false;
-    int k = (int)ceilf(sqrtf(b) / (float)P.binstep);
+    // bin estimate from the hardware square root: any k is safe, the edge
+    // tests accept it only when it is the bin of every d2 in [lo, hi]
+    int k = (int)ceilf(__builtin_amdgcn_sqrtf(b) * P.inv_binstep);
     k = max(k, 1);
-    const double e_hi = (double)k * P.binstep, e_lo = (double)(k - 1) * P.binstep;
-    if (!(hi < e_hi * e_hi * (1.0 - 1e-12))) return false;
-    if (k > 1 && !(lo > e_lo * e_lo * (1.0 + 1e-12))) return false;
-    const double db2 = P.dbulk * P.dbulk;
+    const double kd = (double)k, km = kd - 1.0;
+    if (!(hi < kd * kd * P.e2hi)) return false;
+    if (k > 1 && !(lo > km * km * P.e2lo)) return false;
     if (!P.usecutoff) bulk = 0;   // the dbulk test only matters with usecutoff
-    else if (lo > db2 * (1.0 + 1e-12)) bulk = 1;
-    else if (hi < db2 * (1.0 - 1e-12)) bulk = 0;
+    else if (lo > P.db2hi) bulk = 1;
+    else if (hi < P.db2lo) bulk = 0;
     else return false;
     within = true;
     bin = k - 1;
@@ -716,7 +721,25 @@ __device__ __forceinline__ float wave_max(float x)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
 }
 
-__device__ __forceinline__ void shift_of(const Geo& g, int mx, int my, int mz, float& tx, float& ty, float& tz)
+// The frame fields the pair kernels read on every pair, copied into registers
+// once: read through the Geo reference they would be reloaded from memory after
+// every global store or atomic (which may alias the geometry array).
+struct GeoHot {
+    float H32[9];
+    float cut2_hi, amb, rpre2;
+    double guard;
+    int nc[3];
+    __device__ explicit GeoHot(const Geo& g)
+        : cut2_hi(g.cut2_hi), amb(g.amb), rpre2(g.rpre2), guard(g.guard)
+    {
+#pragma unroll
+        for (int k = 0; k < 9; k++) H32[k] = g.H32[k];
+        nc[0] = g.nc[0]; nc[1] = g.nc[1]; nc[2] = g.nc[2];
+    }
+};
+
+template <class GEO>
+__device__ __forceinline__ void shift_of(const GEO& g, int mx, int my, int mz, float& tx, float& ty, float& tz)
 {
     float fx = (float)mx, fy = (float)my, fz = (float)mz;
     tx = g.H32[0] * fx + g.H32[3] * fy + g.H32[6] * fz;
@@ -844,6 +867,7 @@ struct PairCtx {
     const KP& P;
     const Bufs& B;
     const Geo& g;
+    const GeoHot gh;
     int f;
     const float4* solw;
     const float4* vwf;
@@ -857,7 +881,7 @@ struct PairCtx {
     unsigned* shg;
 
     __device__ PairCtx(const KP& P_, const Bufs& B_, int f_, const TU* xu, const TV* xv, unsigned* sh_, unsigned* shg_)
-        : P(P_), B(B_), g(B_.geo[f_]), f(f_), solw(B_.solw + (int64_t)f_ * P_.Nu), vwf(B_.vw + (int64_t)f_ * P_.Nv),
+        : P(P_), B(B_), g(B_.geo[f_]), gh(B_.geo[f_]), f(f_), solw(B_.solw + (int64_t)f_ * P_.Nu), vwf(B_.vw + (int64_t)f_ * P_.Nv),
           vcen(B_.vcen + (int64_t)f_ * P_.Mv), vcellf(B_.vcell + (int64_t)f_ * P_.Mv), xuf(xu + (int64_t)f_ * P_.Nu * 3),
           xvf(xv + (int64_t)f_ * P_.Nv * 3), dp(B_.drawn_ptr + (int64_t)f_ * (P_.Mu + 1)), di(B_.drawn_idx + (int64_t)f_ * P_.K),
           sh(sh_), shg(shg_) {}
@@ -932,25 +956,25 @@ struct PairCtx {
     {
         d0 = make_int4(u, v, bm.bi, bm.ba);
         d1 = make_int4(brv.bi, bru.bi, 0, 0);
-        const bool amb = (bm.best <= g.cut2_hi && !(bm.second - bm.best > g.amb)) ||
-                         (brv.best <= g.cut2_hi && !(brv.second - brv.best > g.amb)) ||
-                         (bru.best <= g.cut2_hi && !(bru.second - bru.best > g.amb));
+        const bool amb = (bm.best <= gh.cut2_hi && !(bm.second - bm.best > gh.amb)) ||
+                         (brv.best <= gh.cut2_hi && !(brv.second - brv.best > gh.amb)) ||
+                         (bru.best <= gh.cut2_hi && !(bru.second - bru.best > gh.amb));
         if (amb) return 8;
         int mask = 0;
         bool w;
         int bn, bk;
-        if (bm.best <= g.cut2_hi) {
-            if (decide_fp32(P, g.guard, bm.best, w, bn, bk)) {
+        if (bm.best <= gh.cut2_hi) {
+            if (decide_fp32(P, gh.guard, bm.best, w, bn, bk)) {
                 if (w) self_main(u, v, bm.bi, bm.ba, bn, bk);
             } else mask |= 1;
         }
-        if (brv.best <= g.cut2_hi) {
-            if (decide_fp32(P, g.guard, brv.best, w, bn, bk)) {
+        if (brv.best <= gh.cut2_hi) {
+            if (decide_fp32(P, gh.guard, brv.best, w, bn, bk)) {
                 if (w) rdf_hit(bn);
             } else mask |= 2;
         }
-        if (bru.best <= g.cut2_hi) {
-            if (decide_fp32(P, g.guard, bru.best, w, bn, bk)) {
+        if (bru.best <= gh.cut2_hi) {
+            if (decide_fp32(P, gh.guard, bru.best, w, bn, bk)) {
                 if (w) rdf_hit(bn);
             } else mask |= 4;
         }
@@ -1042,7 +1066,7 @@ struct PairCtx {
     __device__ int eval_pair(int4 e, int4& d0, int4& d1) const
     {
         const int u = e.x, v = e.y, cu = e.z;
-        const int nc0 = g.nc[0], nc1 = g.nc[1], nc2 = g.nc[2];
+        const int nc0 = gh.nc[0], nc1 = gh.nc[1], nc2 = gh.nc[2];
         int mx[3], my[3], mz[3];
         int kx = 1, ky = 1, kz = 1;
         if (e.w >= 0) {
@@ -1073,9 +1097,9 @@ struct PairCtx {
             for (int iy = 0; iy < ky; iy++)
             for (int ix = 0; ix < kx; ix++) {
                 float tx, ty, tz;
-                shift_of(g, mx[ix], my[iy], mz[iz], tx, ty, tz);
+                shift_of(gh, mx[ix], my[iy], mz[iz], tx, ty, tz);
                 const float dx = cv.x - (cuc.x + tx), dy = cv.y - (cuc.y + ty), dz = cv.z - (cuc.z + tz);
-                if (dx * dx + dy * dy + dz * dz > g.rpre2) continue;
+                if (dx * dx + dy * dy + dz * dz > gh.rpre2) continue;
 #pragma unroll
                 for (int a = 0; a < NU; a++) {
                     const float yx = Y[a].x - tx, yy = Y[a].y - ty, yz = Y[a].z - tz;
@@ -1094,9 +1118,9 @@ struct PairCtx {
             for (int iy = 0; iy < ky; iy++)
             for (int ix = 0; ix < kx; ix++) {
                 float tx, ty, tz;
-                shift_of(g, mx[ix], my[iy], mz[iz], tx, ty, tz);
+                shift_of(gh, mx[ix], my[iy], mz[iz], tx, ty, tz);
                 const float dx = cv.x - (cuc.x + tx), dy = cv.y - (cuc.y + ty), dz = cv.z - (cuc.z + tz);
-                if (dx * dx + dy * dy + dz * dz > g.rpre2) continue;
+                if (dx * dx + dy * dy + dz * dz > gh.rpre2) continue;
                 for (int a = 0; a < P.nv; a++) {
                     const float4 y = yw[a];
                     const float yx = y.x - tx, yy = y.y - ty, yz = y.z - tz;
@@ -1142,10 +1166,10 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
     const int lane = threadIdx.x & 63;
     const int f = blockIdx.y;
     const PairCtx<TU, TV, NU> ctx(P, B, f, xu, xv, sh, shg);
-    const Geo& g = ctx.g;
+    const GeoHot& gh = ctx.gh;
     const float4* srt = B.sorted + (int64_t)f * B.nitems;
     const int* st = B.start + (int64_t)f * (P.cap + 1);
-    const int nc[3] = {g.nc[0], g.nc[1], g.nc[2]};
+    const int nc[3] = {gh.nc[0], gh.nc[1], gh.nc[2]};
     const int nd[3] = {min(nc[0], 3), min(nc[1], 3), min(nc[2], 3)};
     const bool simple = nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3;
     constexpr int CU = 8;   // candidates per lane per step
@@ -1190,7 +1214,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
                         nb_cell(cc[1], oy - 1, nc[1], my);
                         nb_cell(cc[2], oz - 1, nc[2], mz);
                         code = (mx + 1) + 3 * (my + 1) + 9 * (mz + 1);
-                        shift_of(g, mx, my, mz, t1x, t1y, t1z);
+                        shift_of(gh, mx, my, mz, t1x, t1y, t1z);
                     } else {
                         kx = shifts_onto(cc[0], cx, nc[0], sx);
                         ky = shifts_onto(cc[1], cy, nc[1], sy);
@@ -1218,17 +1242,17 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
                     if (simple) {
                         const float dx = cv.x - (cu4[ww].x + t1x), dy = cv.y - (cu4[ww].y + t1y),
                                     dz = cv.z - (cu4[ww].z + t1z);
-                        pass = pass && dx * dx + dy * dy + dz * dz <= g.rpre2;
+                        pass = pass && dx * dx + dy * dy + dz * dz <= gh.rpre2;
                     } else if (pass) {
                         bool nearby = false;
                         for (int iz = 0; iz < kz; iz++)
                         for (int iy = 0; iy < ky; iy++)
                         for (int ix = 0; ix < kx; ix++) {
                             float tx, ty, tz;
-                            shift_of(g, sx[ix], sy[iy], sz[iz], tx, ty, tz);
+                            shift_of(gh, sx[ix], sy[iy], sz[iz], tx, ty, tz);
                             const float dx = cv.x - (cu4[ww].x + tx), dy = cv.y - (cu4[ww].y + ty),
                                         dz = cv.z - (cu4[ww].z + tz);
-                            nearby |= dx * dx + dy * dy + dz * dz <= g.rpre2;
+                            nearby |= dx * dx + dy * dy + dz * dz <= gh.rpre2;
                         }
                         pass = nearby;
                     }
@@ -2694,6 +2718,11 @@ void job_params(cm_job* J, const cm_job_desc& d)
     P.cutsys = d.usecutoff ? d.cutoff : d.dbulk;
     P.cut2 = P.cutsys * P.cutsys;
     P.dbulk = d.dbulk;
+    P.e2lo = d.binstep * d.binstep * (1.0 + 1e-12);
+    P.e2hi = d.binstep * d.binstep * (1.0 - 1e-12);
+    P.db2lo = d.dbulk * d.dbulk * (1.0 - 1e-12);
+    P.db2hi = d.dbulk * d.dbulk * (1.0 + 1e-12);
+    P.inv_binstep = (float)(1.0 / d.binstep);
     P.K = J->K;
     P.kb = J->kb;
     P.key0 = (uint32_t)d.seed;
