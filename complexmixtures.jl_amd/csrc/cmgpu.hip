@@ -486,13 +486,14 @@ __global__ void k_prep_solvent(KP P, Bufs B, const T* xv)
         double tz = H[2] * n[0] + H[5] * n[1] + H[8] * n[2];
         double x0, y0, z0;
         load3(x, a0, x0, y0, z0);
+        B.vcell[(int64_t)f * P.Mv + v] = cell;
+        if (P.self_) continue;   // the pair kernel reads the solute copies (k_prep_solute_mols)
         for (int a = 0; a < P.nv; a++) {
             double xa, ya, za, ox, oy, oz;
             load3(x, a0 + a, xa, ya, za);
             cm::minimage(g.cell, xa - x0, ya - y0, za - z0, &ox, &oy, &oz);
             B.vw[(int64_t)f * P.Nv + a0 + a] = mk4((float)(x0 + ox - tx), (float)(y0 + oy - ty), (float)(z0 + oz - tz), 0.f);
         }
-        B.vcell[(int64_t)f * P.Mv + v] = cell;
         B.vcen[(int64_t)f * P.Mv + v] = mk4((float)(c[0] - tx), (float)(c[1] - ty), (float)(c[2] - tz), 0.f);
     }
 }
@@ -881,8 +882,11 @@ struct PairCtx {
     unsigned* shg;
 
     __device__ PairCtx(const KP& P_, const Bufs& B_, int f_, const TU* xu, const TV* xv, unsigned* sh_, unsigned* shg_)
-        : P(P_), B(B_), g(B_.geo[f_]), gh(B_.geo[f_]), f(f_), solw(B_.solw + (int64_t)f_ * P_.Nu), vwf(B_.vw + (int64_t)f_ * P_.Nv),
-          vcen(B_.vcen + (int64_t)f_ * P_.Mv), vcellf(B_.vcell + (int64_t)f_ * P_.Mv), xuf(xu + (int64_t)f_ * P_.Nu * 3),
+        : P(P_), B(B_), g(B_.geo[f_]), gh(B_.geo[f_]), f(f_), solw(B_.solw + (int64_t)f_ * P_.Nu),
+          // self: the solvent side reads the solute copies (the same molecules,
+          // wrapped identically), which halves the data each frame touches
+          vwf(P_.self_ ? B_.solw + (int64_t)f_ * P_.Nu : B_.vw + (int64_t)f_ * P_.Nv),
+          vcen(P_.self_ ? B_.ucen + (int64_t)f_ * P_.Mu : B_.vcen + (int64_t)f_ * P_.Mv), vcellf(B_.vcell + (int64_t)f_ * P_.Mv), xuf(xu + (int64_t)f_ * P_.Nu * 3),
           xvf(xv + (int64_t)f_ * P_.Nv * 3), dp(B_.drawn_ptr + (int64_t)f_ * (P_.Mu + 1)), di(B_.drawn_idx + (int64_t)f_ * P_.K),
           sh(sh_), shg(shg_) {}
 
@@ -1153,8 +1157,18 @@ struct PairCtx {
 // of both orientations (same distance and argmin pair; reference atoms and tie
 // order per orientation).
 template <class TU, class TV, int NU>
-__global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const TU* xu, const TV* xv)
+__global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const TU* xu, const TV* xv, int gx)
 {
+    // XCD-aware order: workgroups go round-robin to the 8 XCDs, each with its
+    // own L2.  XCD k takes the k-th eighth of the (frame, block) items in
+    // frame-major order, so each XCD works through its frames one at a time
+    // and its L2 holds the molecules of one frame instead of a slice of
+    // every frame of the batch.  gx blocks cover one frame.
+    const int items = gx * P.F;
+    const int per = (items + 7) >> 3;
+    const int item = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (item >= items) return;   // uniform per block, before any barrier
+    const int f = item / gx, bx = item - (item / gx) * gx;
     extern __shared__ unsigned sh[];
     const int gwords = P.glds ? P.nbins * (P.gnu + (P.self_ ? 0 : P.gnv)) : 0;
     hist_clear(sh, 2 * P.nbins + gwords);
@@ -1164,7 +1178,6 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
     int4* dq = (int4*)(sh + ((2 * P.nbins + gwords + 3) & ~3)) + (BLOCK / 64) * QCAP + (threadIdx.x >> 6) * 2 * DCAP;
     int dqn = 0;
     const int lane = threadIdx.x & 63;
-    const int f = blockIdx.y;
     const PairCtx<TU, TV, NU> ctx(P, B, f, xu, xv, sh, shg);
     const GeoHot& gh = ctx.gh;
     const float4* srt = B.sorted + (int64_t)f * B.nitems;
@@ -1173,9 +1186,9 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
     const int nd[3] = {min(nc[0], 3), min(nc[1], 3), min(nc[2], 3)};
     const bool simple = nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3;
     constexpr int CU = 8;   // candidates per lane per step
-    const int stride = gridDim.x * blockDim.x;
+    const int stride = gx * blockDim.x;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < P.Mv; base += stride) {
+    for (int base = bx * blockDim.x + (threadIdx.x & ~63); base < P.Mv; base += stride) {
         const int q0 = base + lane;
         const bool act = q0 < P.Mv;
         const int v = act ? (P.self_ ? __float_as_int(srt[q0].w) : q0) : 0;
@@ -2426,19 +2439,19 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         const int gwords = P.glds ? P.nbins * (P.gnu + (P.self_ ? 0 : P.gnv)) : 0;
         const size_t shq = sizeof(unsigned) * ((2 * P.nbins + gwords + 3) & ~3) + sizeof(int4) * QCAP * (BLOCK / 64) +
                            (P.self_ ? sizeof(int4) * 2 * DCAP * (BLOCK / 64) : 0);
-        const dim3 gr((unsigned)std::max<int64_t>(1, std::min<int64_t>((P.Mv + BLOCK - 1) / BLOCK, 4096 / F + 1)),
-                      (unsigned)F);
+        const int gx = (int)std::max<int64_t>(1, std::min<int64_t>((P.Mv + BLOCK - 1) / BLOCK, 4096 / F + 1));
+        const dim3 gr((unsigned)(8 * ((gx * F + 7) / 8)));
         // self-MDDF of 3- and 8-atom molecules (water, urea-like) keeps both
         // molecules of a pair in registers
         const int nfix = (P.self_ && P.nu == P.nv && (P.nu == 3 || P.nu == 8)) ? P.nu : 0;
         if (f32) {
-            if (nfix == 3) hipLaunchKernelGGL((k_eval_mols_real<float, float, 3>), gr, dim3(BLOCK), shq, s, P, B, uf, vf);
-            else if (nfix == 8) hipLaunchKernelGGL((k_eval_mols_real<float, float, 8>), gr, dim3(BLOCK), shq, s, P, B, uf, vf);
-            else hipLaunchKernelGGL((k_eval_mols_real<float, float, 0>), gr, dim3(BLOCK), shq, s, P, B, uf, vf);
+            if (nfix == 3) hipLaunchKernelGGL((k_eval_mols_real<float, float, 3>), gr, dim3(BLOCK), shq, s, P, B, uf, vf, gx);
+            else if (nfix == 8) hipLaunchKernelGGL((k_eval_mols_real<float, float, 8>), gr, dim3(BLOCK), shq, s, P, B, uf, vf, gx);
+            else hipLaunchKernelGGL((k_eval_mols_real<float, float, 0>), gr, dim3(BLOCK), shq, s, P, B, uf, vf, gx);
         } else {
-            if (nfix == 3) hipLaunchKernelGGL((k_eval_mols_real<double, double, 3>), gr, dim3(BLOCK), shq, s, P, B, ud, vd);
-            else if (nfix == 8) hipLaunchKernelGGL((k_eval_mols_real<double, double, 8>), gr, dim3(BLOCK), shq, s, P, B, ud, vd);
-            else hipLaunchKernelGGL((k_eval_mols_real<double, double, 0>), gr, dim3(BLOCK), shq, s, P, B, ud, vd);
+            if (nfix == 3) hipLaunchKernelGGL((k_eval_mols_real<double, double, 3>), gr, dim3(BLOCK), shq, s, P, B, ud, vd, gx);
+            else if (nfix == 8) hipLaunchKernelGGL((k_eval_mols_real<double, double, 8>), gr, dim3(BLOCK), shq, s, P, B, ud, vd, gx);
+            else hipLaunchKernelGGL((k_eval_mols_real<double, double, 0>), gr, dim3(BLOCK), shq, s, P, B, ud, vd, gx);
         }
     }
     if (J->profiling) HIPCHK(hipEventRecord(ev[1], s));
