@@ -1157,7 +1157,7 @@ struct PairCtx {
 // of both orientations (same distance and argmin pair; reference atoms and tie
 // order per orientation).
 template <class TU, class TV, int NU>
-__global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const TU* xu, const TV* xv, int gx)
+__global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const TU* xu, const TV* xv, int gx, int S)
 {
     // XCD-aware order: workgroups go round-robin to the 8 XCDs, each with its
     // own L2.  XCD k takes the k-th eighth of the (frame, block) items in
@@ -1188,8 +1188,11 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
     constexpr int CU = 8;   // candidates per lane per step
     const int stride = gx * blockDim.x;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    for (int base = bx * blockDim.x + (threadIdx.x & ~63); base < P.Mv; base += stride) {
-        const int q0 = base + lane;
+    // S lanes per solvent molecule (small systems): lane sub of a molecule
+    // walks the offsets sub, sub + S, ... of the neighbour walk
+    for (int base = bx * blockDim.x + (threadIdx.x & ~63); base < P.Mv * S; base += stride) {
+        const int vl = base + lane;
+        const int q0 = vl / S, sub = vl - (vl / S) * S;
         const bool act = q0 < P.Mv;
         const int v = act ? (P.self_ ? __float_as_int(srt[q0].w) : q0) : 0;
         const float4 cv = ctx.vcen[v];
@@ -1206,15 +1209,19 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
         // (0, 0, 0) (linear index 13 of the 27; z, then y, then x), and pairs
         // inside one cell once with u < v.  Otherwise all offsets with u < v.
         const int o_first = (P.self_ && simple) ? 13 : 0;
-        int o = o_first, kpos = 0, kmax = -1;
+        const int nsteps = (ncells - o_first + S - 1) / S;   // walk steps, the same for every lane
+        int ks = 0, o = o_first + sub, kpos = 0, kmax = -1;
         int cell = 0, cs = 0, cnt = 0, code = -1, kx = 1, ky = 1, kz = 1;
         int sx[3], sy[3], sz[3];
         float t1x = 0.f, t1y = 0.f, t1z = 0.f;
         for (;;) {
-            while (qn < 64 && o < ncells) {
+            while (qn < 64 && ks < nsteps) {
                 if (kmax < 0) {
                     // enter neighbour cell o: offsets -1..1, or 0..nc-1 when nc < 3
-                    const int ox = o % nd[0], oy = (o / nd[0]) % nd[1], oz = o / (nd[0] * nd[1]);
+                    // (a lane whose offsets are used up walks nothing)
+                    const bool ov = o < ncells;
+                    const int oc = ov ? o : o_first;
+                    const int ox = oc % nd[0], oy = (oc / nd[0]) % nd[1], oz = oc / (nd[0] * nd[1]);
                     int m0;
                     const int cx = nb_cell(cc[0], nc[0] >= 3 ? ox - 1 : ox, nc[0], m0);
                     const int cy = nb_cell(cc[1], nc[1] >= 3 ? oy - 1 : oy, nc[1], m0);
@@ -1234,12 +1241,13 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
                         kz = shifts_onto(cc[2], cz, nc[2], sz);
                     }
                     cs = st[cell];
-                    cnt = act ? st[cell + 1] - cs : 0;
+                    cnt = (act && ov) ? st[cell + 1] - cs : 0;
                     kmax = wave_max_i(cnt);
                     kpos = 0;
                 }
                 if (kpos >= kmax) {
-                    o++;
+                    ks++;
+                    o += S;
                     kmax = -1;
                     continue;
                 }
@@ -2439,19 +2447,23 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         const int gwords = P.glds ? P.nbins * (P.gnu + (P.self_ ? 0 : P.gnv)) : 0;
         const size_t shq = sizeof(unsigned) * ((2 * P.nbins + gwords + 3) & ~3) + sizeof(int4) * QCAP * (BLOCK / 64) +
                            (P.self_ ? sizeof(int4) * 2 * DCAP * (BLOCK / 64) : 0);
-        const int gx = (int)std::max<int64_t>(1, std::min<int64_t>((P.Mv + BLOCK - 1) / BLOCK, 4096 / F + 1));
+        // small systems: S lanes per solvent molecule split its neighbour walk
+        // so that the launch still has about 6 wavefronts per SIMD of work
+        const int64_t lanes = (int64_t)P.Mv * F;
+        const int S = (int)std::min<int64_t>(14, std::max<int64_t>(1, (int64_t)(J->cus * 4 * 6 * 64) / std::max<int64_t>(lanes, 1)));
+        const int gx = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)P.Mv * S + BLOCK - 1) / BLOCK, 4096 / F + 1));
         const dim3 gr((unsigned)(8 * ((gx * F + 7) / 8)));
         // self-MDDF of 3- and 8-atom molecules (water, urea-like) keeps both
         // molecules of a pair in registers
         const int nfix = (P.self_ && P.nu == P.nv && (P.nu == 3 || P.nu == 8)) ? P.nu : 0;
         if (f32) {
-            if (nfix == 3) hipLaunchKernelGGL((k_eval_mols_real<float, float, 3>), gr, dim3(BLOCK), shq, s, P, B, uf, vf, gx);
-            else if (nfix == 8) hipLaunchKernelGGL((k_eval_mols_real<float, float, 8>), gr, dim3(BLOCK), shq, s, P, B, uf, vf, gx);
-            else hipLaunchKernelGGL((k_eval_mols_real<float, float, 0>), gr, dim3(BLOCK), shq, s, P, B, uf, vf, gx);
+            if (nfix == 3) hipLaunchKernelGGL((k_eval_mols_real<float, float, 3>), gr, dim3(BLOCK), shq, s, P, B, uf, vf, gx, S);
+            else if (nfix == 8) hipLaunchKernelGGL((k_eval_mols_real<float, float, 8>), gr, dim3(BLOCK), shq, s, P, B, uf, vf, gx, S);
+            else hipLaunchKernelGGL((k_eval_mols_real<float, float, 0>), gr, dim3(BLOCK), shq, s, P, B, uf, vf, gx, S);
         } else {
-            if (nfix == 3) hipLaunchKernelGGL((k_eval_mols_real<double, double, 3>), gr, dim3(BLOCK), shq, s, P, B, ud, vd, gx);
-            else if (nfix == 8) hipLaunchKernelGGL((k_eval_mols_real<double, double, 8>), gr, dim3(BLOCK), shq, s, P, B, ud, vd, gx);
-            else hipLaunchKernelGGL((k_eval_mols_real<double, double, 0>), gr, dim3(BLOCK), shq, s, P, B, ud, vd, gx);
+            if (nfix == 3) hipLaunchKernelGGL((k_eval_mols_real<double, double, 3>), gr, dim3(BLOCK), shq, s, P, B, ud, vd, gx, S);
+            else if (nfix == 8) hipLaunchKernelGGL((k_eval_mols_real<double, double, 8>), gr, dim3(BLOCK), shq, s, P, B, ud, vd, gx, S);
+            else hipLaunchKernelGGL((k_eval_mols_real<double, double, 0>), gr, dim3(BLOCK), shq, s, P, B, ud, vd, gx, S);
         }
     }
     if (J->profiling) HIPCHK(hipEventRecord(ev[1], s));
