@@ -116,6 +116,36 @@ def config3_rates(args):
     return {"frames": nf, "triclinic": True, "input": "f64", **out}
 
 
+def self_water_rate(args):
+    """Self-MDDF of water, coordination numbers: 100 A cubic box, 33,400 3-site
+    waters (the largest self requests of the served stream), 40 frames, FP32."""
+    from complexmixtures_amd import _lib
+    import complexmixtures_amd.serve as SV
+    from complexmixtures_amd.synthetic import served_system
+    nf = 40
+    s = served_system(7, 2000, 100.0, nf)
+    sysm = SV.System(n_solute=2000, n_water=s["n_water"], box=100.0, nframes=nf,
+                     d_solute=_lib.DeviceArray(s["x_solute"]), d_water=_lib.DeviceArray(s["x_solvent"]),
+                     cells=s["cells"], irefatom=SV.default_irefatom(s["x_solvent"][0]))
+    kw, du, dv, _ = SV.job_kwargs(sysm, SV.Request(0.0, 0, nf, False, True, 99))
+    for rep in range(3):
+        job = _lib.Job(**kw)
+        job.set_profiling(True)
+        t = time.perf_counter()
+        job.submit_device(np.arange(1, nf + 1), du, dv, 0, sysm.cells[:nf], np.ones(nf))
+        raw = job.finish()
+        el = time.perf_counter() - t
+        st = job.stats()
+        job.close()
+    sysm.d_solute.free()
+    sysm.d_water.free()
+    # each unordered pair within the cutoff is two records (both orientations)
+    return {"workload": f"self-MDDF coordination numbers, 100 A cubic box, {s['n_water']} waters, {nf} frames, FP32",
+            "frames_per_s": round(nf / el, 2), "eval_ms_per_frame": round(st["eval_ms"] / nf, 4),
+            "pairs_within_cutoff_per_frame": round(float(raw["md_count"].sum()) / 2 / nf),
+            "fp64_exhaustive_resolves": st["fp64_rechecks"]}
+
+
 def served_stream(args):
     """Served mixed-size stream (Poisson arrivals at 20 requests/s) through the
     request server vs the plain one-job-at-a-time path, plus both paths'
@@ -176,6 +206,7 @@ def main():
     ap.add_argument("--serve-frames", type=int, default=200)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--no-config3", action="store_true", help="skip the triclinic mixed-case rates")
+    ap.add_argument("--self-water-only", action="store_true", help="only the self-MDDF water rate")
     ap.add_argument("--config3-frames", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain read-N/write-N elementwise kernel over MIB MiB three times (the "
@@ -199,6 +230,9 @@ def main():
     pkg = G.load_package()
     if args.serve_only:
         print(json.dumps({"served_stream": served_stream(args)}), flush=True)
+        return
+    if args.self_water_only:
+        print(json.dumps({"self_water": self_water_rate(args)}), flush=True)
         return
     import complexmixtures_amd.synthetic as S
     from complexmixtures_amd import _lib
@@ -321,6 +355,7 @@ def main():
         out["served_stream"] = served_stream(args)
     if not args.no_config3:
         out["triclinic_mixed"] = config3_rates(args)
+        out["self_water"] = self_water_rate(args)
     print(json.dumps(out), flush=True)
 
 

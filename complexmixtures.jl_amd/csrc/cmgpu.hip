@@ -843,21 +843,27 @@ __device__ __forceinline__ int wave_max_i(int x)
 }
 
 // Lattice shifts m (per dimension, at most three) that map the cell index cv
-// of one molecule onto the neighbour cell cu of another: the offsets o in
-// {-1, 0, 1} with wrap(cv + o) == cu.  With three or more cells per dimension
-// there is exactly one; small grids map several offsets onto one cell.
-__device__ __forceinline__ int shifts_onto(int cv, int cu, int n, int m[3])
+// of one molecule onto the neighbour cell cu of another, for the offsets o in
+// {-1, 0, 1} with wrap(cv + o) == cu, as a set: bit m + 1.  With three or
+// more cells per dimension there is exactly one; small grids map several
+// offsets onto one cell.  (A bit set rather than an array: dynamically
+// indexed arrays live in scratch memory.)
+__device__ __forceinline__ int shifts_onto(int cv, int cu, int n)
 {
-    int k = 0;
+    int set = 0;
     for (int o = -1; o <= 1; o++) {
         int mm;
-        if (nb_cell(cv, o, n, mm) == cu) {
-            bool dup = false;
-            for (int e = 0; e < k; e++) dup |= (m[e] == mm);
-            if (!dup) m[k++] = mm;
-        }
+        if (nb_cell(cv, o, n, mm) == cu) set |= 1 << (mm + 1);
     }
-    return k;
+    return set;
+}
+
+// the k-th (from 0, ascending) lattice shift of a shift set
+__device__ __forceinline__ int nth_shift(int set, int k)
+{
+    if (k > 0) set &= set - 1;
+    if (k > 1) set &= set - 1;
+    return __builtin_ctz((unsigned)set) - 1;
 }
 
 // Per-frame state of the molecule-pair kernels: records of molecule pairs
@@ -1071,18 +1077,18 @@ struct PairCtx {
     {
         const int u = e.x, v = e.y, cu = e.z;
         const int nc0 = gh.nc[0], nc1 = gh.nc[1], nc2 = gh.nc[2];
-        int mx[3], my[3], mz[3];
-        int kx = 1, ky = 1, kz = 1;
+        int mxs, mys, mzs;   // shift sets (bit m + 1)
         if (e.w >= 0) {
-            mx[0] = e.w % 3 - 1;
-            my[0] = (e.w / 3) % 3 - 1;
-            mz[0] = e.w / 9 - 1;
+            mxs = 1 << (e.w % 3);
+            mys = 1 << ((e.w / 3) % 3);
+            mzs = 1 << (e.w / 9);
         } else {
             const int vc = vcellf[v];
-            kx = shifts_onto(vc % nc0, cu % nc0, nc0, mx);
-            ky = shifts_onto((vc / nc0) % nc1, (cu / nc0) % nc1, nc1, my);
-            kz = shifts_onto(vc / (nc0 * nc1), cu / (nc0 * nc1), nc2, mz);
+            mxs = shifts_onto(vc % nc0, cu % nc0, nc0);
+            mys = shifts_onto((vc / nc0) % nc1, (cu / nc0) % nc1, nc1);
+            mzs = shifts_onto(vc / (nc0 * nc1), cu / (nc0 * nc1), nc2);
         }
+        const int kx = __popc(mxs), ky = __popc(mys), kz = __popc(mzs);
         const float4* xs = solw + (int64_t)u * P.nu;
         const float4* yw = vwf + (int64_t)v * P.nv;
         const float4 cv = vcen[v];
@@ -1097,23 +1103,58 @@ struct PairCtx {
             for (int i = 0; i < NU; i++) X[i] = xs[i];
 #pragma unroll
             for (int a = 0; a < NU; a++) Y[a] = yw[a];
-            for (int iz = 0; iz < kz; iz++)
-            for (int iy = 0; iy < ky; iy++)
-            for (int ix = 0; ix < kx; ix++) {
-                float tx, ty, tz;
-                shift_of(gh, mx[ix], my[iy], mz[iz], tx, ty, tz);
-                const float dx = cv.x - (cuc.x + tx), dy = cv.y - (cuc.y + ty), dz = cv.z - (cuc.z + tz);
-                if (dx * dx + dy * dy + dz * dz > gh.rpre2) continue;
+            if constexpr (NU <= 4) {
+                auto atoms = [&](float tx, float ty, float tz) {
 #pragma unroll
-                for (int a = 0; a < NU; a++) {
-                    const float yx = Y[a].x - tx, yy = Y[a].y - ty, yz = Y[a].z - tz;
+                    for (int a = 0; a < NU; a++) {
+                        const float yx = Y[a].x - tx, yy = Y[a].y - ty, yz = Y[a].z - tz;
 #pragma unroll
-                    for (int i = 0; i < NU; i++) {
-                        const float ex = yx - X[i].x, ey = yy - X[i].y, ez = yz - X[i].z;
-                        const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-                        bm.offer(d2, i, a);
-                        if (a == P.iref) brv.offer(d2, i, a);
-                        if (i == P.iref) bru.offer(d2, a, i);
+                        for (int i = 0; i < NU; i++) {
+                            const float ex = yx - X[i].x, ey = yy - X[i].y, ez = yz - X[i].z;
+                            const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+                            bm.offer(d2, i, a);
+                            if (a == P.iref) brv.offer(d2, i, a);
+                            if (i == P.iref) bru.offer(d2, a, i);
+                        }
+                    }
+                };
+                if (e.w >= 0) {
+                    // one shift, whose centre test the walk has already passed
+                    // with the same arithmetic (small molecules: a second
+                    // inlined copy of the atom loop is cheap)
+                    float tx, ty, tz;
+                    shift_of(gh, e.w % 3 - 1, (e.w / 3) % 3 - 1, e.w / 9 - 1, tx, ty, tz);
+                    atoms(tx, ty, tz);
+                } else {
+                    for (int iz = 0; iz < kz; iz++)
+                    for (int iy = 0; iy < ky; iy++)
+                    for (int ix = 0; ix < kx; ix++) {
+                        float tx, ty, tz;
+                        shift_of(gh, nth_shift(mxs, ix), nth_shift(mys, iy), nth_shift(mzs, iz), tx, ty, tz);
+                        const float dx = cv.x - (cuc.x + tx), dy = cv.y - (cuc.y + ty), dz = cv.z - (cuc.z + tz);
+                        if (dx * dx + dy * dy + dz * dz > gh.rpre2) continue;
+                        atoms(tx, ty, tz);
+                    }
+                }
+            } else {
+                for (int iz = 0; iz < kz; iz++)
+                for (int iy = 0; iy < ky; iy++)
+                for (int ix = 0; ix < kx; ix++) {
+                    float tx, ty, tz;
+                    shift_of(gh, nth_shift(mxs, ix), nth_shift(mys, iy), nth_shift(mzs, iz), tx, ty, tz);
+                    const float dx = cv.x - (cuc.x + tx), dy = cv.y - (cuc.y + ty), dz = cv.z - (cuc.z + tz);
+                    if (dx * dx + dy * dy + dz * dz > gh.rpre2) continue;
+#pragma unroll
+                    for (int a = 0; a < NU; a++) {
+                        const float yx = Y[a].x - tx, yy = Y[a].y - ty, yz = Y[a].z - tz;
+#pragma unroll
+                        for (int i = 0; i < NU; i++) {
+                            const float ex = yx - X[i].x, ey = yy - X[i].y, ez = yz - X[i].z;
+                            const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+                            bm.offer(d2, i, a);
+                            if (a == P.iref) brv.offer(d2, i, a);
+                            if (i == P.iref) bru.offer(d2, a, i);
+                        }
                     }
                 }
             }
@@ -1122,7 +1163,7 @@ struct PairCtx {
             for (int iy = 0; iy < ky; iy++)
             for (int ix = 0; ix < kx; ix++) {
                 float tx, ty, tz;
-                shift_of(gh, mx[ix], my[iy], mz[iz], tx, ty, tz);
+                shift_of(gh, nth_shift(mxs, ix), nth_shift(mys, iy), nth_shift(mzs, iz), tx, ty, tz);
                 const float dx = cv.x - (cuc.x + tx), dy = cv.y - (cuc.y + ty), dz = cv.z - (cuc.z + tz);
                 if (dx * dx + dy * dy + dz * dz > gh.rpre2) continue;
                 for (int a = 0; a < P.nv; a++) {
@@ -1212,7 +1253,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
         const int nsteps = (ncells - o_first + S - 1) / S;   // walk steps, the same for every lane
         int ks = 0, o = o_first + sub, kpos = 0, kmax = -1;
         int cell = 0, cs = 0, cnt = 0, code = -1, kx = 1, ky = 1, kz = 1;
-        int sx[3], sy[3], sz[3];
+        int sxs = 0, sys_ = 0, szs = 0;   // shift sets of the current cell (small grids)
         float t1x = 0.f, t1y = 0.f, t1z = 0.f;
         for (;;) {
             while (qn < 64 && ks < nsteps) {
@@ -1236,9 +1277,12 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
                         code = (mx + 1) + 3 * (my + 1) + 9 * (mz + 1);
                         shift_of(gh, mx, my, mz, t1x, t1y, t1z);
                     } else {
-                        kx = shifts_onto(cc[0], cx, nc[0], sx);
-                        ky = shifts_onto(cc[1], cy, nc[1], sy);
-                        kz = shifts_onto(cc[2], cz, nc[2], sz);
+                        sxs = shifts_onto(cc[0], cx, nc[0]);
+                        sys_ = shifts_onto(cc[1], cy, nc[1]);
+                        szs = shifts_onto(cc[2], cz, nc[2]);
+                        kx = __popc(sxs);
+                        ky = __popc(sys_);
+                        kz = __popc(szs);
                     }
                     cs = st[cell];
                     cnt = (act && ov) ? st[cell + 1] - cs : 0;
@@ -1270,7 +1314,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
                         for (int iy = 0; iy < ky; iy++)
                         for (int ix = 0; ix < kx; ix++) {
                             float tx, ty, tz;
-                            shift_of(gh, sx[ix], sy[iy], sz[iz], tx, ty, tz);
+                            shift_of(gh, nth_shift(sxs, ix), nth_shift(sys_, iy), nth_shift(szs, iz), tx, ty, tz);
                             const float dx = cv.x - (cu4[ww].x + tx), dy = cv.y - (cu4[ww].y + ty),
                                         dz = cv.z - (cu4[ww].z + tz);
                             nearby |= dx * dx + dy * dy + dz * dz <= gh.rpre2;
