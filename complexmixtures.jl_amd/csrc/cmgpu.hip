@@ -88,6 +88,7 @@ struct Geo {                 // one frame of a batch
     int kbits;               // bits per dimension of the Morton keys of the fine cells
     int ortho;               // diagonal cell matrix: cells are boxes (exact near-cell test)
     double guard;            // bound on |d2(FP32) - d2(FP64)| for pairs up to the cutoff
+    float guard32;           // guard + (cutoff^2 + guard + 1) 2^-21, rounded up: covers FP32 b -+ guard32 rounding
 };
 
 struct KP {                  // job parameters, by value
@@ -97,6 +98,9 @@ struct KP {                  // job parameters, by value
     double e2lo, e2hi;       // binstep^2 (1 + 1e-12), binstep^2 (1 - 1e-12): squared bin edges with margins
     double db2lo, db2hi;     // dbulk^2 (1 - 1e-12), dbulk^2 (1 + 1e-12)
     float inv_binstep;       // 1 / binstep in FP32 (bin estimates only)
+    // FP32 thresholds of the all-FP32 decision (decide32), rounded outward:
+    // cutoff^2 down / up, binstep^2 (1 -+ 4e-7) down / up, dbulk^2 (1 -+ 4e-7)
+    float c2lo32, c2hi32, bs2m32, bs2p32, db2lo32, db2hi32;
     int K;                   // random passes per frame (n_random_samples, 0 for coordination numbers)
     int kb;                  // bulk lists per frame (1 in the atoms family, K in the mols family)
     uint32_t key0, key1;
@@ -338,6 +342,7 @@ __global__ void k_grid(KP P, Bufs B, int mols_family)
     double dmax = P.cutsys + 0.5;
     double G = E * (2.0 * 1.7320508075688772 * dmax + 3.0 * E) + 8.0 * 0x1p-24 * dmax * dmax;
     g.guard = G;
+    g.guard32 = nextafterf((float)(G + (P.cut2 + G + 1.0) * 0x1p-21), INFINITY);
     float hi = (float)(P.cut2 + G);
     g.cut2_hi = nextafterf(hi, INFINITY);
     float ulpc = (float)(halfulp32((double)g.cut2_hi) * 2.0);
@@ -649,6 +654,28 @@ __device__ __forceinline__ bool decide_fp32(const KP& P, double G, float b, bool
     return true;
 }
 
+// The decision of decide_fp32 in FP32 only, branch-free: [lo, hi] =
+// b -+ guard32 (guard32 covers the FP32 rounding of both sums, so [lo, hi]
+// contains [b - G, b + G]), and every threshold is rounded outward: the
+// cutoff^2 bounds, k^2 binstep^2 (1 - 4e-7) <= (k binstep)^2 (1 - 1e-12) and
+// (k-1)^2 binstep^2 (1 + 4e-7) >= ((k-1) binstep)^2 (1 + 1e-12) after the FP32
+// roundings of k^2 and the products (each below 1.2e-7 relative), and the
+// dbulk^2 bounds likewise.  Returns 1 when no pair can be within the cutoff,
+// 2 when the record is within with bin and bulk set, 0 when undecided.
+__device__ __forceinline__ int decide32(const KP& P, float G32, float b, int& bin, int& bulk)
+{
+    const float lo = b - G32, hi = b + G32;
+    const int k = max((int)ceilf(__builtin_amdgcn_sqrtf(b) * P.inv_binstep), 1);
+    const float kf = (float)k, km = kf - 1.0f;
+    const bool inbin = hi < kf * kf * P.bs2m32 && (k == 1 || lo > km * km * P.bs2p32);
+    const bool b1 = lo > P.db2hi32, b0 = hi < P.db2lo32;
+    const bool okbulk = !P.usecutoff || b1 || b0;
+    bin = k - 1;
+    bulk = (P.usecutoff && b1) ? 1 : 0;
+    const bool within = hi <= P.c2lo32 && inbin && okbulk;
+    return lo > P.c2hi32 ? 1 : (within ? 2 : 0);
+}
+
 // FP32 running minimum with second best
 struct Best {
     float best, second;
@@ -727,11 +754,11 @@ __device__ __forceinline__ float wave_max(float x)
 // every global store or atomic (which may alias the geometry array).
 struct GeoHot {
     float H32[9];
-    float cut2_hi, amb, rpre2;
+    float cut2_hi, amb, rpre2, guard32;
     double guard;
     int nc[3];
     __device__ explicit GeoHot(const Geo& g)
-        : cut2_hi(g.cut2_hi), amb(g.amb), rpre2(g.rpre2), guard(g.guard)
+        : cut2_hi(g.cut2_hi), amb(g.amb), rpre2(g.rpre2), guard32(g.guard32), guard(g.guard)
     {
 #pragma unroll
         for (int k = 0; k < 9; k++) H32[k] = g.H32[k];
@@ -970,25 +997,14 @@ struct PairCtx {
                          (brv.best <= gh.cut2_hi && !(brv.second - brv.best > gh.amb)) ||
                          (bru.best <= gh.cut2_hi && !(bru.second - bru.best > gh.amb));
         if (amb) return 8;
-        int mask = 0;
-        bool w;
-        int bn, bk;
-        if (bm.best <= gh.cut2_hi) {
-            if (decide_fp32(P, gh.guard, bm.best, w, bn, bk)) {
-                if (w) self_main(u, v, bm.bi, bm.ba, bn, bk);
-            } else mask |= 1;
-        }
-        if (brv.best <= gh.cut2_hi) {
-            if (decide_fp32(P, gh.guard, brv.best, w, bn, bk)) {
-                if (w) rdf_hit(bn);
-            } else mask |= 2;
-        }
-        if (bru.best <= gh.cut2_hi) {
-            if (decide_fp32(P, gh.guard, bru.best, w, bn, bk)) {
-                if (w) rdf_hit(bn);
-            } else mask |= 4;
-        }
-        return mask;
+        int bm_bin, bm_bulk, rv_bin, ru_bin, dummy;
+        const int dmain = bm.best <= gh.cut2_hi ? decide32(P, gh.guard32, bm.best, bm_bin, bm_bulk) : 1;
+        const int drv = brv.best <= gh.cut2_hi ? decide32(P, gh.guard32, brv.best, rv_bin, dummy) : 1;
+        const int dru = bru.best <= gh.cut2_hi ? decide32(P, gh.guard32, bru.best, ru_bin, dummy) : 1;
+        if (dmain == 2) self_main(u, v, bm.bi, bm.ba, bm_bin, bm_bulk);
+        if (drv == 2) rdf_hit(rv_bin);
+        if (dru == 2) rdf_hit(ru_bin);
+        return (dmain == 0 ? 1 : 0) | (drv == 0 ? 2 : 0) | (dru == 0 ? 4 : 0);
     }
 
     // Self mode, deferred FP64 stage of one pair (mask and entry from
@@ -2792,6 +2808,17 @@ void job_params(cm_job* J, const cm_job_desc& d)
     P.db2lo = d.dbulk * d.dbulk * (1.0 - 1e-12);
     P.db2hi = d.dbulk * d.dbulk * (1.0 + 1e-12);
     P.inv_binstep = (float)(1.0 / d.binstep);
+    {
+        auto down = [](double x) { float f = (float)x; return (double)f > x ? std::nextafter(f, -INFINITY) : f; };
+        auto up = [](double x) { float f = (float)x; return (double)f < x ? std::nextafter(f, INFINITY) : f; };
+        const double bs2 = d.binstep * d.binstep, db2 = d.dbulk * d.dbulk;
+        P.c2lo32 = down(P.cut2);
+        P.c2hi32 = up(P.cut2);
+        P.bs2m32 = down(bs2 * (1.0 - 4e-7));
+        P.bs2p32 = up(bs2 * (1.0 + 4e-7));
+        P.db2lo32 = down(db2 * (1.0 - 4e-7));
+        P.db2hi32 = up(db2 * (1.0 + 4e-7));
+    }
     P.K = J->K;
     P.kb = J->kb;
     P.key0 = (uint32_t)d.seed;
