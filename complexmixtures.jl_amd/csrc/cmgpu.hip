@@ -95,8 +95,6 @@ struct KP {                  // job parameters, by value
     int Mu, nu, Mv, nv;
     int self_, iref, nbins, usecutoff;
     double binstep, cutsys, cut2, dbulk;
-    double e2lo, e2hi;       // binstep^2 (1 + 1e-12), binstep^2 (1 - 1e-12): squared bin edges with margins
-    double db2lo, db2hi;     // dbulk^2 (1 - 1e-12), dbulk^2 (1 + 1e-12)
     float inv_binstep;       // 1 / binstep in FP32 (bin estimates only)
     // FP32 thresholds of the all-FP32 decision (decide32), rounded outward:
     // cutoff^2 down / up, binstep^2 (1 -+ 4e-7) down / up, dbulk^2 (1 -+ 4e-7)
@@ -628,40 +626,18 @@ __device__ __forceinline__ bool inbulk(const KP& P, const Rec& r)
 // recompute, when every FP64 d2 in [b - G, b + G] gives the same cutoff test,
 // bin and dbulk test.  The bin of d2 is max(1, ceil(fl(sqrt(d2)) / binstep));
 // between ((k-1) binstep)^2 (1 + 1e-12) and (k binstep)^2 (1 - 1e-12) it is k
-// whatever the FP64 rounding of the square root, the quotient and the edges
-// (all below 1e-15 relative; the edges are formed as k^2 * binstep^2 (1 -+
-// 1e-12), a few ulp from the exact products), and likewise d > dbulk is
-// decided outside dbulk^2 (1 -+ 1e-12).  Sets within, the 0-based bin and the dbulk flag;
-// returns false when the interval straddles a decision.
-__device__ __forceinline__ bool decide_fp32(const KP& P, double G, float b, bool& within, int& bin, int& bulk)
-{
-    const double lo = fmax((double)b - G, 0.0), hi = (double)b + G;
-    if (lo > P.cut2) { within = false; return true; }
-    if (!(hi <= P.cut2)) return false;
-    // bin estimate from the hardware square root: any k is safe, the edge
-    // tests accept it only when it is the bin of every d2 in [lo, hi]
-    int k = (int)ceilf(__builtin_amdgcn_sqrtf(b) * P.inv_binstep);
-    k = max(k, 1);
-    const double kd = (double)k, km = kd - 1.0;
-    if (!(hi < kd * kd * P.e2hi)) return false;
-    if (k > 1 && !(lo > km * km * P.e2lo)) return false;
-    if (!P.usecutoff) bulk = 0;   // the dbulk test only matters with usecutoff
-    else if (lo > P.db2hi) bulk = 1;
-    else if (hi < P.db2lo) bulk = 0;
-    else return false;
-    within = true;
-    bin = k - 1;
-    return true;
-}
-
-// The decision of decide_fp32 in FP32 only, branch-free: [lo, hi] =
-// b -+ guard32 (guard32 covers the FP32 rounding of both sums, so [lo, hi]
-// contains [b - G, b + G]), and every threshold is rounded outward: the
-// cutoff^2 bounds, k^2 binstep^2 (1 - 4e-7) <= (k binstep)^2 (1 - 1e-12) and
-// (k-1)^2 binstep^2 (1 + 4e-7) >= ((k-1) binstep)^2 (1 + 1e-12) after the FP32
-// roundings of k^2 and the products (each below 1.2e-7 relative), and the
-// dbulk^2 bounds likewise.  Returns 1 when no pair can be within the cutoff,
-// 2 when the record is within with bin and bulk set, 0 when undecided.
+// whatever the FP64 rounding of the square root and the quotient (below 1e-15
+// relative), and likewise d > dbulk is decided outside dbulk^2 (1 -+ 1e-12).
+// All in FP32 and branch-free: [lo, hi] = b -+ guard32 (guard32 covers the
+// FP32 rounding of both sums, so [lo, hi] contains [b - G, b + G]), and every
+// threshold is rounded outward: the cutoff^2 bounds, k^2 binstep^2 (1 - 4e-7)
+// <= (k binstep)^2 (1 - 1e-12) and (k-1)^2 binstep^2 (1 + 4e-7) >= ((k-1)
+// binstep)^2 (1 + 1e-12) after the FP32 roundings of k^2 and the products
+// (each below 1.2e-7 relative), and the dbulk^2 bounds likewise.  The bin
+// estimate k comes from the hardware square root; any k is safe, the edge
+// tests accept it only when it is the bin of every d2 in [lo, hi].  Returns 1
+// when no pair can be within the cutoff, 2 when the record is within with the
+// 0-based bin and the dbulk flag set, 0 when the interval straddles a decision.
 __device__ __forceinline__ int decide32(const KP& P, float G32, float b, int& bin, int& bulk)
 {
     const float lo = b - G32, hi = b + G32;
@@ -807,10 +783,10 @@ __device__ Rec resolve_mols(const KP& P, const Geo& g, const TU* xuf, int u, int
     int64_t ibase = (int64_t)u * P.nu, jbase = (int64_t)v * P.nv;
     if (bm.best <= g.cut2_hi) {
         if (bm.second - bm.best > g.amb) {
-            bool w;
             int bn, bk;
-            if (decide_fp32(P, g.guard, bm.best, w, bn, bk)) {
-                if (w) { r.within = 1; r.i = bm.bi; r.j = jbase + bm.ba; r.d = bm.best; r.bin = bn; r.bulk = bk; }
+            const int dd = decide32(P, g.guard32, bm.best, bn, bk);
+            if (dd != 0) {
+                if (dd == 2) { r.within = 1; r.i = bm.bi; r.j = jbase + bm.ba; r.d = bm.best; r.bin = bn; r.bulk = bk; }
             } else {
                 double x0, x1, x2, y0, y1, y2;
                 load3(xuf, ibase + bm.bi, x0, x1, x2);
@@ -822,10 +798,10 @@ __device__ Rec resolve_mols(const KP& P, const Geo& g, const TU* xuf, int u, int
     }
     if (br.best <= g.cut2_hi) {
         if (br.second - br.best > g.amb) {
-            bool w;
             int bn, bk;
-            if (decide_fp32(P, g.guard, br.best, w, bn, bk)) {
-                if (w) { r.ref = 1; r.dref = br.best; r.rbin = bn; }
+            const int dd = decide32(P, g.guard32, br.best, bn, bk);
+            if (dd != 0) {
+                if (dd == 2) { r.ref = 1; r.dref = br.best; r.rbin = bn; }
             } else {
                 double x0, x1, x2, y0, y1, y2;
                 load3(xuf, ibase + br.bi, x0, x1, x2);
@@ -2053,10 +2029,10 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
         if (leader) {
             if (mb <= g.cut2_hi) {
                 if (ms - mb > g.amb) {
-                    bool w;
                     int bn, bk;
-                    if (decide_fp32(P, g.guard, mb, w, bn, bk)) {
-                        if (w) { r.within = 1; r.i = mi; r.j = jbase + ma; r.d = mb; r.bin = bn; r.bulk = bk; }
+                    const int dd = decide32(P, g.guard32, mb, bn, bk);
+                    if (dd != 0) {
+                        if (dd == 2) { r.within = 1; r.i = mi; r.j = jbase + ma; r.d = mb; r.bin = bn; r.bulk = bk; }
                     } else {
                         double x0, x1, x2, y0, y1, y2;
                         load3(xuf, mi, x0, x1, x2);
@@ -2068,10 +2044,10 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
             }
             if (rb <= g.cut2_hi) {
                 if (rsec - rb > g.amb) {
-                    bool w;
                     int bn, bk;
-                    if (decide_fp32(P, g.guard, rb, w, bn, bk)) {
-                        if (w) { r.ref = 1; r.dref = rb; r.rbin = bn; }
+                    const int dd = decide32(P, g.guard32, rb, bn, bk);
+                    if (dd != 0) {
+                        if (dd == 2) { r.ref = 1; r.dref = rb; r.rbin = bn; }
                     } else {
                         double x0, x1, x2, y0, y1, y2;
                         load3(xuf, ri, x0, x1, x2);
@@ -2803,10 +2779,6 @@ void job_params(cm_job* J, const cm_job_desc& d)
     P.cutsys = d.usecutoff ? d.cutoff : d.dbulk;
     P.cut2 = P.cutsys * P.cutsys;
     P.dbulk = d.dbulk;
-    P.e2lo = d.binstep * d.binstep * (1.0 + 1e-12);
-    P.e2hi = d.binstep * d.binstep * (1.0 - 1e-12);
-    P.db2lo = d.dbulk * d.dbulk * (1.0 - 1e-12);
-    P.db2hi = d.dbulk * d.dbulk * (1.0 + 1e-12);
     P.inv_binstep = (float)(1.0 / d.binstep);
     {
         auto down = [](double x) { float f = (float)x; return (double)f > x ? std::nextafter(f, -INFINITY) : f; };
