@@ -335,6 +335,7 @@ def main():
                      "algorithmic_bytes_per_launch": round(bpf * frames_per_launch),
                      "kernel": "k_eval_tiles (real + random passes)", "pairs_per_frame": round(W),
                      "eval_ms_per_frame": round(st["eval_ms"] / nf, 5),
+                     "eval_ms_job": round(st["eval_ms"], 3), "eval_launches_job": st["eval_launches"],
                      "hbm_GBps_algorithmic": round(bpf * nf / eval_s / 1e9, 2),
                      "fp64_exhaustive_resolves": st["fp64_rechecks"]},
         "tuning": tuning,

@@ -2644,7 +2644,9 @@ int batch_plan(const cm_job* J, int32_t nframes, const double* cells, int ds, bo
     const int64_t budget = J->d.memory_budget > 0 ? J->d.memory_budget : (int64_t)4 << 30;
     int64_t Fmax = std::max<int64_t>(1, budget / std::max<int64_t>(pfb, 1));
     if (J->d.max_batch_frames > 0) Fmax = std::min<int64_t>(Fmax, J->d.max_batch_frames);
-    Fmax = std::min<int64_t>(Fmax, 64);
+    // at most 48 frames per batch: the random-pass evaluation of config-2 sized
+    // jobs ran 25% slower per frame with 64-frame batches than with 32-48
+    Fmax = std::min<int64_t>(Fmax, 48);
     *capmax_ = capmax;
     *kcapmax_ = kcapmax;
     *Fmax_ = Fmax;
@@ -2673,8 +2675,12 @@ int submit_common(cm_job* J, int32_t nframes, const int64_t* frame_index, const 
     while (f0 < nframes) {
         // a batch holds frames of one weight (the weight multiplies the integer hits at flush)
         double w = weights[f0];
-        int F = 1;
-        while (f0 + F < nframes && F < Fmax && weights[f0 + F] == w) F++;
+        // the frames of this weight in equal batches (200 frames: 5 x 40, not
+        // 4 x 48 + 8)
+        int run = 1;
+        while (f0 + run < nframes && weights[f0 + run] == w) run++;
+        const int nb = (int)((run + Fmax - 1) / Fmax);
+        const int F = (run + nb - 1) / nb;
         if (J->pending && J->pending_w != w) {
             int r = flush(J, J->pending_w);
             if (r) return r;
