@@ -28,6 +28,9 @@ def hipcc():
 def command(out=OUT, extra=()):
     return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
             "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            # a local that shadows another (e.g. an LDS pointer hidden by a
+            # private array of the same name) is an error, not a warning
+            "-Werror=shadow",
             "-I", os.path.join(ROOT, "include"), "-I", os.path.join(PKG_DIR, "csrc"),
             *extra, "-o", out, SRC]
 

@@ -1935,9 +1935,9 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
                 if (keep && x0 <= x1) {
                     int my, mz;
                     const int cy = wrap_index(iy, g.nc[1], my), cz = wrap_index(iz, g.nc[2], mz);
-                    const int row = (cz * g.nc[1] + cy) * g.nc[0] - mx * g.nc[0];
-                    cstart = st[row + x0];
-                    cnt = st[row + x1 + 1] - cstart;
+                    const int crow = (cz * g.nc[1] + cy) * g.nc[0] - mx * g.nc[0];
+                    cstart = st[crow + x0];
+                    cnt = st[crow + x1 + 1] - cstart;
                     shift_of(g, mx, my, mz, tx, ty, tz);
                 }
             }
