@@ -488,3 +488,59 @@ def test_self_coordination_numbers(pkg, oracle):
     a, _ = oracle_raw(pkg, oracle, traj, opts, coordination_number_only=True)
     b = gpu_raw(pkg, traj, opts, coordination_number_only=True)
     assert_raw_equal(a, b)
+
+
+def test_self_water_half_shell_with_random_passes(pkg, oracle):
+    """Self-MDDF of waters with three or more cells per dimension (the
+    half-shell walk of the molecule-pair kernel, both orientations from one
+    evaluation, FP64 parts resolved from the deferred queue) and ideal-gas
+    passes (the dbulk flags written for both orientations), against the
+    oracle."""
+    S = syn()
+    rng = np.random.default_rng(46)
+    box = 40.0
+    n = 2100
+    x = S.place_molecules(rng, S.water_template(), rng.uniform(0, box, (n, 3)))
+    xv = np.stack([x + rng.normal(scale=0.2, size=x.shape) for _ in range(3)])
+    sel = pkg.AtomSelection(list(range(1, 3 * n + 1)), natomspermol=3)
+    traj = pkg.MemoryTrajectory(xv, xv, np.repeat(np.diag([box] * 3)[None], 3, axis=0), sel, sel)
+    opts = pkg.Options(dbulk=10.0, n_random_samples=4, seed=47, silent=True)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts)
+    assert_raw_equal(a, b)
+
+
+def test_self_water_walk_split_does_not_change_results():
+    """Full-size self-MDDF water (100 A box, 33,400 molecules): one job over
+    8 frames walks one lane per molecule; 8 one-frame jobs split each
+    molecule's neighbour walk over several lanes.  The summed counts must be
+    identical (integer hits, unit weights)."""
+    from complexmixtures_amd import _lib
+    import complexmixtures_amd.serve as SV
+    from complexmixtures_amd.synthetic import served_system
+    nf = 8
+    s = served_system(7, 2000, 100.0, nf)
+    du = _lib.DeviceArray(s["x_solute"])
+    dw = _lib.DeviceArray(s["x_solvent"])
+    sysm = SV.System(n_solute=2000, n_water=s["n_water"], box=100.0, nframes=nf, d_solute=du, d_water=dw,
+                     cells=s["cells"], irefatom=SV.default_irefatom(s["x_solvent"][0]))
+    kw, _, _, _ = SV.job_kwargs(sysm, SV.Request(0.0, 0, nf, False, True, 99))
+    job = _lib.Job(**kw)
+    job.submit_device(np.arange(1, nf + 1), dw, dw, 0, s["cells"], np.ones(nf))
+    together = job.finish()
+    job.close()
+    parts = None
+    for f in range(nf):
+        job = _lib.Job(**kw)
+        job.submit_device(np.array([f + 1]), dw, dw, f, s["cells"][f:f + 1], np.ones(1))
+        raw = job.finish()
+        job.close()
+        parts = raw if parts is None else {k: parts[k] + raw[k] for k in raw}
+    du.free()
+    dw.free()
+    assert float(together["md_count"].sum()) > 1e6
+    for k in together:
+        if k == "volume_total":
+            assert np.isclose(together[k], parts[k], rtol=1e-14)
+        else:
+            assert np.array_equal(together[k], parts[k]), k
