@@ -1246,7 +1246,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
         int ks = 0, o = o_first + sub, kpos = 0, kmax = -1;
         int cell = 0, cs = 0, cnt = 0, code = -1, kx = 1, ky = 1, kz = 1;
         int sxs = 0, sys_ = 0, szs = 0;   // shift sets of the current cell (small grids)
-        float t1x = 0.f, t1y = 0.f, t1z = 0.f;
+        float t1x = 0.f, t1y = 0.f, t1z = 0.f;   // regular grids: own centre minus the cell's shift
         for (;;) {
             while (qn < 64 && ks < nsteps) {
                 if (kmax < 0) {
@@ -1268,6 +1268,12 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
                         nb_cell(cc[2], oz - 1, nc[2], mz);
                         code = (mx + 1) + 3 * (my + 1) + 9 * (mz + 1);
                         shift_of(gh, mx, my, mz, t1x, t1y, t1z);
+                        // the centre test below is a pre-filter with a margin
+                        // (rpre), so its rounding does not matter: the shift is
+                        // taken off the lane's own centre once per cell
+                        t1x = cv.x - t1x;
+                        t1y = cv.y - t1y;
+                        t1z = cv.z - t1z;
                     } else {
                         sxs = shifts_onto(cc[0], cx, nc[0]);
                         sys_ = shifts_onto(cc[1], cy, nc[1]);
@@ -1297,8 +1303,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
                     const int u = __float_as_int(cu4[ww].w);
                     bool pass = kpos + ww < cnt && (P.self_ ? ((simple && o > 13) || u < v) : true);
                     if (simple) {
-                        const float dx = cv.x - (cu4[ww].x + t1x), dy = cv.y - (cu4[ww].y + t1y),
-                                    dz = cv.z - (cu4[ww].z + t1z);
+                        const float dx = t1x - cu4[ww].x, dy = t1y - cu4[ww].y, dz = t1z - cu4[ww].z;
                         pass = pass && dx * dx + dy * dy + dz * dz <= gh.rpre2;
                     } else if (pass) {
                         bool nearby = false;
