@@ -62,9 +62,15 @@ class RequestTimes(ctypes.Structure):
 EXPORTS = ["cm_job_create", "cm_job_submit", "cm_job_submit_device", "cm_job_finish", "cm_job_set_profiling",
            "cm_job_set_tuning", "cm_job_kernel_stats", "cm_job_destroy", "cm_last_error", "cm_device_alloc", "cm_device_upload",
            "cm_device_free", "cm_device_synchronize", "cm_job_footprint", "cm_server_create", "cm_server_enqueue",
-           "cm_server_wait", "cm_server_destroy"]
+           "cm_server_wait", "cm_server_destroy", "cm_release_cached_memory"]
 
 _lib = None
+
+
+def release_cached_memory():
+    """Returns the device and page-locked host blocks that finished jobs left
+    cached for later jobs (cm_release_cached_memory)."""
+    check(lib().cm_release_cached_memory())
 
 
 class GPUError(RuntimeError):
@@ -98,6 +104,7 @@ def lib():
         L.cm_server_wait.argtypes = [ctypes.c_void_p, _u64, _P(RawOut), _P(RequestTimes)]
         L.cm_server_destroy.argtypes = [ctypes.c_void_p]
         L.cm_server_destroy.restype = None
+        L.cm_release_cached_memory.argtypes = []
         for f in ("cm_job_create", "cm_job_submit", "cm_job_submit_device", "cm_job_finish", "cm_job_set_profiling",
                   "cm_job_set_tuning", "cm_job_kernel_stats", "cm_device_alloc", "cm_device_upload", "cm_device_free",
                   "cm_device_synchronize", "cm_job_footprint", "cm_server_create", "cm_server_enqueue",

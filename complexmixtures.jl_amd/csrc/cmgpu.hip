@@ -51,6 +51,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -107,6 +108,7 @@ struct KP {                  // job parameters, by value
     int F;                   // frames in the batch
     int64_t Nu, Nv;
     int mpt;                 // molecules per chunk (tiles family)
+    int mol_lane;            // tiles family, 3-atom solvent molecules: one molecule per lane row
     int tcap;                // staged solute atoms per round (tiles family)
     int glds;                // molecule-pair kernels: group counts privatised in LDS
     int gnu, gnv;            // group counts of the solute / solvent side
@@ -1812,10 +1814,45 @@ __device__ __forceinline__ Rec rec_shfl_xor(const Rec& r, int o)
     return q;
 }
 
+// Column reduction of a running minimum over the C lanes of a row: DPP
+// quad permutations for the partners 1 and 2 lanes away (no LDS traffic),
+// a lane shuffle beyond.
+template <int O>
+__device__ __forceinline__ int xor_lane(int x)
+{
+    if constexpr (O == 1) return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xf, 0xf, false);   // quad_perm 1,0,3,2
+    else if constexpr (O == 2) return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xf, 0xf, false);   // quad_perm 2,3,0,1
+    else return __shfl_xor(x, O, 64);
+}
+
+template <int O>
+__device__ __forceinline__ void col_merge(Best& b)
+{
+    const float b2 = __int_as_float(xor_lane<O>(__float_as_int(b.best)));
+    const float s2 = __int_as_float(xor_lane<O>(__float_as_int(b.second)));
+    const int i2 = xor_lane<O>(b.bi);
+    const float ns = fminf(fmaxf(b.best, b2), fminf(b.second, s2));
+    if (b2 < b.best) b.bi = i2;
+    b.best = fminf(b.best, b2);
+    b.second = ns;
+}
+
+template <int C>
+__device__ __forceinline__ void col_reduce(Best& b)
+{
+    if constexpr (C >= 2) col_merge<1>(b);
+    if constexpr (C >= 4) col_merge<2>(b);
+    if constexpr (C >= 8) col_merge<4>(b);
+}
+
 // Minimum distances of the solvent molecules of one chunk per wavefront
-// (64 threads).  Lane = (atom row, column); a row is one solvent atom and
-// the C columns of a row split the staged solute atoms between them.
-template <class TU, class TV, bool RANDOM, int C>
+// (64 threads).  Lane = (row, column); the C columns of a row split the
+// staged solute atoms between them.  NVM = 0: a row is one solvent atom and
+// the atoms of a molecule are combined across rows.  NVM > 0 (molecules of
+// NVM atoms, e.g. water): a row is one molecule, each lane evaluates every
+// staged atom against all NVM atoms it holds in registers (one LDS read per
+// NVM pairs) and combines them in the lane.
+template <class TU, class TV, bool RANDOM, int C, int NVM>
 __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu, const TV* xv)
 {
     // dynamic LDS: the staging buffer (P.tcap float4), then the chunk table
@@ -1851,16 +1888,31 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
         const Geo& g = B.geo[f];
         const int m0 = (ch - B.chunk_base[fq]) * P.mpt;
         const int nm = min(P.mpt, B.nnear[fp] - m0);
-        const int ml = row / P.nv, a = row - (row / P.nv) * P.nv;
+        const int ml = NVM ? row : row / P.nv, a = NVM ? 0 : row - (row / P.nv) * P.nv;
         const bool active = ml < nm;
         const int v = active ? B.msorted[fp * P.Mv + m0 + ml] : 0;
         const float4* vwb = RANDOM ? B.vw_r + ((int64_t)f * P.K + pidx) * P.Nv : B.vw + (int64_t)f * P.Nv;
         const int64_t pbase = (int64_t)(m0 + ml) * P.nv;   // sorted position of the molecule
-        const float4 y = active ? vwb[pbase + a] : mk4(0.f, 0.f, 0.f, 0.f);
-        const float lx = wave_min(active ? y.x : INFINITY), ly = wave_min(active ? y.y : INFINITY),
-                    lz = wave_min(active ? y.z : INFINITY);
-        const float hx = wave_max(active ? y.x : -INFINITY), hy = wave_max(active ? y.y : -INFINITY),
-                    hz = wave_max(active ? y.z : -INFINITY);
+        float4 y = mk4(0.f, 0.f, 0.f, 0.f);
+        float4 ym[NVM > 0 ? NVM : 1];   // molecule per lane: its atoms
+        float alx, aly, alz, ahx, ahy, ahz;   // box of the lane's atoms
+        if constexpr (NVM > 0) {
+#pragma unroll
+            for (int q = 0; q < NVM; q++) ym[q] = active ? vwb[pbase + q] : mk4(0.f, 0.f, 0.f, 0.f);
+            alx = ahx = ym[0].x; aly = ahy = ym[0].y; alz = ahz = ym[0].z;
+#pragma unroll
+            for (int q = 1; q < NVM; q++) {
+                alx = fminf(alx, ym[q].x); aly = fminf(aly, ym[q].y); alz = fminf(alz, ym[q].z);
+                ahx = fmaxf(ahx, ym[q].x); ahy = fmaxf(ahy, ym[q].y); ahz = fmaxf(ahz, ym[q].z);
+            }
+        } else {
+            y = active ? vwb[pbase + a] : mk4(0.f, 0.f, 0.f, 0.f);
+            alx = ahx = y.x; aly = ahy = y.y; alz = ahz = y.z;
+        }
+        const float lx = wave_min(active ? alx : INFINITY), ly = wave_min(active ? aly : INFINITY),
+                    lz = wave_min(active ? alz : INFINITY);
+        const float hx = wave_max(active ? ahx : -INFINITY), hy = wave_max(active ? ahy : -INFINITY),
+                    hz = wave_max(active ? ahz : -INFINITY);
         const float rs = g.rstage, rs2 = rs * rs;
         int ilo[3], ihi[3];
         cell_range(g, lx, ly, lz, hx, hy, hz, rs, ilo, ihi);
@@ -1870,11 +1922,40 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
         const float4* srt = B.sorted + (int64_t)f * B.nitems;
         Best bb;
         bb.init();
+        Best bq[NVM > 0 ? NVM : 1];   // molecule per lane: each atom's running minimum
+#pragma unroll
+        for (int q = 0; q < (NVM > 0 ? NVM : 1); q++) bq[q].init();
         int rounds = 0;
         int nst = 0;   // staged atoms (uniform)
         const unsigned long long lt_mask = (1ull << lane) - 1ull;
         auto evaluate = [&](int n) {
             __syncthreads();
+            if constexpr (NVM > 0) {
+                int k = col;
+                for (; k + C < n; k += 2 * C) {
+                    const float4 s0 = stg[k], s1 = stg[k + C];
+#pragma unroll
+                    for (int q = 0; q < NVM; q++) {
+                        const float dx = ym[q].x - s0.x, dy = ym[q].y - s0.y, dz = ym[q].z - s0.z;
+                        bq[q].offer(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), __float_as_int(s0.w), 0);
+                    }
+#pragma unroll
+                    for (int q = 0; q < NVM; q++) {
+                        const float dx = ym[q].x - s1.x, dy = ym[q].y - s1.y, dz = ym[q].z - s1.z;
+                        bq[q].offer(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), __float_as_int(s1.w), 0);
+                    }
+                }
+                if (k < n) {
+                    const float4 s0 = stg[k];
+#pragma unroll
+                    for (int q = 0; q < NVM; q++) {
+                        const float dx = ym[q].x - s0.x, dy = ym[q].y - s0.y, dz = ym[q].z - s0.z;
+                        bq[q].offer(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), __float_as_int(s0.w), 0);
+                    }
+                }
+                __syncthreads();
+                return;
+            }
             int k = col;
             for (; k + 3 * C < n; k += 4 * C) {
                 float4 s0 = stg[k], s1 = stg[k + C], s2 = stg[k + 2 * C], s3 = stg[k + 3 * C];
@@ -1995,31 +2076,43 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
         const bool one_round = rounds == 0;
         const int last_n = nst;   // the last staged batch stays in LDS until the next chunk
         if (last_n > 0) evaluate(last_n);
-        // columns of a row -> the row's atom
-        for (int o = 1; o < C; o <<= 1) {
-            float b2 = __shfl_xor(bb.best, o, 64), s2 = __shfl_xor(bb.second, o, 64);
-            int i2 = __shfl_xor(bb.bi, o, 64);
-            float ns = fminf(fmaxf(bb.best, b2), fminf(bb.second, s2));
-            if (b2 < bb.best) bb.bi = i2;
-            bb.best = fminf(bb.best, b2);
-            bb.second = ns;
+        float mb, ms, rb, rsec;
+        int mi, ma, ri;
+        if constexpr (NVM > 0) {
+            // columns -> each atom's minimum; atoms -> the molecule, in the lane
+#pragma unroll
+            for (int q = 0; q < NVM; q++) col_reduce<C>(bq[q]);
+            mb = bq[0].best; ms = bq[0].second; mi = bq[0].bi; ma = 0;
+            rb = bq[0].best; rsec = bq[0].second; ri = bq[0].bi;
+#pragma unroll
+            for (int q = 1; q < NVM; q++) {
+                const float ns = fminf(fmaxf(mb, bq[q].best), fminf(ms, bq[q].second));
+                if (bq[q].best < mb) { mi = bq[q].bi; ma = q; }
+                mb = fminf(mb, bq[q].best);
+                ms = ns;
+                if (q == P.iref) { rb = bq[q].best; rsec = bq[q].second; ri = bq[q].bi; }
+            }
+        } else {
+            // columns of a row -> the row's atom
+            col_reduce<C>(bb);
+            // atoms of a molecule -> the molecule (leader lane: atom 0, column 0)
+            const int mrow0 = ml * P.nv;
+            mb = bb.best; ms = bb.second;
+            mi = bb.bi; ma = 0;
+            for (int q = 1; q < P.nv; q++) {
+                const int src = ((mrow0 + q) * C) & 63;
+                float b2 = __shfl(bb.best, src, 64), s2 = __shfl(bb.second, src, 64);
+                int i2 = __shfl(bb.bi, src, 64);
+                float ns = fminf(fmaxf(mb, b2), fminf(ms, s2));
+                if (b2 < mb) { mi = i2; ma = q; }
+                mb = fminf(mb, b2);
+                ms = ns;
+            }
+            const int srcr = ((mrow0 + P.iref) * C) & 63;
+            rb = __shfl(bb.best, srcr, 64);
+            rsec = __shfl(bb.second, srcr, 64);
+            ri = __shfl(bb.bi, srcr, 64);
         }
-        // atoms of a molecule -> the molecule (leader lane: atom 0, column 0)
-        const int mrow0 = ml * P.nv;
-        float mb = bb.best, ms = bb.second;
-        int mi = bb.bi, ma = 0;
-        for (int q = 1; q < P.nv; q++) {
-            const int src = ((mrow0 + q) * C) & 63;
-            float b2 = __shfl(bb.best, src, 64), s2 = __shfl(bb.second, src, 64);
-            int i2 = __shfl(bb.bi, src, 64);
-            float ns = fminf(fmaxf(mb, b2), fminf(ms, s2));
-            if (b2 < mb) { mi = i2; ma = q; }
-            mb = fminf(mb, b2);
-            ms = ns;
-        }
-        const int srcr = ((mrow0 + P.iref) * C) & 63;
-        const float rb = __shfl(bb.best, srcr, 64), rsec = __shfl(bb.second, srcr, 64);
-        const int ri = __shfl(bb.bi, srcr, 64);
         const bool leader = active && a == 0 && col == 0;
         const TU* xuf = xu + (int64_t)f * P.Nu * 3;
         const int64_t jbase = (int64_t)v * P.nv;
@@ -2070,14 +2163,20 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
             const int L = __ffsll((long long)amask) - 1;
             amask &= amask - 1ull;
             const float bandm = __shfl(mb, L, 64) + g.amb, bandr = __shfl(rb, L, 64) + g.amb;
-            const int mlL = (L / C) / P.nv;
+            const int mlL = NVM ? L / C : (L / C) / P.nv;
             const int vL = __shfl(v, L, 64);
             const int64_t jb = (int64_t)vL * P.nv;
             const int64_t pb = (int64_t)(m0 + mlL) * P.nv;
             Rec loc = rec_zero();
             for (int q = 0; q < P.nv; q++) {
-                const int sl = ((mlL * P.nv + q) * C) & 63;
-                const float yx = __shfl(y.x, sl, 64), yy = __shfl(y.y, sl, 64), yz = __shfl(y.z, sl, 64);
+                const int sl = NVM ? (mlL * C) & 63 : ((mlL * P.nv + q) * C) & 63;
+                float4 yq = y;
+                if constexpr (NVM > 0) {
+#pragma unroll
+                    for (int t = 0; t < NVM; t++)
+                        if (t == q) yq = ym[t];
+                }
+                const float yx = __shfl(yq.x, sl, 64), yy = __shfl(yq.y, sl, 64), yz = __shfl(yq.z, sl, 64);
                 double Y0, Y1, Y2;
                 if (RANDOM) {
                     const double* pp = B.P + (((int64_t)f * P.K + pidx) * P.Nv + pb + q) * 3;
@@ -2178,17 +2277,153 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// Memory released by finished jobs, kept for the next ones (a process that
+// runs one mddf call after another would otherwise pay hipMalloc/hipFree,
+// which synchronise the device, for about 1 GB of batch scratch and tens of
+// MB of page-locked read-back buffers per call).  Blocks are reused for
+// requests of at least half their size; beyond a cap the largest are
+// released.  Heap singletons: never destroyed, so nothing runs at exit.
+struct BlockCache {
+    std::mutex m;
+    std::multimap<size_t, std::pair<int, void*>> blocks;   // bytes -> (device, pointer)
+    size_t total = 0;
+};
+
+BlockCache& dev_cache()
+{
+    static BlockCache* c = new BlockCache;
+    return *c;
+}
+
+BlockCache& pin_cache()
+{
+    static BlockCache* c = new BlockCache;
+    return *c;
+}
+
+constexpr size_t DEV_CACHE_MAX = (size_t)48 << 30;
+constexpr size_t PIN_CACHE_MAX = (size_t)4 << 30;
+
+void* cache_take(BlockCache& c, size_t bytes, int dev, size_t* got)
+{
+    std::lock_guard<std::mutex> lk(c.m);
+    for (auto it = c.blocks.lower_bound(bytes); it != c.blocks.end() && it->first <= 2 * bytes; ++it) {
+        if (it->second.first != dev) continue;
+        void* p = it->second.second;
+        *got = it->first;
+        c.total -= it->first;
+        c.blocks.erase(it);
+        return p;
+    }
+    return nullptr;
+}
+
+// returns a block; blocks beyond the cap (largest first) come back to be freed
+std::vector<std::pair<int, void*>> cache_put(BlockCache& c, void* p, size_t bytes, int dev, size_t cap)
+{
+    std::vector<std::pair<int, void*>> out;
+    std::lock_guard<std::mutex> lk(c.m);
+    c.blocks.emplace(bytes, std::make_pair(dev, p));
+    c.total += bytes;
+    while (c.total > cap && !c.blocks.empty()) {
+        auto it = std::prev(c.blocks.end());
+        c.total -= it->first;
+        out.push_back(it->second);
+        c.blocks.erase(it);
+    }
+    return out;
+}
+
+std::vector<std::pair<int, void*>> cache_drain(BlockCache& c)
+{
+    std::lock_guard<std::mutex> lk(c.m);
+    std::vector<std::pair<int, void*>> out;
+    for (auto& kv : c.blocks) out.push_back(kv.second);
+    c.blocks.clear();
+    c.total = 0;
+    return out;
+}
+
+int current_device()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    return dev;
+}
+
+// device block of at least `bytes`: a cached one, else hipMalloc (after
+// releasing the cache when the device is full)
+int dev_take(DevBuf& b, size_t bytes)
+{
+    const int dev = current_device();
+    size_t got = 0;
+    if (void* p = cache_take(dev_cache(), bytes, dev, &got)) {
+        b.p = p;
+        b.bytes = got;
+        return CM_OK;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        for (auto& blk : cache_drain(dev_cache())) (void)hipFree(blk.second);
+        e = hipMalloc(&b.p, bytes);
+    }
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(CM_ENOMEM, std::string("hipMalloc failed for ") + std::to_string(bytes) + " bytes");
+    }
+    b.bytes = bytes;
+    return CM_OK;
+}
+
+// a block no stream uses any more goes back to the cache
+void dev_put(DevBuf& b)
+{
+    if (!b.p) return;
+    for (auto& blk : cache_put(dev_cache(), b.p, b.bytes, current_device(), DEV_CACHE_MAX)) (void)hipFree(blk.second);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+struct PinBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+int pin_take(PinBuf& b, size_t bytes)
+{
+    size_t got = 0;
+    if (void* p = cache_take(pin_cache(), bytes, 0, &got)) {
+        b.p = p;
+        b.bytes = got;
+        return CM_OK;
+    }
+    if (hipHostMalloc(&b.p, bytes, hipHostMallocDefault) != hipSuccess) {
+        b.p = nullptr;
+        return fail(CM_ENOMEM, std::string("hipHostMalloc failed for ") + std::to_string(bytes) + " bytes");
+    }
+    b.bytes = bytes;
+    return CM_OK;
+}
+
+void pin_put(PinBuf& b)
+{
+    if (!b.p) return;
+    for (auto& blk : cache_put(pin_cache(), b.p, b.bytes, 0, PIN_CACHE_MAX)) (void)hipHostFree(blk.second);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+// grows a job buffer; a block being replaced may still be read by the job's
+// stream, so it is freed (hipFree waits for the device), not cached
 int ensure(DevBuf& b, size_t bytes)
 {
     if (bytes == 0) bytes = 16;
     if (b.bytes >= bytes) return CM_OK;
-    if (b.p) hipFree(b.p);
+    if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.bytes = 0;
-    hipError_t e = hipMalloc(&b.p, bytes);
-    if (e != hipSuccess) return fail(CM_ENOMEM, std::string("hipMalloc failed for ") + std::to_string(bytes) + " bytes");
-    b.bytes = bytes;
-    return CM_OK;
+    return dev_take(b, bytes);
 }
 
 void plane_spacings(const double* H, double sp[3])
@@ -2236,8 +2471,7 @@ struct cm_job {
     std::vector<Geo> hgeo, hgeo_out;
     std::vector<int> huk, hdp, hdi;
     std::vector<int32_t> hgm;        // group maps (host copy of the upload)
-    std::vector<double> hacc;        // accumulators read back at finish
-    std::vector<ull> hhits;
+    PinBuf hres;                     // accumulators and hit counts read back at finish (page-locked)
 };
 
 namespace {
@@ -2286,7 +2520,8 @@ void launch_tiles_one(hipStream_t s, int grid, int cus, const KP& P, const Bufs&
 {
     const int nfp = P.F * (RANDOM ? P.K : 1);
     const size_t shm = sizeof(float4) * (size_t)P.tcap + sizeof(int) * (size_t)std::min(nfp + 1, CBCAP);
-    auto kern = k_eval_tiles<TU, TV, RANDOM, C>;
+    // 3-atom solvent molecules: one molecule per lane
+    auto kern = P.mol_lane ? k_eval_tiles<TU, TV, RANDOM, C, 3> : k_eval_tiles<TU, TV, RANDOM, C, 0>;
     if (grid <= 0) {
         int nb = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 64, shm) != hipSuccess || nb < 1) nb = 1;
@@ -2809,10 +3044,13 @@ void job_params(cm_job* J, const cm_job_desc& d)
     P.Nu = Nu;
     P.Nv = Nv;
     P.NP = 1 + J->K;
-    // one column: every lane of a wavefront a solvent atom (the per-chunk
-    // costs of smaller chunks outweigh their shorter staged lists)
-    J->tile_c = 1;
-    P.mpt = 64 / P.nv;
+    // 3-atom solvent molecules: one molecule per lane row, 4 columns (16
+    // molecules per chunk); otherwise one column, every lane a solvent atom
+    // (the per-chunk costs of smaller chunks outweigh their shorter staged
+    // lists)
+    P.mol_lane = P.nv == 3 ? 1 : 0;
+    J->tile_c = P.mol_lane ? 4 : 1;
+    P.mpt = P.mol_lane ? 64 / J->tile_c : 64 / P.nv;
     P.tcap = TCAP_DEFAULT;
     P.gnu = d.n_groups_u;
     P.gnv = d.n_groups_v;
@@ -2868,7 +3106,7 @@ int job_setup(cm_job* J, const cm_job_desc& d)
     return CM_OK;
 }
 
-// flushes the pending weight and reads the accumulators back into J->hacc / J->hhits
+// flushes the pending weight and reads the accumulators and hits back into J->hres
 int job_collect(cm_job* J)
 {
     if (J->pending) {
@@ -2876,21 +3114,26 @@ int job_collect(cm_job* J)
         if (r) return r;
         J->pending = false;
     }
-    J->hacc.resize(J->acc_len);
-    J->hhits.resize(4 * (size_t)J->d.nbins);
-    HIPCHK(hipMemcpyAsync(J->hacc.data(), J->acc.p, 8 * J->acc_len, hipMemcpyDeviceToHost, J->stream));
-    HIPCHK(hipMemcpyAsync(J->hhits.data(), J->hits.p, 8 * 4 * J->d.nbins, hipMemcpyDeviceToHost, J->stream));
+    const size_t need = 8 * (size_t)J->acc_len + 8 * 4 * (size_t)J->d.nbins;
+    if (J->hres.bytes < need) {
+        pin_put(J->hres);
+        int r = pin_take(J->hres, need);
+        if (r) return r;
+    }
+    HIPCHK(hipMemcpyAsync(J->hres.p, J->acc.p, 8 * J->acc_len, hipMemcpyDeviceToHost, J->stream));
+    HIPCHK(hipMemcpyAsync((char*)J->hres.p + 8 * J->acc_len, J->hits.p, 8 * 4 * J->d.nbins, hipMemcpyDeviceToHost,
+                          J->stream));
     HIPCHK(hipStreamSynchronize(J->stream));
     J->finished = true;
     return CM_OK;
 }
 
 // raw accumulators of a collected job into the caller's cm_raw
-void fill_raw(const cm_job_desc& d, const std::vector<double>& acc, const std::vector<ull>& hits, double volume,
-              cm_raw* out)
+void fill_raw(const cm_job_desc& d, int64_t acc_len, const PinBuf& res, double volume, cm_raw* out)
 {
     const int64_t nb = d.nbins, ngu = d.n_groups_u, ngv = d.n_groups_v;
-    const double* a = acc.data();
+    const double* a = (const double*)res.p;
+    const ull* hits = (const ull*)((const char*)res.p + 8 * acc_len);
     auto cp = [&](double* dst, const double* src, int64_t n) { if (dst) memcpy(dst, src, 8 * n); };
     cp(out->md_count, a, nb);
     cp(out->md_count_random, a + nb, nb);
@@ -2903,10 +3146,10 @@ void fill_raw(const cm_job_desc& d, const std::vector<double>& acc, const std::v
     cp(out->solvent_group_count_random, g + (2 * ngu + ngv) * nb, ngv * nb);
     out->volume_total = volume;
     auto cpu = [&](uint64_t* dst, const ull* src) { if (dst) memcpy(dst, src, 8 * nb); };
-    cpu(out->hits_md, hits.data());
-    cpu(out->hits_md_random, hits.data() + nb);
-    cpu(out->hits_rdf, hits.data() + 2 * nb);
-    cpu(out->hits_rdf_random, hits.data() + 3 * nb);
+    cpu(out->hits_md, hits);
+    cpu(out->hits_md_random, hits + nb);
+    cpu(out->hits_rdf, hits + 2 * nb);
+    cpu(out->hits_rdf_random, hits + 3 * nb);
 }
 
 }  // namespace
@@ -2952,7 +3195,7 @@ int cm_job_finish(cm_job* J, cm_raw* out)
     if (J->finished) return fail(CM_ESTATE, "job already finished");
     int r = job_collect(J);
     if (r) return r;
-    fill_raw(J->d, J->hacc, J->hhits, J->volume_total, out);
+    fill_raw(J->d, J->acc_len, J->hres, J->volume_total, out);
     return CM_OK;
 }
 
@@ -2976,11 +3219,11 @@ int cm_job_set_tuning(cm_job* J, double cell_fraction, int32_t columns, int32_t 
     if (columns != 0 && columns != 1 && columns != 2 && columns != 4 && columns != 8)
         return fail(CM_EINVAL, "columns must be 0 (default), 1, 2, 4 or 8");
     if (columns == 0) columns = J->tile_c;
-    if ((64 / columns) / J->kp.nv < 1) return fail(CM_EINVAL, "too many columns for the solvent molecule size");
+    if (!J->kp.mol_lane && (64 / columns) / J->kp.nv < 1) return fail(CM_EINVAL, "too many columns for the solvent molecule size");
     if (eval_workgroups < 0) return fail(CM_EINVAL, "eval_workgroups must be >= 0");
     J->cell_frac = cell_fraction;
     J->tile_c = columns;
-    J->kp.mpt = (64 / columns) / J->kp.nv;
+    J->kp.mpt = J->kp.mol_lane ? 64 / columns : (64 / columns) / J->kp.nv;
     J->eval_grid = eval_workgroups;
     return CM_OK;
 }
@@ -3010,12 +3253,20 @@ void cm_job_destroy(cm_job* J)
                       &J->vw_r, &J->vcell_r, &J->vcen_r, &J->bflag, &J->blist, &J->nbulk, &J->uk, &J->drawn_ptr,
                       &J->drawn_idx, &J->xin_u, &J->xin_v, &J->near, &J->mkey, &J->mcnt, &J->mstart,
                       &J->msorted, &J->nnear, &J->chunk_base};
-    for (DevBuf* b : bufs) if (b->p) hipFree(b->p);
+    for (DevBuf* b : bufs) dev_put(*b);
+    pin_put(J->hres);
     if (J->e0) hipEventDestroy(J->e0);
     if (J->e1) hipEventDestroy(J->e1);
     if (J->e2) hipEventDestroy(J->e2);
     if (J->stream) hipStreamDestroy(J->stream);
     delete J;
+}
+
+int cm_release_cached_memory(void)
+{
+    for (auto& blk : cache_drain(dev_cache())) HIPCHK(hipFree(blk.second));
+    for (auto& blk : cache_drain(pin_cache())) HIPCHK(hipHostFree(blk.second));
+    return CM_OK;
 }
 
 int cm_device_alloc(size_t bytes, void** dptr)
@@ -3067,10 +3318,11 @@ struct SrvReq {
     bool done = false;
     int status = CM_OK;
     std::string err;
-    std::vector<double> acc;
-    std::vector<ull> hits;
+    PinBuf res;                      // read-back accumulators (taken over from the slot's job)
+    int64_t acc_len = 0;
     double volume = 0.0;
     cm_request_times t{};
+    ~SrvReq() { pin_put(res); }
 };
 
 int64_t job_footprint(const cm_job_desc& d, int32_t nframes, int32_t dtype, const double* cells, int64_t* bytes)
@@ -3122,8 +3374,9 @@ int run_request(cm_job* J, SrvReq* q)
     if ((r = submit_common(J, q->nframes, q->fi.data(), q->xu, q->xv, q->dtype, q->cells.data(), q->w.data(), false)))
         return r;
     if ((r = job_collect(J))) return r;
-    q->acc = J->hacc;
-    q->hits = J->hhits;
+    q->res = J->hres;
+    J->hres = PinBuf{};
+    q->acc_len = J->acc_len;
     q->volume = J->volume_total;
     return CM_OK;
 }
@@ -3276,7 +3529,7 @@ int cm_server_wait(cm_server* S, uint64_t ticket, cm_raw* out, cm_request_times*
     if (times) *times = q->t;
     const int st = q->status;
     if (st == CM_OK) {
-        if (out) fill_raw(q->d, q->acc, q->hits, q->volume, out);
+        if (out) fill_raw(q->d, q->acc_len, q->res, q->volume, out);
     } else {
         g_err = q->err;
     }

@@ -13,7 +13,7 @@ systems whose solute size is log-uniform over 200-20,000 atoms in cubic water
 boxes of 30-150 A, 10-200 frames per request, 30% with the ideal-gas reference
 passes (n_random_samples 10; the others are coordination-number requests) and
 20% self-MDDF (water vs water).  The autotuner picks the cell size and the
-tile (columns per atom row) of the single-solute kernels per system by timing
+tile (columns per row) of the single-solute kernels per system by timing
 them on a few frames.
 """
 import hashlib
@@ -217,13 +217,23 @@ def service_breakdown(reqs, service):
     return out
 
 
-TUNING_CANDIDATES = [dict(cell_fraction=cf, columns=c) for cf in (0.35, 0.5, 0.75, 1.0) for c in (1, 2)]
+def tuning_candidates(natoms_per_mol_v):
+    """Cell fractions x columns per row: a row is a whole molecule for 3-atom
+    solvent molecules (2-8 columns: 32-8 molecules per chunk), one solvent
+    atom otherwise (1-2 columns)."""
+    cols = (2, 4, 8) if natoms_per_mol_v == 3 else (1, 2)
+    return [dict(cell_fraction=cf, columns=c) for cf in (0.35, 0.5, 0.75, 1.0) for c in cols]
 
 
-def autotune_job(job_kw, d_solute, d_solvent, cells, frames=4, candidates=TUNING_CANDIDATES):
+TUNING_CANDIDATES = tuning_candidates(3)
+
+
+def autotune_job(job_kw, d_solute, d_solvent, cells, frames=4, candidates=None):
     """Times the single-solute kernels of one job description on its first
     frames for each candidate tuning (kernel time from HIP events) and returns
     (best tuning, [(ms, tuning)]); every candidate must give the same counts."""
+    if candidates is None:
+        candidates = tuning_candidates(job_kw["natoms_per_mol_v"])
     nf = min(frames, d_solute.shape[0])
     best, ref, timing = None, None, []
     for cand in candidates:
@@ -244,7 +254,7 @@ def autotune_job(job_kw, d_solute, d_solvent, cells, frames=4, candidates=TUNING
     return dict(best[1]), timing
 
 
-def autotune(system, frames=4, candidates=TUNING_CANDIDATES):
+def autotune(system, frames=4, candidates=None):
     """Autotunes the cross (single-solute) requests of a system with the
     ideal-gas passes on and keeps the result in system.tuning."""
     req = Request(arrival=0.0, system=0, nframes=min(frames, system.nframes), mddf=True, self_=False, seed=12345)

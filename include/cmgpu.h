@@ -131,10 +131,11 @@ int cm_job_finish(cm_job *job, cm_raw *out);
 
 /* Tuning of the single-solute kernels (no counterpart in the reference; the
  * reference's Options.lcell plays this role for CellListMap): fine cell side
- * as a fraction of the cutoff (0.2-1.0), solute-atom columns per solvent atom
- * row of a wavefront (1, 2, 4 or 8; 0 keeps the default, 1) and workgroups of
- * the persistent evaluation kernel (0 = as many as the device holds at once).
- * Results do not depend on these. */
+ * as a fraction of the cutoff (0.2-1.0), columns per row of a wavefront (1, 2,
+ * 4 or 8; a row is one solvent atom, or one whole molecule for 3-atom solvent
+ * molecules; 0 keeps the default: 4 for 3-atom molecules, else 1) and
+ * workgroups of the persistent evaluation kernel (0 = as many as the device
+ * holds at once).  Results do not depend on these. */
 int cm_job_set_tuning(cm_job *job, double cell_fraction, int32_t columns, int32_t eval_workgroups);
 
 /* Kernel timing of the job (HIP events on the job's stream). */
@@ -145,6 +146,12 @@ void cm_job_destroy(cm_job *job);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *cm_last_error(void);
+
+/* Device blocks and page-locked host buffers of destroyed jobs are kept for
+ * later jobs of the process (a job of config-2 size holds about 1 GB of batch
+ * scratch; allocating and freeing it per call synchronises the device).  This
+ * returns them to the system.  No counterpart in the reference. */
+int cm_release_cached_memory(void);
 
 /* Device memory for callers that keep trajectories resident in HBM. */
 int cm_device_alloc(size_t bytes, void **dptr);

@@ -544,3 +544,19 @@ def test_self_water_walk_split_does_not_change_results():
             assert np.isclose(together[k], parts[k], rtol=1e-14)
         else:
             assert np.array_equal(together[k], parts[k]), k
+
+
+def test_cached_memory_reuse_and_release(pkg):
+    """Jobs take the device and page-locked blocks that earlier jobs left
+    cached; the counts do not depend on whether a block was reused or fresh
+    (released in between)."""
+    from complexmixtures_amd import _lib
+    s = syn().config1(nframes=3)
+    traj = mem(pkg, s)
+    opts = pkg.Options(dbulk=10.0, n_random_samples=3, seed=48, silent=True)
+    a = gpu_raw(pkg, traj, opts)
+    b = gpu_raw(pkg, traj, opts)
+    _lib.release_cached_memory()
+    c = gpu_raw(pkg, traj, opts)
+    assert_raw_equal(a, b)
+    assert_raw_equal(a, c)
