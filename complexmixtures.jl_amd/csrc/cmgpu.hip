@@ -704,6 +704,11 @@ __device__ __forceinline__ float fmin_sel(float a, float b) { return b < a ? b :
 __device__ __forceinline__ float fmax_sel(float a, float b) { return b > a ? b : a; }
 
 // minimum / maximum over the wavefront, the same in every lane
+// a wave-uniform float moved to a scalar register
+__device__ __forceinline__ float sgpr_f(float x)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
 __device__ __forceinline__ float wave_min(float x)
 {
     const int id = __float_as_int(INFINITY);
@@ -1513,6 +1518,7 @@ __global__ void k_place(KP P, Bufs B, const TV* xv)
 // tiles family: one solute molecule, solvent molecules of up to 64 atoms
 // ---------------------------------------------------------------------------
 constexpr int TCAP_DEFAULT = 640;   // staged solute atoms per round (10 KiB of LDS)
+constexpr int TCAP_MOL = 512;       // the same with the squared norms (20 B each: 10 KiB)
 constexpr int CBCAP = 1024;         // chunk-table entries kept in LDS
 constexpr int SU = 4;               // staging: ranks per lane per step
 
@@ -1855,12 +1861,18 @@ __device__ __forceinline__ void col_reduce(Best& b)
 template <class TU, class TV, bool RANDOM, int C, int NVM>
 __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu, const TV* xv)
 {
-    // dynamic LDS: the staging buffer (P.tcap float4), then the chunk table
-    // of this launch when it has at most CBCAP entries
+    // dynamic LDS: the staging buffer (P.tcap float4: atom, index; relative
+    // to the chunk centre when NVM > 0), with NVM > 0 the squared norms of the
+    // staged atoms (P.tcap float), then the chunk table of this launch when it
+    // has at most CBCAP entries
     extern __shared__ float4 dyn[];
     float4* stg = dyn;
-    int* cbs = (int*)(dyn + P.tcap);
+    float* stw = (float*)(dyn + P.tcap);
+    int* cbs = (int*)(stw + (NVM > 0 ? P.tcap : 0));
     const int tcap = P.tcap;
+    // staging ranks per lane per step (fewer with a molecule per lane, whose
+    // evaluation holds more registers)
+    constexpr int SUk = NVM > 0 ? 2 : SU;
     const int lane = threadIdx.x;
     const int row = lane / C, col = lane - (lane / C) * C;
     const int p0 = RANDOM ? 1 : 0;
@@ -1914,6 +1926,35 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
         const float hx = wave_max(active ? ahx : -INFINITY), hy = wave_max(active ? ahy : -INFINITY),
                     hz = wave_max(active ? ahz : -INFINITY);
         const float rs = g.rstage, rs2 = rs * rs;
+        // NVM > 0: distances relative to the chunk centre c.  With y' = y - c
+        // and s' = s - c, |y' - s'|^2 = (|s'|^2 - 2 y'.s') + |y'|^2: three FMAs
+        // per pair against the staged (s', |s'|^2), |y'|^2 added once per atom.
+        // With u = 2^-24, Y >= |y'| and S >= |s'| for every staged atom, the
+        // rounding of the c subtractions moves y' - s' by at most u (Y + S),
+        // so d^2 by at most 2 u (Y + S) d (d <= cutsys + 0.5 where it
+        // matters); the norms (3 u S^2, 3 u Y^2), the FMA chain (3 u (S^2 +
+        // 2 Y S)) and the last sum (u (Y + S)^2) add at most 7 u (Y + S)^2.
+        // Both widen the frame's guard for this chunk.
+        // The molecule's atoms are kept as -2 y' (exact scaling of y').
+        // The per-chunk values are wave-uniform and held in scalar registers.
+        float ccx = 0.f, ccy = 0.f, ccz = 0.f;
+        float Gcf = (float)g.guard, cut2_hi_c = g.cut2_hi, amb_c = g.amb, G32c = g.guard32;
+        if constexpr (NVM > 0) {
+            ccx = sgpr_f(0.5f * (lx + hx)); ccy = sgpr_f(0.5f * (ly + hy)); ccz = sgpr_f(0.5f * (lz + hz));
+            const double ex = (double)hx - lx, ey = (double)hy - ly, ez = (double)hz - lz;
+            // (+ the rounding of c itself, at most 2^-24 |c| per component)
+            const double Yb = 0.5 * sqrt(ex * ex + ey * ey + ez * ez) * (1.0 + 1e-6) +
+                              1e-6 * ((double)fabsf(ccx) + fabsf(ccy) + fabsf(ccz)) + 1e-5;
+            const double Sb = Yb + (double)rs * (1.0 + 1e-5) + 1e-4;
+            const double Gc = g.guard + 0x1p-24 * (Yb + Sb) * (2.0 * (P.cutsys + 0.5) + 7.0 * (Yb + Sb)) + 1e-9;
+            Gcf = sgpr_f(nextafterf((float)Gc, INFINITY));
+            cut2_hi_c = sgpr_f(nextafterf((float)(P.cut2 + Gc), INFINITY));
+            amb_c = sgpr_f(nextafterf((float)(2.0 * Gc) + 8.0f * (float)(halfulp32((double)cut2_hi_c) * 2.0), INFINITY));
+            G32c = sgpr_f(nextafterf((float)(Gc + (P.cut2 + Gc + 1.0) * 0x1p-21), INFINITY));
+#pragma unroll
+            for (int q = 0; q < NVM; q++)
+                ym[q] = mk4(-2.0f * (ym[q].x - ccx), -2.0f * (ym[q].y - ccy), -2.0f * (ym[q].z - ccz), ym[q].w);
+        }
         int ilo[3], ihi[3];
         cell_range(g, lx, ly, lz, hx, hy, hz, rs, ilo, ihi);
         const int nx = ihi[0] - ilo[0] + 1, ny = ihi[1] - ilo[1] + 1, nz = ihi[2] - ilo[2] + 1;
@@ -1931,27 +1972,27 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
         auto evaluate = [&](int n) {
             __syncthreads();
             if constexpr (NVM > 0) {
+                // |s'|^2 - 2 y'.s' per atom of the molecule (|y'|^2 added after
+                // the last round); four staged atoms per step, loads first
                 int k = col;
-                for (; k + C < n; k += 2 * C) {
-                    const float4 s0 = stg[k], s1 = stg[k + C];
+                for (; k + 3 * C < n; k += 4 * C) {
+                    float4 sv[4];
+                    float wv[4];
 #pragma unroll
-                    for (int q = 0; q < NVM; q++) {
-                        const float dx = ym[q].x - s0.x, dy = ym[q].y - s0.y, dz = ym[q].z - s0.z;
-                        bq[q].offer(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), __float_as_int(s0.w), 0);
-                    }
+                    for (int u = 0; u < 4; u++) { sv[u] = stg[k + u * C]; wv[u] = stw[k + u * C]; }
 #pragma unroll
-                    for (int q = 0; q < NVM; q++) {
-                        const float dx = ym[q].x - s1.x, dy = ym[q].y - s1.y, dz = ym[q].z - s1.z;
-                        bq[q].offer(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), __float_as_int(s1.w), 0);
-                    }
+                    for (int u = 0; u < 4; u++)
+#pragma unroll
+                        for (int q = 0; q < NVM; q++)
+                            bq[q].offer(fmaf(ym[q].x, sv[u].x, fmaf(ym[q].y, sv[u].y, fmaf(ym[q].z, sv[u].z, wv[u]))),
+                                        __float_as_int(sv[u].w), 0);
                 }
-                if (k < n) {
+                for (; k < n; k += C) {
                     const float4 s0 = stg[k];
+                    const float w0 = stw[k];
 #pragma unroll
-                    for (int q = 0; q < NVM; q++) {
-                        const float dx = ym[q].x - s0.x, dy = ym[q].y - s0.y, dz = ym[q].z - s0.z;
-                        bq[q].offer(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), __float_as_int(s0.w), 0);
-                    }
+                    for (int q = 0; q < NVM; q++)
+                        bq[q].offer(fmaf(ym[q].x, s0.x, fmaf(ym[q].y, s0.y, fmaf(ym[q].z, s0.z, w0))), __float_as_int(s0.w), 0);
                 }
                 __syncthreads();
                 return;
@@ -2030,13 +2071,13 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
             const int incl = wave_incl_scan(cnt);
             const int excl = incl - cnt;
             const int T = __builtin_amdgcn_readlane(incl, 63);
-            for (int t0 = 0; t0 < T; t0 += 64 * SU) {
+            for (int t0 = 0; t0 < T; t0 += 64 * SUk) {
                 // owners of the ranks (largest j with excl[j] <= t), then all
                 // loads in flight before any is used
-                int sidx[SU];
-                float sx[SU], sy[SU], sz[SU];
+                int sidx[SUk];
+                float sx[SUk], sy[SUk], sz[SUk];
 #pragma unroll
-                for (int u = 0; u < SU; u++) {
+                for (int u = 0; u < SUk; u++) {
                     const int t = t0 + 64 * u + lane;
                     int j = 0;
 #pragma unroll
@@ -2050,13 +2091,13 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
                     sz[u] = __shfl(tz, j, 64);
                     sidx[u] = t < T ? sj + (t - ej) : 0;
                 }
-                float4 q[SU];
+                float4 q[SUk];
 #pragma unroll
-                for (int u = 0; u < SU; u++) q[u] = srt[sidx[u]];
+                for (int u = 0; u < SUk; u++) q[u] = srt[sidx[u]];
 #pragma unroll
-                for (int u = 0; u < SU; u++) q[u] = mk4(q[u].x + sx[u], q[u].y + sy[u], q[u].z + sz[u], q[u].w);
+                for (int u = 0; u < SUk; u++) q[u] = mk4(q[u].x + sx[u], q[u].y + sy[u], q[u].z + sz[u], q[u].w);
 #pragma unroll
-                for (int u = 0; u < SU; u++) {
+                for (int u = 0; u < SUk; u++) {
                     if (t0 + 64 * u >= T) break;
                     if (nst + 64 > tcap) {
                         evaluate(nst);
@@ -2068,7 +2109,16 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
                     const float dz = fmaxf(fmaxf(lz - q[u].z, q[u].z - hz), 0.f);
                     const bool pass = t0 + 64 * u + lane < T && dx * dx + dy * dy + dz * dz <= rs2;
                     const unsigned long long m = __ballot(pass);
-                    if (pass) stg[nst + __popcll(m & lt_mask)] = q[u];
+                    if (pass) {
+                        const int slot = nst + __popcll(m & lt_mask);
+                        if constexpr (NVM > 0) {
+                            const float sx_ = q[u].x - ccx, sy_ = q[u].y - ccy, sz_ = q[u].z - ccz;
+                            stg[slot] = mk4(sx_, sy_, sz_, q[u].w);
+                            stw[slot] = fmaf(sx_, sx_, fmaf(sy_, sy_, sz_ * sz_));
+                        } else {
+                            stg[slot] = q[u];
+                        }
+                    }
                     nst += __popcll(m);
                 }
             }
@@ -2079,7 +2129,13 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
         float mb, ms, rb, rsec;
         int mi, ma, ri;
         if constexpr (NVM > 0) {
-            // columns -> each atom's minimum; atoms -> the molecule, in the lane
+            // + |y'|^2; columns -> each atom's minimum; atoms -> the molecule, in the lane
+#pragma unroll
+            for (int q = 0; q < NVM; q++) {
+                const float yy2 = 0.25f * fmaf(ym[q].x, ym[q].x, fmaf(ym[q].y, ym[q].y, ym[q].z * ym[q].z));
+                bq[q].best += yy2;
+                bq[q].second += yy2;
+            }
 #pragma unroll
             for (int q = 0; q < NVM; q++) col_reduce<C>(bq[q]);
             mb = bq[0].best; ms = bq[0].second; mi = bq[0].bi; ma = 0;
@@ -2125,10 +2181,10 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
         Rec r = rec_zero();
         bool exh = false;
         if (leader) {
-            if (mb <= g.cut2_hi) {
-                if (ms - mb > g.amb) {
+            if (mb <= cut2_hi_c) {
+                if (ms - mb > amb_c) {
                     int bn, bk;
-                    const int dd = decide32(P, g.guard32, mb, bn, bk);
+                    const int dd = decide32(P, G32c, mb, bn, bk);
                     if (dd != 0) {
                         if (dd == 2) { r.within = 1; r.i = mi; r.j = jbase + ma; r.d = mb; r.bin = bn; r.bulk = bk; }
                     } else {
@@ -2140,10 +2196,10 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
                     }
                 } else exh = true;
             }
-            if (rb <= g.cut2_hi) {
-                if (rsec - rb > g.amb) {
+            if (rb <= cut2_hi_c) {
+                if (rsec - rb > amb_c) {
                     int bn, bk;
-                    const int dd = decide32(P, g.guard32, rb, bn, bk);
+                    const int dd = decide32(P, G32c, rb, bn, bk);
                     if (dd != 0) {
                         if (dd == 2) { r.ref = 1; r.dref = rb; r.rbin = bn; }
                     } else {
@@ -2162,7 +2218,7 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
         while (amask) {
             const int L = __ffsll((long long)amask) - 1;
             amask &= amask - 1ull;
-            const float bandm = __shfl(mb, L, 64) + g.amb, bandr = __shfl(rb, L, 64) + g.amb;
+            const float bandm = __shfl(mb, L, 64) + amb_c, bandr = __shfl(rb, L, 64) + amb_c;
             const int mlL = NVM ? L / C : (L / C) / P.nv;
             const int vL = __shfl(v, L, 64);
             const int64_t jb = (int64_t)vL * P.nv;
@@ -2176,7 +2232,9 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
                     for (int t = 0; t < NVM; t++)
                         if (t == q) yq = ym[t];
                 }
-                const float yx = __shfl(yq.x, sl, 64), yy = __shfl(yq.y, sl, 64), yz = __shfl(yq.z, sl, 64);
+                float yx = __shfl(yq.x, sl, 64), yy = __shfl(yq.y, sl, 64), yz = __shfl(yq.z, sl, 64);
+                // NVM > 0: back from -2 y' (relative, exact) ...
+                if constexpr (NVM > 0) { yx *= -0.5f; yy *= -0.5f; yz *= -0.5f; }
                 double Y0, Y1, Y2;
                 if (RANDOM) {
                     const double* pp = B.P + (((int64_t)f * P.K + pidx) * P.Nv + pb + q) * 3;
@@ -2184,6 +2242,7 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
                 } else load3(xv + (int64_t)f * P.Nv * 3, jb + q, Y0, Y1, Y2);
                 const bool isref = q == P.iref;
                 if (one_round) {
+                    // NVM > 0: y' against the staged atoms relative to the chunk centre
                     for (int k = lane; k < last_n; k += 64) {
                         float4 s = stg[k];
                         float dx = yx - s.x, dy = yy - s.y, dz = yz - s.z;
@@ -2202,8 +2261,10 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
                     // within band + guard), every pair there in FP64
                     int jlo[3], jhi[3];
                     const float band = isref ? fmaxf(bandm, bandr) : bandm;
-                    const float rband = fminf(sqrtf(band + (float)g.guard) + 1e-3f, rs);
-                    cell_range(g, yx, yy, yz, yx, yy, yz, rband, jlo, jhi);
+                    const float rband = fminf(sqrtf(band + Gcf) + 1e-3f, rs);
+                    // ... and to absolute coordinates (rounding within the 1e-3 slack)
+                    const float yax = NVM > 0 ? yx + ccx : yx, yay = NVM > 0 ? yy + ccy : yy, yaz = NVM > 0 ? yz + ccz : yz;
+                    cell_range(g, yax, yay, yaz, yax, yay, yaz, rband, jlo, jhi);
                     const int mxn = jhi[0] - jlo[0] + 1, myn = jhi[1] - jlo[1] + 1, mzn = jhi[2] - jlo[2] + 1;
                     const int ncq = mxn * myn * mzn;
                     for (int ci = lane; ci < ncq; ci += 64) {
@@ -2519,7 +2580,8 @@ template <class TU, class TV, bool RANDOM, int C>
 void launch_tiles_one(hipStream_t s, int grid, int cus, const KP& P, const Bufs& B, const void* xu, const void* xv)
 {
     const int nfp = P.F * (RANDOM ? P.K : 1);
-    const size_t shm = sizeof(float4) * (size_t)P.tcap + sizeof(int) * (size_t)std::min(nfp + 1, CBCAP);
+    const size_t shm = (sizeof(float4) + (P.mol_lane ? sizeof(float) : 0)) * (size_t)P.tcap +
+                       sizeof(int) * (size_t)std::min(nfp + 1, CBCAP);
     // 3-atom solvent molecules: one molecule per lane
     auto kern = P.mol_lane ? k_eval_tiles<TU, TV, RANDOM, C, 3> : k_eval_tiles<TU, TV, RANDOM, C, 0>;
     if (grid <= 0) {
@@ -3051,7 +3113,9 @@ void job_params(cm_job* J, const cm_job_desc& d)
     P.mol_lane = P.nv == 3 ? 1 : 0;
     J->tile_c = P.mol_lane ? 4 : 1;
     P.mpt = P.mol_lane ? 64 / J->tile_c : 64 / P.nv;
-    P.tcap = TCAP_DEFAULT;
+    // molecule per lane stages |s'|^2 beside each atom: fewer per round so
+    // that 12 one-wavefront blocks per CU still fit
+    P.tcap = P.mol_lane ? TCAP_MOL : TCAP_DEFAULT;
     P.gnu = d.n_groups_u;
     P.gnv = d.n_groups_v;
     // molecule-pair kernels keep the group counts in LDS when they fit
