@@ -131,12 +131,44 @@ def _summary(lat, t_first, t_last, n):
             "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean())}
 
 
+class _Digests:
+    """SHA-1 digests computed on a worker thread, off both paths' clocks (the
+    hash releases the GIL, as the ctypes calls do)."""
+
+    def __init__(self, n, keep_raw):
+        self.out = [None] * n
+        self.raw = [None] * n if keep_raw else None
+        self.q = queue.Queue()
+        self.th = threading.Thread(target=self._run)
+        self.th.start()
+
+    def _run(self):
+        while True:
+            item = self.q.get()
+            if item is None:
+                return
+            i, raw = item
+            self.out[i] = digest(raw)
+            if self.raw is not None:
+                self.raw[i] = raw
+
+    def put(self, i, raw):
+        self.q.put((i, raw))
+
+    def close(self):
+        self.q.put(None)
+        self.th.join()
+
+
 def run_served(pool, reqs, slots=4, memory_budget=0, keep_raw=False):
-    """Requests enqueued at their arrival times on a cm_server; latency is
-    completion minus arrival."""
+    """Requests enqueued at their arrival times on a cm_server.  A request is
+    done when its counts are in the caller's arrays (cm_server_wait returned);
+    latency is that time minus its arrival, on the same clock as run_plain."""
     srv = _lib.Server(slots, memory_budget)
     handoff = queue.Queue()
-    res = [None] * len(reqs)
+    done = [None] * len(reqs)
+    dg = _Digests(len(reqs), keep_raw)
+    t0 = time.perf_counter()
 
     def collect():
         while True:
@@ -144,12 +176,12 @@ def run_served(pool, reqs, slots=4, memory_budget=0, keep_raw=False):
             if item is None:
                 return
             i, tk = item
-            raw, tm = srv.wait(tk)
-            res[i] = (tm, digest(raw), raw if keep_raw else None)
+            raw, _ = srv.wait(tk)
+            done[i] = time.perf_counter() - t0
+            dg.put(i, raw)
 
     th = threading.Thread(target=collect)
     th.start()
-    t0 = time.perf_counter()
     try:
         for i, r in enumerate(reqs):
             dt = t0 + r.arrival - time.perf_counter()
@@ -164,42 +196,45 @@ def run_served(pool, reqs, slots=4, memory_budget=0, keep_raw=False):
         handoff.put(None)
         th.join()
         srv.close()
-    lat = [tm["finished"] - tm["enqueued"] for tm, _, _ in res]
-    out = _summary(lat, min(tm["enqueued"] for tm, _, _ in res), max(tm["finished"] for tm, _, _ in res), len(reqs))
-    out["digests"] = [d for _, d, _ in res]
+        dg.close()
+    lat = [done[i] - r.arrival for i, r in enumerate(reqs)]
+    out = _summary(lat, min(r.arrival for r in reqs), max(done), len(reqs))
+    out["digests"] = dg.out
     if keep_raw:
-        out["raw"] = [x for _, _, x in res]
+        out["raw"] = dg.raw
     return out
 
 
 def run_plain(pool, reqs, keep_raw=False):
     """The same requests one at a time (each starts when it has arrived and the
-    previous one is done)."""
+    previous one is done; done = counts in the caller's arrays)."""
+    dg = _Digests(len(reqs), keep_raw)
     t0 = time.perf_counter()
-    lat, digests, raws, fin, service = [], [], [], [], []
-    for r in reqs:
-        dt = t0 + r.arrival - time.perf_counter()
-        if dt > 0:
-            time.sleep(dt)
-        t_start = time.perf_counter()
-        s = pool[r.system]
-        kw, du, dv, tuning = job_kwargs(s, r)
-        job = _lib.Job(tuning=tuning or None, **kw)
-        job.submit_device(np.arange(1, r.nframes + 1), du, dv, 0, s.cells[:r.nframes], np.ones(r.nframes))
-        raw = job.finish()
-        job.close()
-        t = time.perf_counter() - t0
-        service.append(time.perf_counter() - t_start)
-        fin.append(t)
-        lat.append(t - r.arrival)
-        digests.append(digest(raw))
-        if keep_raw:
-            raws.append(raw)
+    lat, fin, service = [], [], []
+    try:
+        for i, r in enumerate(reqs):
+            dt = t0 + r.arrival - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            t_start = time.perf_counter()
+            s = pool[r.system]
+            kw, du, dv, tuning = job_kwargs(s, r)
+            job = _lib.Job(tuning=tuning or None, **kw)
+            job.submit_device(np.arange(1, r.nframes + 1), du, dv, 0, s.cells[:r.nframes], np.ones(r.nframes))
+            raw = job.finish()
+            job.close()
+            t = time.perf_counter() - t0
+            service.append(time.perf_counter() - t_start)
+            fin.append(t)
+            lat.append(t - r.arrival)
+            dg.put(i, raw)
+    finally:
+        dg.close()
     out = _summary(lat, min(r.arrival for r in reqs), max(fin), len(reqs))
-    out["digests"] = digests
+    out["digests"] = dg.out
     out["service_s"] = service
     if keep_raw:
-        out["raw"] = raws
+        out["raw"] = dg.raw
     return out
 
 
