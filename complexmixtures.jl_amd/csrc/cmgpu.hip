@@ -1699,12 +1699,12 @@ __global__ void __launch_bounds__(1024) k_scan_rows(KP P, Bufs B, int p0, int np
 // molecules of passes [p0, p0+np) sorted by cell
 __global__ void k_mol_scatter(KP P, Bufs B, int p0, int np)
 {
-    int64_t total = (int64_t)P.F * np * P.Mv;
-    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
-        int64_t fq = it / P.Mv;
-        int v = (int)(it - fq * P.Mv);
-        int f = (int)(fq / np);
-        int p = p0 + (int)(fq - (int64_t)f * np);
+    // grid: x over the molecules, y over the (frame, pass) pairs
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P.Mv) return;
+    for (int fq = blockIdx.y; fq < P.F * np; fq += gridDim.y) {
+        const int f = fq / np;
+        const int p = p0 + fq - f * np;
         int64_t fp = (int64_t)f * P.NP + p;
         int key = B.mkey[fp * P.Mv + v];
         if (key < 0) continue;
@@ -1753,12 +1753,12 @@ __global__ void __launch_bounds__(1024) k_chunk_table(KP P, Bufs B, int p0, int 
 // (the others cannot have an atom within the cutoff of the solute)
 __global__ void k_place_keys(KP P, Bufs B)
 {
-    int64_t total = (int64_t)P.F * P.K * P.Mv;
-    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
-        int64_t fk = it / P.Mv;
-        int v = (int)(it - fk * P.Mv);
-        int f = (int)(fk / P.K);
-        int k = (int)(fk - (int64_t)f * P.K);
+    // grid: x over the slots, y over the (frame, pass) pairs
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P.Mv) return;
+    for (int fk = blockIdx.y; fk < P.F * P.K; fk += gridDim.y) {
+        const int f = fk / P.K;
+        const int k = fk - f * P.K;
         const Geo& g = B.geo[f];
         int64_t fp = (int64_t)f * P.NP + 1 + k;
         cm::u32x4 r = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT}, P.key0, P.key1);
@@ -1776,12 +1776,12 @@ __global__ void k_place_keys(KP P, Bufs B)
 template <class TV>
 __global__ void k_place_near(KP P, Bufs B, const TV* xv)
 {
-    int64_t total = (int64_t)P.F * P.K * P.Mv;
-    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
-        int64_t fk = it / P.Mv;
-        int pos = (int)(it - fk * P.Mv);
-        int f = (int)(fk / P.K);
-        int k = (int)(fk - (int64_t)f * P.K);
+    // grid: x over the sorted positions, y over the (frame, pass) pairs
+    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= P.Mv) return;
+    for (int fk = blockIdx.y; fk < P.F * P.K; fk += gridDim.y) {
+        const int f = fk / P.K;
+        const int k = fk - f * P.K;
         int64_t fp = (int64_t)f * P.NP + 1 + k;
         if (pos >= B.nnear[fp]) continue;
         const int v = B.msorted[fp * P.Mv + pos];
@@ -2325,6 +2325,13 @@ int fail(int code, const std::string& msg)
         if (e_ != hipSuccess) return fail(CM_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// x: blocks over n molecules or slots; y: one per (frame, pass) pair, up to
+// the grid limit (the kernels stride over y beyond it)
+dim3 grid_slots(int n, int pairs)
+{
+    return dim3((unsigned)std::max(1, (n + BLOCK - 1) / BLOCK), (unsigned)std::max(1, std::min(pairs, 65535)));
+}
+
 int grid_for(int64_t n)
 {
     int64_t b = (n + BLOCK - 1) / BLOCK;
@@ -2765,7 +2772,7 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         if (f32) hipLaunchKernelGGL(k_prep_solvent_tiles<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);
         else hipLaunchKernelGGL(k_prep_solvent_tiles<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vd);
         hipLaunchKernelGGL(k_scan_rows, dim3(F), dim3(1024), 0, s, P, B, 0, 1);
-        hipLaunchKernelGGL(k_mol_scatter, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, 0, 1);
+        hipLaunchKernelGGL(k_mol_scatter, grid_slots(P.Mv, F), dim3(BLOCK), 0, s, P, B, 0, 1);
         if (f32) hipLaunchKernelGGL(k_fill_real<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);
         else hipLaunchKernelGGL(k_fill_real<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vd);
         hipLaunchKernelGGL(k_chunk_table, dim3(1), dim3(1024), 0, s, P, B, 0, 1);
@@ -2811,11 +2818,11 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         hipLaunchKernelGGL(k_bulk_compact, dim3(F * P.kb), dim3(1024), 0, s, P, B);
         int64_t n = (int64_t)F * P.K * P.Mv;
         if (tiles) {
-            hipLaunchKernelGGL(k_place_keys, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B);
+            hipLaunchKernelGGL(k_place_keys, grid_slots(P.Mv, F * P.K), dim3(BLOCK), 0, s, P, B);
             hipLaunchKernelGGL(k_scan_rows, dim3(F * P.K), dim3(1024), 0, s, P, B, 1, P.K);
-            hipLaunchKernelGGL(k_mol_scatter, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, 1, P.K);
-            if (f32) hipLaunchKernelGGL(k_place_near<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);
-            else hipLaunchKernelGGL(k_place_near<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vd);
+            hipLaunchKernelGGL(k_mol_scatter, grid_slots(P.Mv, F * P.K), dim3(BLOCK), 0, s, P, B, 1, P.K);
+            if (f32) hipLaunchKernelGGL(k_place_near<float>, grid_slots(P.Mv, F * P.K), dim3(BLOCK), 0, s, P, B, vf);
+            else hipLaunchKernelGGL(k_place_near<double>, grid_slots(P.Mv, F * P.K), dim3(BLOCK), 0, s, P, B, vd);
             hipLaunchKernelGGL(k_chunk_table, dim3(1), dim3(1024), 0, s, P, B, 1, P.K);
         } else {
             if (f32) hipLaunchKernelGGL(k_place<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);
