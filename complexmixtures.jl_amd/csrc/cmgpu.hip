@@ -1485,12 +1485,12 @@ struct MolDst {   // placed atoms: FP64 (exact resolve) and FP32 (evaluation)
 template <class TV>
 __global__ void k_place(KP P, Bufs B, const TV* xv)
 {
-    int64_t total = (int64_t)P.F * P.K * P.Mv;
-    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
-        int64_t fk = it / P.Mv;
-        int v = (int)(it - fk * P.Mv);
-        int f = (int)(fk / P.K);
-        int k = (int)(fk - (int64_t)f * P.K);
+    // grid: x over the slots, y over the (frame, pass) pairs
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= P.Mv) return;
+    for (int fk = blockIdx.y; fk < P.F * P.K; fk += gridDim.y) {
+        const int f = fk / P.K;
+        const int k = fk - f * P.K;
         const Geo& g = B.geo[f];
         cm::u32x4 r = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT}, P.key0, P.key1);
         cm::u32x4 r2 = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT + 1u}, P.key0, P.key1);
@@ -1505,12 +1505,12 @@ __global__ void k_place(KP P, Bufs B, const TV* xv)
         // the new centre H*uc lies inside the cell: its cell follows from uc
         int cell;
         cell_of_frac(g, uc[0], uc[1], uc[2], cell);
-        B.vcell_r[fk * P.Mv + v] = cell;
+        B.vcell_r[(int64_t)fk * P.Mv + v] = cell;
         const double* H = g.cell.H;
         double c0 = H[0] * uc[0] + H[3] * uc[1] + H[6] * uc[2];
         double c1 = H[1] * uc[0] + H[4] * uc[1] + H[7] * uc[2];
         double c2 = H[2] * uc[0] + H[5] * uc[1] + H[8] * uc[2];
-        B.vcen_r[fk * P.Mv + v] = mk4((float)c0, (float)c1, (float)c2, 0.f);
+        B.vcen_r[(int64_t)fk * P.Mv + v] = mk4((float)c0, (float)c1, (float)c2, 0.f);
     }
 }
 
@@ -2816,7 +2816,6 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
     // random passes
     if (P.K > 0) {
         hipLaunchKernelGGL(k_bulk_compact, dim3(F * P.kb), dim3(1024), 0, s, P, B);
-        int64_t n = (int64_t)F * P.K * P.Mv;
         if (tiles) {
             hipLaunchKernelGGL(k_place_keys, grid_slots(P.Mv, F * P.K), dim3(BLOCK), 0, s, P, B);
             hipLaunchKernelGGL(k_scan_rows, dim3(F * P.K), dim3(1024), 0, s, P, B, 1, P.K);
@@ -2825,8 +2824,8 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
             else hipLaunchKernelGGL(k_place_near<double>, grid_slots(P.Mv, F * P.K), dim3(BLOCK), 0, s, P, B, vd);
             hipLaunchKernelGGL(k_chunk_table, dim3(1), dim3(1024), 0, s, P, B, 1, P.K);
         } else {
-            if (f32) hipLaunchKernelGGL(k_place<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);
-            else hipLaunchKernelGGL(k_place<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vd);
+            if (f32) hipLaunchKernelGGL(k_place<float>, grid_slots(P.Mv, F * P.K), dim3(BLOCK), 0, s, P, B, vf);
+            else hipLaunchKernelGGL(k_place<double>, grid_slots(P.Mv, F * P.K), dim3(BLOCK), 0, s, P, B, vd);
         }
         HIPCHK(hipGetLastError());
         if (J->profiling) HIPCHK(hipEventRecord(ev[2], s));
@@ -2835,6 +2834,7 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         } else {
             const int gwords = P.glds ? P.nbins * (P.gnu + (P.self_ ? 0 : P.gnv)) : 0;
             const size_t shr = sizeof(unsigned) * (2 * P.nbins + gwords);
+            const int64_t n = (int64_t)F * P.K * P.Mv;
             if (f32) hipLaunchKernelGGL(k_eval_mols_rand<float>, dim3(grid_for(n)), dim3(BLOCK), shr, s, P, B, uf);
             else hipLaunchKernelGGL(k_eval_mols_rand<double>, dim3(grid_for(n)), dim3(BLOCK), shr, s, P, B, ud);
         }
