@@ -490,6 +490,35 @@ def test_self_coordination_numbers(pkg, oracle):
     assert_raw_equal(a, b)
 
 
+@pytest.mark.parametrize("irefatom", [2, 3])
+def test_self_water_reference_atom(pkg, oracle, irefatom):
+    """Self-MDDF of waters with a hydrogen as the reference atom: the rdf
+    counts and the bulk flags of the ideal-gas passes come from that atom in
+    both orientations of every pair, against the oracle."""
+    S = syn()
+    rng = np.random.default_rng(50 + irefatom)
+    box = 34.0
+    x = S.place_molecules(rng, S.water_template(), rng.uniform(0, box, (1200, 3)))
+    xv = np.stack([x + rng.normal(scale=0.2, size=x.shape) for _ in range(2)])
+    sel = pkg.AtomSelection(list(range(1, 3601)), natomspermol=3)
+    traj = pkg.MemoryTrajectory(xv, xv, np.repeat(np.diag([box] * 3)[None], 2, axis=0), sel, sel)
+    opts = pkg.Options(dbulk=9.0, n_random_samples=3, seed=52, silent=True, irefatom=irefatom)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts)
+    assert_raw_equal(a, b)
+
+
+def test_parity_self_urea_reference_atom(pkg, oracle):
+    """Self-MDDF of 8-atom molecules with the sixth atom as the reference
+    atom, against the oracle."""
+    s = syn().config3(nframes=2, n_solute=10, n_urea=600, n_water=10, solvent="self", seed=5, a=36.0)
+    traj = mem(pkg, s)
+    opts = pkg.Options(dbulk=9.0, n_random_samples=3, seed=11, silent=True, irefatom=6)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts)
+    assert_raw_equal(a, b)
+
+
 def test_self_water_half_shell_with_random_passes(pkg, oracle):
     """Self-MDDF of waters with three or more cells per dimension (the
     half-shell walk of the molecule-pair kernel, both orientations from one
