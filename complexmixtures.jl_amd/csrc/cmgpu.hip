@@ -154,7 +154,7 @@ struct Bufs {
     // accumulators
     ull* cnt_md; ull* cnt_rdf; ull* cnt_md_r; ull* cnt_rdf_r;
     ull* cnt_gu; ull* cnt_gv; ull* cnt_gu_r; ull* cnt_gv_r;
-    ull* stats;              // [0] exhaustive resolves
+    ull* stats;              // [0] exhaustive resolves, [2] geometry error bits of the job's frames
 };
 
 struct Rec {
@@ -313,6 +313,17 @@ __device__ __forceinline__ double halfulp32(double x)
 {
     if (!(x > 1e-30)) return 0x1p-150;
     return ldexp(1.0, ilogb(x) - 24);
+}
+
+// geometry errors of a batch's frames into the job's error bits (bit 0:
+// molecules too large for the cell, bit 1: grid larger than its allocation),
+// read with the accumulators at the end of the job
+__global__ void k_geo_flags(KP P, Bufs B)
+{
+    for (int f = threadIdx.x; f < P.F; f += blockDim.x) {
+        const int e = B.geo[f].err;
+        if (e) atomicOr(&B.stats[2], e == 2 ? 1ull : 2ull);
+    }
 }
 
 __global__ void k_grid(KP P, Bufs B, int mols_family)
@@ -2536,7 +2547,7 @@ struct cm_job {
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     double eval_ms = 0.0, total_ms = 0.0;
     int64_t eval_launches = 0, total_launches = 0, frames = 0;
-    std::vector<Geo> hgeo, hgeo_out;
+    std::vector<Geo> hgeo;
     std::vector<int> huk, hdp, hdi;
     std::vector<int32_t> hgm;        // group maps (host copy of the upload)
     PinBuf hres;                     // accumulators and hit counts read back at finish (page-locked)
@@ -2840,18 +2851,13 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         }
         if (J->profiling) HIPCHK(hipEventRecord(ev[3], s));
     }
+    // geometry errors detected on the device: kept in the job's error bits and
+    // reported when the job is collected, so the host does not wait for the
+    // batch here and prepares the next one while it runs
+    hipLaunchKernelGGL(k_geo_flags, dim3(1), dim3(64), 0, s, P, B);
     HIPCHK(hipGetLastError());
-    // geometry errors detected on the device
-    J->hgeo_out.assign(F, Geo());
-    HIPCHK(hipMemcpyAsync(J->hgeo_out.data(), J->geo.p, sizeof(Geo) * F, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    for (int f = 0; f < F; f++) {
-        if (J->hgeo_out[f].err == 2)
-            return fail(CM_EUNSUPPORTED, "molecules too large for the unit cell: cutoff plus molecule radii exceed a plane spacing");
-        if (J->hgeo_out[f].err)
-            return fail(CM_EHIP, "internal error: cell grid larger than its allocation");
-    }
     if (J->profiling) {
+        HIPCHK(hipStreamSynchronize(s));
         float ms = 0.f, tot = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
         J->eval_ms += ms;
@@ -3185,7 +3191,9 @@ int job_collect(cm_job* J)
         if (r) return r;
         J->pending = false;
     }
-    const size_t need = 8 * (size_t)J->acc_len + 8 * 4 * (size_t)J->d.nbins;
+    // accumulators, hits, then the geometry error bits
+    const size_t off_err = 8 * (size_t)J->acc_len + 8 * 4 * (size_t)J->d.nbins;
+    const size_t need = off_err + 8;
     if (J->hres.bytes < need) {
         pin_put(J->hres);
         int r = pin_take(J->hres, need);
@@ -3194,7 +3202,14 @@ int job_collect(cm_job* J)
     HIPCHK(hipMemcpyAsync(J->hres.p, J->acc.p, 8 * J->acc_len, hipMemcpyDeviceToHost, J->stream));
     HIPCHK(hipMemcpyAsync((char*)J->hres.p + 8 * J->acc_len, J->hits.p, 8 * 4 * J->d.nbins, hipMemcpyDeviceToHost,
                           J->stream));
+    HIPCHK(hipMemcpyAsync((char*)J->hres.p + off_err, (char*)J->stats.p + 16, 8, hipMemcpyDeviceToHost, J->stream));
     HIPCHK(hipStreamSynchronize(J->stream));
+    uint64_t gerr;
+    memcpy(&gerr, (const char*)J->hres.p + off_err, 8);
+    if (gerr & 1)
+        return fail(CM_EUNSUPPORTED, "molecules too large for the unit cell: cutoff plus molecule radii exceed a plane spacing");
+    if (gerr)
+        return fail(CM_EHIP, "internal error: cell grid larger than its allocation");
     J->finished = true;
     return CM_OK;
 }

@@ -116,7 +116,10 @@ int cm_job_create(const cm_job_desc *desc, cm_job **job);
  * depend on how frames are batched); weights are the frame weights (frames of
  * weight 0 must not be passed, as in goto_nextframe!, src/mddf.jl:95-111).
  * Replaces the per-frame calls of mddf_frame! / coordination_number_frame!
- * (src/mddf.jl:329-331). */
+ * (src/mddf.jl:329-331).  The call returns once the frames' work is queued on
+ * the job's stream (host inputs are copied before it returns); a geometry
+ * that the device rejects (molecules too large for the unit cell) is
+ * reported by cm_job_finish. */
 int cm_job_submit(cm_job *job, int32_t nframes, const int64_t *frame_index, const void *x_solute,
                   const void *x_solvent, int32_t dtype, const double *cells, const double *weights);
 
