@@ -1529,6 +1529,7 @@ __global__ void k_place(KP P, Bufs B, const TV* xv)
 // tiles family: one solute molecule, solvent molecules of up to 64 atoms
 // ---------------------------------------------------------------------------
 constexpr int TCAP_DEFAULT = 640;   // staged solute atoms per round (10 KiB of LDS)
+constexpr int RB_PIECES = 8;        // read-back pieces of cm_job_finish
 constexpr int TCAP_MOL = 512;       // the same with the squared norms (20 B each: 10 KiB)
 constexpr int CBCAP = 1024;         // chunk-table entries kept in LDS
 constexpr int SU = 4;               // staging: ranks per lane per step
@@ -2545,6 +2546,7 @@ struct cm_job {
     bool finished = false;
     bool profiling = false;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    hipEvent_t rb_ev[RB_PIECES + 1] = {};   // pipelined read-back (cm_job_finish)
     double eval_ms = 0.0, total_ms = 0.0;
     int64_t eval_launches = 0, total_launches = 0, frames = 0;
     std::vector<Geo> hgeo;
@@ -3183,6 +3185,99 @@ int job_setup(cm_job* J, const cm_job_desc& d)
     return CM_OK;
 }
 
+// Read-back of cm_job_finish, pipelined: the accumulators come back in
+// RB_PIECES pieces, each with an event, and worker threads copy 2 MB tasks
+// into the caller's arrays as soon as the piece holding a task has landed, so
+// the copies into (often freshly allocated, page-faulting) caller memory
+// overlap the transfer.  The geometry error bits come back first.
+int job_finish_pipelined(cm_job* J, cm_raw* out)
+{
+    if (J->pending) {
+        int r = flush(J, J->pending_w);
+        if (r) return r;
+        J->pending = false;
+    }
+    const cm_job_desc& d = J->d;
+    const int64_t nb = d.nbins, ngu = d.n_groups_u, ngv = d.n_groups_v;
+    const size_t acc_bytes = 8 * (size_t)J->acc_len;
+    const size_t off_hits = acc_bytes, off_err = acc_bytes + 8 * 4 * (size_t)nb;
+    const size_t need = off_err + 8;
+    if (J->hres.bytes < need) {
+        pin_put(J->hres);
+        int r = pin_take(J->hres, need);
+        if (r) return r;
+    }
+    for (auto& e : J->rb_ev)
+        if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    char* h = (char*)J->hres.p;
+    hipStream_t s = J->stream;
+    HIPCHK(hipMemcpyAsync(h + off_err, (char*)J->stats.p + 16, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(h + off_hits, J->hits.p, 8 * 4 * nb, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(J->rb_ev[RB_PIECES], s));
+    const size_t piece = ((acc_bytes + RB_PIECES - 1) / RB_PIECES + 4095) & ~(size_t)4095;
+    for (int k = 0; k < RB_PIECES; k++) {
+        const size_t o = std::min(acc_bytes, k * piece), n = std::min(acc_bytes, o + piece) - o;
+        if (n) HIPCHK(hipMemcpyAsync(h + o, (char*)J->acc.p + o, n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipEventRecord(J->rb_ev[k], s));
+    }
+    HIPCHK(hipEventSynchronize(J->rb_ev[RB_PIECES]));
+    uint64_t gerr;
+    memcpy(&gerr, h + off_err, 8);
+    if (gerr & 1)
+        return fail(CM_EUNSUPPORTED, "molecules too large for the unit cell: cutoff plus molecule radii exceed a plane spacing");
+    if (gerr)
+        return fail(CM_EHIP, "internal error: cell grid larger than its allocation");
+    // the caller's arrays as byte ranges of the accumulator block, in order
+    struct Seg { size_t o, n; char* dst; };
+    std::vector<Seg> segs;
+    size_t o = 0;
+    auto add = [&](double* dst, int64_t n) { if (dst && n) segs.push_back({o, 8 * (size_t)n, (char*)dst}); o += 8 * (size_t)n; };
+    add(out->md_count, nb);
+    add(out->md_count_random, nb);
+    add(out->rdf_count, nb);
+    add(out->rdf_count_random, nb);
+    add(out->solute_group_count, ngu * nb);
+    add(out->solvent_group_count, ngv * nb);
+    add(out->solute_group_count_random, ngu * nb);
+    add(out->solvent_group_count_random, ngv * nb);
+    constexpr size_t TASK = (size_t)2 << 20;
+    const size_t ntask = (acc_bytes + TASK - 1) / TASK;
+    hipError_t werr = hipSuccess;
+    std::mutex werr_mu;
+    auto work = [&](size_t t0, size_t step) {
+        for (size_t t = t0; t < ntask; t += step) {
+            const size_t a = t * TASK, b = std::min(acc_bytes, a + TASK);
+            // pieces land in stream order: the piece holding the task's last byte
+            const hipError_t e = hipEventSynchronize(J->rb_ev[(b - 1) / piece]);
+            if (e != hipSuccess) {
+                std::lock_guard<std::mutex> lk(werr_mu);
+                werr = e;
+                return;
+            }
+            for (const Seg& sg : segs) {
+                const size_t lo = std::max(a, sg.o), hi = std::min(b, sg.o + sg.n);
+                if (lo < hi) memcpy(sg.dst + (lo - sg.o), h + lo, hi - lo);
+            }
+        }
+    };
+    const size_t nt = std::min<size_t>(8, ntask);
+    std::vector<std::thread> th;
+    for (size_t w = 1; w < nt; w++) th.emplace_back(work, w, nt);
+    if (nt > 0) work(0, std::max<size_t>(nt, 1));
+    for (auto& x : th) x.join();
+    if (werr != hipSuccess) return fail(CM_EHIP, std::string("read-back: ") + hipGetErrorString(werr));
+    HIPCHK(hipEventSynchronize(J->rb_ev[RB_PIECES - 1]));
+    out->volume_total = J->volume_total;
+    const ull* hits = (const ull*)(h + off_hits);
+    auto cpu = [&](uint64_t* dst, const ull* src) { if (dst) memcpy(dst, src, 8 * nb); };
+    cpu(out->hits_md, hits);
+    cpu(out->hits_md_random, hits + nb);
+    cpu(out->hits_rdf, hits + 2 * nb);
+    cpu(out->hits_rdf_random, hits + 3 * nb);
+    J->finished = true;
+    return CM_OK;
+}
+
 // flushes the pending weight and reads the accumulators and hits back into J->hres
 int job_collect(cm_job* J)
 {
@@ -3301,10 +3396,7 @@ int cm_job_finish(cm_job* J, cm_raw* out)
 {
     if (!J || !out) return fail(CM_EINVAL, "null argument");
     if (J->finished) return fail(CM_ESTATE, "job already finished");
-    int r = job_collect(J);
-    if (r) return r;
-    fill_raw(J->d, J->acc_len, J->hres, J->volume_total, out);
-    return CM_OK;
+    return job_finish_pipelined(J, out);
 }
 
 int cm_job_set_profiling(cm_job* J, int32_t on)
@@ -3366,6 +3458,8 @@ void cm_job_destroy(cm_job* J)
     if (J->e0) hipEventDestroy(J->e0);
     if (J->e1) hipEventDestroy(J->e1);
     if (J->e2) hipEventDestroy(J->e2);
+    for (auto& e : J->rb_ev)
+        if (e) hipEventDestroy(e);
     if (J->stream) hipStreamDestroy(J->stream);
     delete J;
 }
