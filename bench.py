@@ -146,6 +146,30 @@ def self_water_rate(args):
             "fp64_exhaustive_resolves": st["fp64_rechecks"]}
 
 
+def serve_capacity_once(args):
+    """One capacity run of one serving path (every request of the served
+    stream queued at once), after the same pool, autotuning and first use of
+    the kernels as served_stream: the workload of a kernel trace of that path."""
+    import complexmixtures_amd.serve as SV
+    pool = SV.make_pool(args.serve_systems, 20261206, args.serve_frames)
+    for s in pool:
+        SV.autotune(s, frames=4)
+    rng = np.random.default_rng(20261207)
+    reqs = SV.request_stream(rng, args.serve_requests, len(pool), args.serve_frames, rate=20.0)
+    sat = [SV.Request(0.0, r.system, r.nframes, r.mddf, r.self_, r.seed) for r in reqs]
+    SV.run_plain(pool, [r for r in sat if r.self_][:1] + sat[:1])
+    _lib_sync()
+    path = args.serve_capacity_path
+    r = SV.run_served(pool, sat, slots=args.slots) if path == "served" else SV.run_plain(pool, sat)
+    return {"path": path, "requests": len(sat), "jobs_per_s": round(r["jobs_per_s"], 3),
+            "makespan_s": round(r["makespan_s"], 4), "p99_s": round(r["p99_s"], 4)}
+
+
+def _lib_sync():
+    from complexmixtures_amd import _lib
+    _lib.check(_lib.lib().cm_device_synchronize())
+
+
 def served_stream(args):
     """Served mixed-size stream (Poisson arrivals at 20 requests/s) through the
     request server vs the plain one-job-at-a-time path, plus both paths'
@@ -214,6 +238,8 @@ def main():
     ap.add_argument("--serve-systems", type=int, default=12)
     ap.add_argument("--serve-frames", type=int, default=200)
     ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--serve-capacity-path", choices=("served", "plain"), default=None,
+                    help="only one capacity run of that serving path (for kernel traces)")
     ap.add_argument("--no-config3", action="store_true", help="skip the triclinic mixed-case rates")
     ap.add_argument("--self-water-only", action="store_true", help="only the self-MDDF water rate")
     ap.add_argument("--config3-frames", type=int, default=20)
@@ -237,6 +263,9 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 3}))
         return
     pkg = G.load_package()
+    if args.serve_capacity_path:
+        print(json.dumps({"serve_capacity": serve_capacity_once(args)}), flush=True)
+        return
     if args.serve_only:
         print(json.dumps({"served_stream": served_stream(args)}), flush=True)
         return
