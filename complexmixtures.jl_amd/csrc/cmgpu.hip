@@ -714,12 +714,13 @@ __device__ __forceinline__ int wave_incl_scan(int v)
 __device__ __forceinline__ float fmin_sel(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float fmax_sel(float a, float b) { return b > a ? b : a; }
 
-// minimum / maximum over the wavefront, the same in every lane
 // a wave-uniform float moved to a scalar register
 __device__ __forceinline__ float sgpr_f(float x)
 {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
 }
+
+// minimum / maximum over the wavefront, the same in every lane
 __device__ __forceinline__ float wave_min(float x)
 {
     const int id = __float_as_int(INFINITY);
@@ -1685,19 +1686,25 @@ __global__ void __launch_bounds__(1024) k_scan_rows(KP P, Bufs B, int p0, int np
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int base = 0; base < n; base += blockDim.x) {
-        int idx = base + threadIdx.x;
-        int v = idx < n ? c[idx] : 0;
-        int incl = v;
-        for (int o = 1; o < 64; o <<= 1) {
-            int t = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += t;
-        }
+    // four consecutive keys per thread, the threads' sums scanned over the
+    // wavefront (DPP) and across the block's wavefronts
+    for (int base = 0; base < n; base += 4 * blockDim.x) {
+        const int i0 = base + 4 * threadIdx.x;
+        int v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = i0 + u < n ? c[i0 + u] : 0;
+        const int tsum = v[0] + v[1] + v[2] + v[3];
+        const int incl = wave_incl_scan(tsum);
         if (lane == 63) wsum[wid] = incl;
         __syncthreads();
         int woff = 0, tot = 0;
         for (int w = 0; w < nw; w++) { if (w < wid) woff += wsum[w]; tot += wsum[w]; }
-        if (idx < n) { st[idx] = carry + woff + incl - v; c[idx] = 0; }
+        int e = carry + woff + incl - tsum;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (i0 + u < n) { st[i0 + u] = e; c[i0 + u] = 0; }
+            e += v[u];
+        }
         __syncthreads();
         if (threadIdx.x == 0) carry += tot;
         __syncthreads();
