@@ -227,7 +227,10 @@ class Job:
 
     def submit_device(self, frame_index, d_solute, d_solvent, frame0, cells, weights):
         """Device-resident coordinates (DeviceArray of [F][N][3]); frames
-        frame0 .. frame0+len(frame_index)-1 of those arrays."""
+        frame0 .. frame0+len(frame_index)-1 of those arrays.  The job keeps
+        references to both arrays until it is closed: its kernels read them
+        after this call returns."""
+        self._inputs = getattr(self, "_inputs", []) + [d_solute, d_solvent]
         dt = CM_F32 if d_solute.dtype == np.float32 else CM_F64
         fi = np.ascontiguousarray(frame_index, dtype=np.int64)
         c9 = _cells9(cells)
@@ -259,6 +262,7 @@ class Job:
         if self.h:
             lib().cm_job_destroy(self.h)
             self.h = ctypes.c_void_p()
+        self._inputs = []
 
     def __del__(self):
         try:

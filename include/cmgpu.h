@@ -124,7 +124,9 @@ int cm_job_submit(cm_job *job, int32_t nframes, const int64_t *frame_index, cons
                   const void *x_solvent, int32_t dtype, const double *cells, const double *weights);
 
 /* As cm_job_submit, with coordinates already resident in device memory
- * (pointers from cm_device_alloc); cells and weights stay host arrays. */
+ * (pointers from cm_device_alloc); cells and weights stay host arrays.  The
+ * kernels read the device coordinates after the call returns: they must stay
+ * allocated and unchanged until cm_job_finish returns. */
 int cm_job_submit_device(cm_job *job, int32_t nframes, const int64_t *frame_index, const void *d_x_solute,
                          const void *d_x_solvent, int32_t dtype, const double *cells, const double *weights);
 
