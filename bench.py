@@ -61,7 +61,37 @@ def bytes_per_frame(n_u, n_v_atoms, m_v, n_rand, coord_bytes=4):
     return 3 * coord_bytes * (n_u + n_v_atoms) + n_rand * 24 * n_v_atoms + 16 * m_v * (1 + n_rand)
 
 
+def usable_cores():
+    """CPU cores this process may run on: the affinity mask, capped by the
+    cgroup CPU quota and by the host's declared CPU share (OMP_NUM_THREADS)
+    when they are set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(float(quota) / float(period))))
+    except (OSError, ValueError):
+        pass
+    try:
+        n = min(n, max(1, int(os.environ.get("OMP_NUM_THREADS", n))))
+    except ValueError:
+        pass
+    return n
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(pkg, O, s, opts, tm, nframes_sample, threads):
+    """The oracle (FP64, cell lists, the reference's chunked threads) over the
+    first frames; returns (frames/s, seconds, raw accumulators)."""
     from complexmixtures_amd.trajectory import convert_unitcell
     job = O.Job(1, s["solute"].natomspermol, s["solvent"].nmols, s["solvent"].natomspermol,
                 irefatom=tm.irefatom - 1, cutoff=opts.cutoff, dbulk=opts.dbulk,
@@ -71,9 +101,28 @@ def cpu_baseline(pkg, O, s, opts, tm, nframes_sample, threads):
     xv = s["x_solvent"][:f].astype(np.float64)
     cells = np.stack([convert_unitcell(c, True) for c in s["cells"][:f]])
     t = time.perf_counter()
-    O.run(job, list(range(1, f + 1)), xu, xv, cells, np.ones(f))
+    raw = O.run(job, list(range(1, f + 1)), xu, xv, cells, np.ones(f))
     dt = time.perf_counter() - t
-    return f / dt, dt
+    return f / dt, dt, raw
+
+
+SQ_FILE = os.path.join(ROOT, "profiles", "sq_counters_config2_200frames.md")
+
+
+def sq_valu_summary():
+    """VALU issue fraction and SQ_INSTS_VALU per frame of the random-pass
+    evaluation kernel from the SQ counter summary in profiles/ (None when
+    absent)."""
+    try:
+        txt = open(SQ_FILE).read()
+    except OSError:
+        return None
+    import re
+    m = re.search(r"valu_issue_fraction:\s*([0-9.eE+]+)", txt)
+    n = re.search(r"valu_wave_instructions_per_frame:\s*([0-9.eE+]+)", txt)
+    if not (m and n):
+        return None
+    return float(m.group(1)), float(n.group(1))
 
 
 def config3_rates(args):
@@ -300,13 +349,16 @@ def main():
         from complexmixtures_amd.mddf import job_kwargs
         tuning, _ = SV.autotune_job(job_kwargs(R, opts, tm, False), du, dv, cells, frames=8)
 
-    def one_job(profile=False):
+    def one_job(profile=False, nframes=nf):
         job = _job_for(R, opts, tm, False, tuning=tuning)
         if profile:
             job.set_profiling(True)
-        job.submit_device(frames, du, dv, 0, cells, weights)
+        job.submit_device(frames[:nframes], du, dv, 0, cells[:nframes], weights[:nframes])
         raw = job.finish()
-        st = job.stats() if profile else None
+        st = None
+        if profile:
+            st = job.stats()
+            st["launches"] = job.launch_times()
         job.close()
         return raw, st
 
@@ -331,15 +383,22 @@ def main():
                                   "fp64_rechecks": st["fp64_rechecks"], "same_counts": same}), flush=True)
         return
     for _ in range(args.warmup):
-        one_job()
+        one_job(profile=True)
     _lib.check(_lib.lib().cm_device_synchronize())
+    # every timed job records HIP events around its evaluation launches (no
+    # host wait; read after the job), so the roofline is the timed region's own
+    sts = []
     t = time.perf_counter()
     for _ in range(args.steps):
-        raw, _ = one_job()
+        raw, st = one_job(profile=True)
+        sts.append(st)
     _lib.check(_lib.lib().cm_device_synchronize())
     el = time.perf_counter() - t
     fps = nf * args.steps / el
-    _, st = one_job(profile=True)
+    st = {"eval_ms": sum(x["eval_ms"] for x in sts) / len(sts),
+          "eval_launches": sts[0]["eval_launches"], "fp64_rechecks": sts[-1]["fp64_rechecks"]}
+    real_ms = np.concatenate([x["launches"][0] for x in sts])
+    rand_ms = np.concatenate([x["launches"][1] for x in sts])
     # the same job from host buffers (cm_job_submit: coordinates cross PCIe per batch)
     job = _job_for(R, opts, tm, False, tuning=tuning)
     t1 = time.perf_counter()
@@ -357,9 +416,13 @@ def main():
     achieved = ops / eval_s / 1e12
     bpf = bytes_per_frame(n_u, n_v, s["solvent"].nmols, opts.n_random_samples)
     tpf = measured_traffic_per_frame()
-    # frames per launch pair (real + random evaluation) of the profiled job
-    launch_pairs = max(1, st["eval_launches"] // 2)
+    # frames per launch pair (real + random evaluation) of a job
+    launch_pairs = max(1, len(real_ms) // args.steps)
     frames_per_launch = -(-nf // launch_pairs)
+    # the same from the median launch pair instead of the mean
+    med_pair_s = (float(np.median(real_ms)) + float(np.median(rand_ms))) / 1e3
+    achieved_med = 12.0 * W * frames_per_launch / med_pair_s / 1e12
+    sq = sq_valu_summary()
     out = {
         "metric": METRIC, "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 3), "higher_is_better": True,
@@ -376,6 +439,16 @@ def main():
                      "kernel": "k_eval_tiles (real + random passes)", "pairs_per_frame": round(W),
                      "eval_ms_per_frame": round(st["eval_ms"] / nf, 5),
                      "eval_ms_job": round(st["eval_ms"], 3), "eval_launches_job": st["eval_launches"],
+                     "timing": f"HIP events around every evaluation launch of the {args.steps} timed jobs",
+                     "frac_from_median_launch": round(achieved_med / PEAK_FP32_TFLOPS, 5),
+                     "launch_us_real": {"median": round(1e3 * float(np.median(real_ms)), 1),
+                                        "mean": round(1e3 * float(np.mean(real_ms)), 1),
+                                        "max": round(1e3 * float(np.max(real_ms)), 1)},
+                     "launch_us_random": {"median": round(1e3 * float(np.median(rand_ms)), 1),
+                                          "mean": round(1e3 * float(np.mean(rand_ms)), 1),
+                                          "max": round(1e3 * float(np.max(rand_ms)), 1)},
+                     "valu_issue_fraction_random_pass": sq[0] if sq else None,
+                     "valu_lane_instructions_per_W_pair": round(64 * sq[1] / W, 1) if sq else None,
                      "hbm_GBps_algorithmic": round(bpf * nf / eval_s / 1e9, 2),
                      "fp64_exhaustive_resolves": st["fp64_rechecks"]},
         "tuning": tuning,
@@ -384,12 +457,22 @@ def main():
     }
     if not args.no_cpu:
         O = G.load_oracle()
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or usable_cores()
         f = args.cpu_frames or min(nf, 10 * threads)
-        v, dt = cpu_baseline(pkg, O, s, opts, tm, f, threads)
+        v, dt, oraw = cpu_baseline(pkg, O, s, opts, tm, f, threads)
+        # the GPU's counts over the same frames, bit for bit
+        graw, _ = one_job(nframes=f)
+        keys = {"md_count": "md_count", "md_count_random": "md_count_random", "rdf_count": "rdf_count",
+                "rdf_count_random": "rdf_count_random", "solute_group_count": "gu", "solvent_group_count": "gv",
+                "solute_group_count_random": "gu_random", "solvent_group_count_random": "gv_random"}
+        exact = all(np.array_equal(graw[k], oraw[o]) for k, o in keys.items())
         out["cpu_baseline"] = {"value": round(v, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+                               "cpu": cpu_model(), "cores_usable": usable_cores(), "cpu_count": os.cpu_count(),
                                "sample": f"first {f} frames of the same trajectory, {dt:.1f} s",
                                "gpu_over_cpu": round(fps / v, 1)}
+        out["bit_exact"] = {"value": bool(exact),
+                            "what": f"every raw accumulator of the GPU job over the first {f} frames equals the "
+                                    "FP64 oracle's (the CPU baseline's own run)"}
     du.free()
     dv.free()
     if not args.no_serve:

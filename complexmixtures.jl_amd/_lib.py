@@ -60,7 +60,7 @@ class RequestTimes(ctypes.Structure):
 
 
 EXPORTS = ["cm_job_create", "cm_job_submit", "cm_job_submit_device", "cm_job_finish", "cm_job_set_profiling",
-           "cm_job_set_tuning", "cm_job_kernel_stats", "cm_job_destroy", "cm_last_error", "cm_device_alloc", "cm_device_upload",
+           "cm_job_set_tuning", "cm_job_kernel_stats", "cm_job_launch_times", "cm_job_destroy", "cm_last_error", "cm_device_alloc", "cm_device_upload",
            "cm_device_free", "cm_device_synchronize", "cm_job_footprint", "cm_server_create", "cm_server_enqueue",
            "cm_server_wait", "cm_server_destroy", "cm_release_cached_memory"]
 
@@ -92,6 +92,7 @@ def lib():
         L.cm_job_set_profiling.argtypes = [ctypes.c_void_p, _i32]
         L.cm_job_set_tuning.argtypes = [ctypes.c_void_p, ctypes.c_double, _i32, _i32]
         L.cm_job_kernel_stats.argtypes = [ctypes.c_void_p, _P(KernelStats)]
+        L.cm_job_launch_times.argtypes = [ctypes.c_void_p, _i32, _P(_d), _P(_d), _P(_i32)]
         L.cm_job_destroy.argtypes = [ctypes.c_void_p]
         L.cm_job_destroy.restype = None
         L.cm_last_error.restype = ctypes.c_char_p
@@ -106,7 +107,7 @@ def lib():
         L.cm_server_destroy.restype = None
         L.cm_release_cached_memory.argtypes = []
         for f in ("cm_job_create", "cm_job_submit", "cm_job_submit_device", "cm_job_finish", "cm_job_set_profiling",
-                  "cm_job_set_tuning", "cm_job_kernel_stats", "cm_device_alloc", "cm_device_upload", "cm_device_free",
+                  "cm_job_set_tuning", "cm_job_kernel_stats", "cm_job_launch_times", "cm_device_alloc", "cm_device_upload", "cm_device_free",
                   "cm_device_synchronize", "cm_job_footprint", "cm_server_create", "cm_server_enqueue",
                   "cm_server_wait"):
             getattr(L, f).restype = ctypes.c_int
@@ -258,6 +259,16 @@ class Job:
         check(lib().cm_job_kernel_stats(self.h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in KernelStats._fields_}
 
+    def launch_times(self):
+        """Per profiled batch: (real-pass, random-pass) evaluation launch
+        durations in ms (-1 without random passes)."""
+        n = _i32()
+        check(lib().cm_job_launch_times(self.h, 0, None, None, ctypes.byref(n)))
+        real = np.zeros(max(n.value, 1))
+        rnd = np.zeros(max(n.value, 1))
+        check(lib().cm_job_launch_times(self.h, n.value, _dp(real), _dp(rnd), ctypes.byref(n)))
+        return real[:n.value], rnd[:n.value]
+
     def close(self):
         if self.h:
             lib().cm_job_destroy(self.h)
@@ -295,14 +306,22 @@ class Server:
                       int(t.get("columns", 0)), 0)
         ticket = _u64()
         check(lib().cm_server_enqueue(self.h, ctypes.byref(req), ctypes.byref(ticket)))
-        self._meta[ticket.value] = (int(job_kw["nbins"]), int(job_kw["n_groups_u"]), int(job_kw["n_groups_v"]))
+        # the request's kernels read both device arrays whenever a slot takes
+        # it: they stay referenced until wait() has returned its counts (or
+        # the server is closed)
+        self._meta[ticket.value] = (int(job_kw["nbins"]), int(job_kw["n_groups_u"]), int(job_kw["n_groups_v"]),
+                                    d_solute, d_solvent)
         return ticket.value
 
     def wait(self, ticket):
-        nb, ngu, ngv = self._meta.pop(ticket)
+        nb, ngu, ngv, _du, _dv = self._meta[ticket]
         out, hits, r = _raw_buffers(nb, ngu, ngv)
         tm = RequestTimes()
-        check(lib().cm_server_wait(self.h, _u64(ticket), ctypes.byref(r), ctypes.byref(tm)))
+        try:
+            check(lib().cm_server_wait(self.h, _u64(ticket), ctypes.byref(r), ctypes.byref(tm)))
+        finally:
+            # the request is complete (or refused) once cm_server_wait returns
+            self._meta.pop(ticket, None)
         out["volume_total"] = r.volume_total
         out.update(hits)
         return out, {"enqueued": tm.enqueued, "started": tm.started, "finished": tm.finished}
@@ -311,6 +330,7 @@ class Server:
         if self.h:
             lib().cm_server_destroy(self.h)
             self.h = ctypes.c_void_p()
+        self._meta = {}
 
     def __del__(self):
         try:

@@ -44,6 +44,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -2313,6 +2314,16 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
     }
 }
 
+// Fills n16 16-byte words and then ntail single bytes with zeros (a kernel,
+// not hipMemsetAsync: memset nodes captured in HIP graphs have given wrong
+// results on replays here, and a caller may capture these launches).
+__global__ void k_zero(uint4* p, int64_t n16, unsigned char* tail, int ntail)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n16; k += stride) p[k] = make_uint4(0u, 0u, 0u, 0u);
+    if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
 // integer hits -> FP64 accumulators
 __global__ void k_flush(ull* cnt, double* acc, ull* hits, int64_t n, int64_t nhist, int64_t half_lo, int64_t half_hi,
                         double w)
@@ -2357,6 +2368,19 @@ int grid_for(int64_t n)
     if (b < 1) b = 1;
     if (b > 4096) b = 4096;
     return (int)b;
+}
+
+// zeroes `bytes` bytes at p (16-byte aligned: every buffer here starts on a
+// 256-byte boundary) on stream s
+int zero_async(void* p, size_t bytes, hipStream_t s)
+{
+    if (!bytes) return CM_OK;
+    const int64_t n16 = (int64_t)(bytes / 16);
+    const int ntail = (int)(bytes - 16 * (size_t)n16);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n16 + BLOCK - 1) / BLOCK, 2048))),
+                       dim3(BLOCK), 0, s, (uint4*)p, n16, (unsigned char*)p + 16 * n16, ntail);
+    HIPCHK(hipGetLastError());
+    return CM_OK;
 }
 
 struct DevBuf {
@@ -2501,16 +2525,35 @@ void pin_put(PinBuf& b)
     b.bytes = 0;
 }
 
-// grows a job buffer; a block being replaced may still be read by the job's
-// stream, so it is freed (hipFree waits for the device), not cached
-int ensure(DevBuf& b, size_t bytes)
+// Grows a job buffer.  A block being replaced may still be read by work
+// queued on the job's stream: the host waits for that stream only (not the
+// device, as hipFree would, which would stall the other slots of a request
+// server), then the block goes back to the cache for later jobs.
+int ensure(hipStream_t s, DevBuf& b, size_t bytes)
 {
     if (bytes == 0) bytes = 16;
     if (b.bytes >= bytes) return CM_OK;
-    if (b.p) (void)hipFree(b.p);
+    if (b.p) {
+        if (s) {
+            const hipError_t e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return fail(CM_EHIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+        }
+        dev_put(b);
+    }
     b.p = nullptr;
     b.bytes = 0;
     return dev_take(b, bytes);
+}
+
+// host memory the runtime knows as page-locked (hipHostMalloc'd or registered)
+bool page_locked(const void* p)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();   // pageable memory is unknown to the runtime
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
 }
 
 void plane_spacings(const double* H, double sp[3])
@@ -2552,7 +2595,13 @@ struct cm_job {
     int cap_cells = 0;
     bool finished = false;
     bool profiling = false;
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    // profiling: five events per batch (batch start, real-pass evaluation
+    // start/end, random-pass evaluation start/end), recorded without a host
+    // wait and resolved when the statistics are read
+    std::vector<std::array<hipEvent_t, 5>> prof_ev;
+    size_t prof_used = 0, prof_done = 0;
+    std::vector<char> prof_random;                     // the batch had random passes
+    std::vector<std::pair<float, float>> launch_ms;   // per batch: real, random evaluation (ms; -1 when absent)
     hipEvent_t rb_ev[RB_PIECES + 1] = {};   // pipelined read-back (cm_job_finish)
     double eval_ms = 0.0, total_ms = 0.0;
     int64_t eval_launches = 0, total_launches = 0, frames = 0;
@@ -2745,12 +2794,12 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         HIPCHK(hipMemcpyAsync(J->drawn_ptr.p, J->hdp.data(), sizeof(int) * J->hdp.size(), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(J->drawn_idx.p, J->hdi.data(), sizeof(int) * J->hdi.size(), hipMemcpyHostToDevice, s));
     } else if (J->K > 0) {
-        HIPCHK(hipMemsetAsync(J->uk.p, 0, sizeof(int) * (size_t)F * J->K, s));
+        if (int r = zero_async(J->uk.p, sizeof(int) * (size_t)F * J->K, s)) return r;
     }
-    HIPCHK(hipMemsetAsync(J->cells_cnt.p, 0, sizeof(int) * (size_t)F * (cap + 1), s));
+    if (int r = zero_async(J->cells_cnt.p, sizeof(int) * (size_t)F * (cap + 1), s)) return r;
     if (tiles) {
-        HIPCHK(hipMemsetAsync(J->near.p, 0, (size_t)F * cap, s));
-        HIPCHK(hipMemsetAsync(J->mcnt.p, 0, sizeof(int) * (size_t)F * P.NP * (kcap + 1), s));
+        if (int r = zero_async(J->near.p, (size_t)F * cap, s)) return r;
+        if (int r = zero_async(J->mcnt.p, sizeof(int) * (size_t)F * P.NP * (kcap + 1), s)) return r;
     }
 
     const bool f32 = dtype == CM_F32;
@@ -2759,10 +2808,17 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
     const double* ud = (const double*)xu_dev;
     const double* vd = (const double*)xv_dev;
     const size_t shm = sizeof(unsigned) * 2 * P.nbins;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     if (J->profiling) {
-        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-        HIPCHK(hipEventRecord(J->e0, s));
+        if (J->prof_used == J->prof_ev.size()) {
+            std::array<hipEvent_t, 5> a{};
+            for (auto& e : a) HIPCHK(hipEventCreate(&e));
+            J->prof_ev.push_back(a);
+        }
+        for (int k = 0; k < 5; k++) ev[k] = J->prof_ev[J->prof_used][k];
+        J->prof_used++;
+        HIPCHK(hipEventRecord(ev[4], s));
+        J->prof_random.push_back(P.K > 0 ? 1 : 0);
     }
     const int mols = tiles ? 0 : 1;
     {
@@ -2865,24 +2921,6 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
     // batch here and prepares the next one while it runs
     hipLaunchKernelGGL(k_geo_flags, dim3(1), dim3(64), 0, s, P, B);
     HIPCHK(hipGetLastError());
-    if (J->profiling) {
-        HIPCHK(hipStreamSynchronize(s));
-        float ms = 0.f, tot = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
-        J->eval_ms += ms;
-        J->eval_launches += 1;
-        if (P.K > 0) {
-            HIPCHK(hipEventElapsedTime(&ms, ev[2], ev[3]));
-            J->eval_ms += ms;
-            J->eval_launches += 1;
-            HIPCHK(hipEventElapsedTime(&tot, J->e0, ev[3]));
-        } else {
-            HIPCHK(hipEventElapsedTime(&tot, J->e0, ev[1]));
-        }
-        J->total_ms += tot;
-        J->total_launches += 1;
-        for (auto& e : ev) hipEventDestroy(e);
-    }
     J->frames += F;
     return CM_OK;
 }
@@ -2892,44 +2930,44 @@ int alloc_batch(cm_job* J, int F, int64_t cap, int64_t kcap, int dtype_size, boo
     int r;
     int64_t Mu = J->d.nmols_u, Mv = J->d.nmols_v, NP = 1 + J->K;
     int64_t items = J->atoms_family ? J->d.natoms_per_mol_u : Mu;
-    if ((r = ensure(J->geo, sizeof(Geo) * F))) return r;
-    if ((r = ensure(J->cells_cnt, sizeof(int) * F * (cap + 1)))) return r;
-    if ((r = ensure(J->cells_start, sizeof(int) * F * (cap + 1)))) return r;
-    if ((r = ensure(J->tmp4, 16 * F * items))) return r;
-    if ((r = ensure(J->tmpcell, 4 * F * items))) return r;
-    if ((r = ensure(J->sorted, 16 * F * items))) return r;
-    if ((r = ensure(J->vw, 16 * F * J->Nv))) return r;
+    if ((r = ensure(J->stream, J->geo, sizeof(Geo) * F))) return r;
+    if ((r = ensure(J->stream, J->cells_cnt, sizeof(int) * F * (cap + 1)))) return r;
+    if ((r = ensure(J->stream, J->cells_start, sizeof(int) * F * (cap + 1)))) return r;
+    if ((r = ensure(J->stream, J->tmp4, 16 * F * items))) return r;
+    if ((r = ensure(J->stream, J->tmpcell, 4 * F * items))) return r;
+    if ((r = ensure(J->stream, J->sorted, 16 * F * items))) return r;
+    if ((r = ensure(J->stream, J->vw, 16 * F * J->Nv))) return r;
     if (!J->atoms_family) {
-        if ((r = ensure(J->solw, 16 * F * J->Nu))) return r;
-        if ((r = ensure(J->ucen, 16 * F * Mu))) return r;
-        if ((r = ensure(J->drawn_ptr, 4 * F * (Mu + 1)))) return r;
-        if ((r = ensure(J->drawn_idx, 4 * F * std::max(J->K, 1)))) return r;
-        if ((r = ensure(J->vcell, 4 * F * Mv))) return r;
-        if ((r = ensure(J->vcen, 16 * F * Mv))) return r;
+        if ((r = ensure(J->stream, J->solw, 16 * F * J->Nu))) return r;
+        if ((r = ensure(J->stream, J->ucen, 16 * F * Mu))) return r;
+        if ((r = ensure(J->stream, J->drawn_ptr, 4 * F * (Mu + 1)))) return r;
+        if ((r = ensure(J->stream, J->drawn_idx, 4 * F * std::max(J->K, 1)))) return r;
+        if ((r = ensure(J->stream, J->vcell, 4 * F * Mv))) return r;
+        if ((r = ensure(J->stream, J->vcen, 16 * F * Mv))) return r;
     } else {
-        if ((r = ensure(J->near, (size_t)F * cap))) return r;
-        if ((r = ensure(J->mkey, (size_t)4 * F * NP * Mv))) return r;
-        if ((r = ensure(J->mcnt, (size_t)4 * F * NP * (kcap + 1)))) return r;
-        if ((r = ensure(J->mstart, (size_t)4 * F * NP * (kcap + 1)))) return r;
-        if ((r = ensure(J->msorted, (size_t)4 * F * NP * Mv))) return r;
-        if ((r = ensure(J->nnear, (size_t)4 * F * NP))) return r;
-        if ((r = ensure(J->chunk_base, (size_t)4 * (F * std::max(J->K, 1) + 1)))) return r;
+        if ((r = ensure(J->stream, J->near, (size_t)F * cap))) return r;
+        if ((r = ensure(J->stream, J->mkey, (size_t)4 * F * NP * Mv))) return r;
+        if ((r = ensure(J->stream, J->mcnt, (size_t)4 * F * NP * (kcap + 1)))) return r;
+        if ((r = ensure(J->stream, J->mstart, (size_t)4 * F * NP * (kcap + 1)))) return r;
+        if ((r = ensure(J->stream, J->msorted, (size_t)4 * F * NP * Mv))) return r;
+        if ((r = ensure(J->stream, J->nnear, (size_t)4 * F * NP))) return r;
+        if ((r = ensure(J->stream, J->chunk_base, (size_t)4 * (F * std::max(J->K, 1) + 1)))) return r;
     }
     if (J->K > 0) {
-        if ((r = ensure(J->P, (size_t)24 * F * J->K * J->Nv))) return r;
-        if ((r = ensure(J->vw_r, (size_t)16 * F * J->K * J->Nv))) return r;
+        if ((r = ensure(J->stream, J->P, (size_t)24 * F * J->K * J->Nv))) return r;
+        if ((r = ensure(J->stream, J->vw_r, (size_t)16 * F * J->K * J->Nv))) return r;
         if (!J->atoms_family) {
-            if ((r = ensure(J->vcell_r, (size_t)4 * F * J->K * Mv))) return r;
-            if ((r = ensure(J->vcen_r, (size_t)16 * F * J->K * Mv))) return r;
+            if ((r = ensure(J->stream, J->vcell_r, (size_t)4 * F * J->K * Mv))) return r;
+            if ((r = ensure(J->stream, J->vcen_r, (size_t)16 * F * J->K * Mv))) return r;
         }
-        if ((r = ensure(J->bflag, (size_t)F * J->kb * Mv))) return r;
-        if ((r = ensure(J->blist, (size_t)4 * F * J->kb * Mv))) return r;
-        if ((r = ensure(J->nbulk, (size_t)4 * F * J->kb))) return r;
-        if ((r = ensure(J->uk, (size_t)4 * F * J->K))) return r;
+        if ((r = ensure(J->stream, J->bflag, (size_t)F * J->kb * Mv))) return r;
+        if ((r = ensure(J->stream, J->blist, (size_t)4 * F * J->kb * Mv))) return r;
+        if ((r = ensure(J->stream, J->nbulk, (size_t)4 * F * J->kb))) return r;
+        if ((r = ensure(J->stream, J->uk, (size_t)4 * F * J->K))) return r;
     }
     if (host_input) {
-        if ((r = ensure(J->xin_u, (size_t)dtype_size * F * J->Nu * 3))) return r;
-        if ((r = ensure(J->xin_v, (size_t)dtype_size * F * J->Nv * 3))) return r;
+        if ((r = ensure(J->stream, J->xin_u, (size_t)dtype_size * F * J->Nu * 3))) return r;
+        if ((r = ensure(J->stream, J->xin_v, (size_t)dtype_size * F * J->Nv * 3))) return r;
     }
     return CM_OK;
 }
@@ -3022,10 +3060,15 @@ int submit_common(cm_job* J, int32_t nframes, const int64_t* frame_index, const 
         const void* du;
         const void* dv;
         if (host_input) {
-            HIPCHK(hipMemcpyAsync(J->xin_u.p, (const char*)xu + (size_t)f0 * J->Nu * 3 * ds, (size_t)F * J->Nu * 3 * ds,
-                                  hipMemcpyHostToDevice, J->stream));
-            HIPCHK(hipMemcpyAsync(J->xin_v.p, (const char*)xv + (size_t)f0 * J->Nv * 3 * ds, (size_t)F * J->Nv * 3 * ds,
-                                  hipMemcpyHostToDevice, J->stream));
+            const void* su = (const char*)xu + (size_t)f0 * J->Nu * 3 * ds;
+            const void* sv = (const char*)xv + (size_t)f0 * J->Nv * 3 * ds;
+            HIPCHK(hipMemcpyAsync(J->xin_u.p, su, (size_t)F * J->Nu * 3 * ds, hipMemcpyHostToDevice, J->stream));
+            HIPCHK(hipMemcpyAsync(J->xin_v.p, sv, (size_t)F * J->Nv * 3 * ds, hipMemcpyHostToDevice, J->stream));
+            // Copies from pageable memory are staged before hipMemcpyAsync
+            // returns; copies from page-locked memory run later, in stream
+            // order.  The caller may reuse its arrays once the call returns,
+            // so page-locked sources are waited for.
+            if (page_locked(su) || page_locked(sv)) HIPCHK(hipStreamSynchronize(J->stream));
             du = J->xin_u.p;
             dv = J->xin_v.p;
         } else {
@@ -3092,6 +3135,9 @@ void job_params(cm_job* J, const cm_job_desc& d)
     J->profiling = false;
     J->eval_ms = J->total_ms = 0.0;
     J->eval_launches = J->total_launches = J->frames = 0;
+    J->prof_used = J->prof_done = 0;
+    J->prof_random.clear();
+    J->launch_ms.clear();
     J->cell_frac = 0.5;
     J->eval_grid = 0;
     // tiles family: one solute molecule, solvent molecules that fit a wavefront
@@ -3158,14 +3204,14 @@ int job_setup(cm_job* J, const cm_job_desc& d)
     const int64_t Nv = J->Nv;
     const int64_t nb = d.nbins;
     int r;
-    if ((r = ensure(J->cnt, 8 * J->acc_len))) return r;
-    if ((r = ensure(J->acc, 8 * J->acc_len))) return r;
-    if ((r = ensure(J->hits, 8 * 4 * nb))) return r;
-    if ((r = ensure(J->stats, 64))) return r;
-    HIPCHK(hipMemsetAsync(J->cnt.p, 0, 8 * J->acc_len, J->stream));
-    HIPCHK(hipMemsetAsync(J->acc.p, 0, 8 * J->acc_len, J->stream));
-    HIPCHK(hipMemsetAsync(J->hits.p, 0, 8 * 4 * nb, J->stream));
-    HIPCHK(hipMemsetAsync(J->stats.p, 0, 64, J->stream));
+    if ((r = ensure(J->stream, J->cnt, 8 * J->acc_len))) return r;
+    if ((r = ensure(J->stream, J->acc, 8 * J->acc_len))) return r;
+    if ((r = ensure(J->stream, J->hits, 8 * 4 * nb))) return r;
+    if ((r = ensure(J->stream, J->stats, 64))) return r;
+    if ((r = zero_async(J->cnt.p, 8 * J->acc_len, J->stream))) return r;
+    if ((r = zero_async(J->acc.p, 8 * J->acc_len, J->stream))) return r;
+    if ((r = zero_async(J->hits.p, 8 * 4 * nb, J->stream))) return r;
+    if ((r = zero_async(J->stats.p, 64, J->stream))) return r;
     // group maps
     std::vector<int32_t>& gm = J->hgm;
     gm.clear();
@@ -3181,7 +3227,7 @@ int job_setup(cm_job* J, const cm_job_desc& d)
         ov_i = gm.size(); gm.insert(gm.end(), d.groups_v_idx, d.groups_v_idx + m); gm.push_back(0);
     }
     if (!gm.empty()) {
-        if ((r = ensure(J->gmap, 4 * gm.size()))) return r;
+        if ((r = ensure(J->stream, J->gmap, 4 * gm.size()))) return r;
         HIPCHK(hipMemcpyAsync(J->gmap.p, gm.data(), 4 * gm.size(), hipMemcpyHostToDevice, J->stream));
         HIPCHK(hipStreamSynchronize(J->stream));
         int32_t* base = (int32_t*)J->gmap.p;
@@ -3189,6 +3235,19 @@ int job_setup(cm_job* J, const cm_job_desc& d)
         if (ov_p >= 0) { P.gv_ptr = base + ov_p; P.gv_idx = base + ov_i; }
     }
     J->d.groups_u_ptr = J->d.groups_u_idx = J->d.groups_v_ptr = J->d.groups_v_idx = nullptr;
+    return CM_OK;
+}
+
+// The geometry error bits a job's batches left on the device (k_geo_flags):
+// bit 0 molecules too large for the cell of the molecule-pair kernels, bit 1
+// a cell grid larger than its allocation.  The same decoding for cm_job_finish
+// and for the request server.
+int geometry_error(uint64_t gerr)
+{
+    if (gerr & 1)
+        return fail(CM_EUNSUPPORTED, "molecules too large for the unit cell: cutoff plus molecule radii exceed a plane spacing");
+    if (gerr)
+        return fail(CM_EHIP, "internal error: cell grid larger than its allocation");
     return CM_OK;
 }
 
@@ -3230,10 +3289,7 @@ int job_finish_pipelined(cm_job* J, cm_raw* out)
     HIPCHK(hipEventSynchronize(J->rb_ev[RB_PIECES]));
     uint64_t gerr;
     memcpy(&gerr, h + off_err, 8);
-    if (gerr & 1)
-        return fail(CM_EUNSUPPORTED, "molecules too large for the unit cell: cutoff plus molecule radii exceed a plane spacing");
-    if (gerr)
-        return fail(CM_EHIP, "internal error: cell grid larger than its allocation");
+    if (int r = geometry_error(gerr)) return r;
     // the caller's arrays as byte ranges of the accumulator block, in order
     struct Seg { size_t o, n; char* dst; };
     std::vector<Seg> segs;
@@ -3308,10 +3364,7 @@ int job_collect(cm_job* J)
     HIPCHK(hipStreamSynchronize(J->stream));
     uint64_t gerr;
     memcpy(&gerr, (const char*)J->hres.p + off_err, 8);
-    if (gerr & 1)
-        return fail(CM_EUNSUPPORTED, "molecules too large for the unit cell: cutoff plus molecule radii exceed a plane spacing");
-    if (gerr)
-        return fail(CM_EHIP, "internal error: cell grid larger than its allocation");
+    if (int r = geometry_error(gerr)) return r;
     J->finished = true;
     return CM_OK;
 }
@@ -3362,6 +3415,29 @@ void fill_raw(const cm_job_desc& d, int64_t acc_len, const PinBuf& res, double v
     cpu(out->hits_rdf_random, hits + 3 * nb);
 }
 
+// Times of the batches profiled since the last call: waits for the job's
+// stream, reads the event pairs and adds them to the job's totals.
+int prof_resolve(cm_job* J)
+{
+    if (J->prof_done == J->prof_used) return CM_OK;
+    HIPCHK(hipStreamSynchronize(J->stream));
+    for (size_t b = J->prof_done; b < J->prof_used; b++) {
+        const auto& e = J->prof_ev[b];
+        const bool rnd = J->prof_random[b] != 0;
+        float real = 0.f, ran = -1.f, tot = 0.f;
+        HIPCHK(hipEventElapsedTime(&real, e[0], e[1]));
+        if (rnd) HIPCHK(hipEventElapsedTime(&ran, e[2], e[3]));
+        HIPCHK(hipEventElapsedTime(&tot, e[4], rnd ? e[3] : e[1]));
+        J->eval_ms += real + (rnd ? ran : 0.f);
+        J->eval_launches += rnd ? 2 : 1;
+        J->total_ms += tot;
+        J->total_launches += 1;
+        J->launch_ms.emplace_back(real, ran);
+    }
+    J->prof_done = J->prof_used;
+    return CM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -3409,11 +3485,6 @@ int cm_job_finish(cm_job* J, cm_raw* out)
 int cm_job_set_profiling(cm_job* J, int32_t on)
 {
     if (!J) return fail(CM_EINVAL, "null job");
-    if (on && !J->e0) {
-        HIPCHK(hipEventCreate(&J->e0));
-        HIPCHK(hipEventCreate(&J->e1));
-        HIPCHK(hipEventCreate(&J->e2));
-    }
     J->profiling = on != 0;
     return CM_OK;
 }
@@ -3438,6 +3509,7 @@ int cm_job_set_tuning(cm_job* J, double cell_fraction, int32_t columns, int32_t 
 int cm_job_kernel_stats(cm_job* J, cm_kernel_stats* out)
 {
     if (!J || !out) return fail(CM_EINVAL, "null argument");
+    if (int r = prof_resolve(J)) return r;
     ull st[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(st, J->stats.p, 16, hipMemcpyDeviceToHost, J->stream));
     HIPCHK(hipStreamSynchronize(J->stream));
@@ -3448,6 +3520,18 @@ int cm_job_kernel_stats(cm_job* J, cm_kernel_stats* out)
     out->frames = J->frames;
     out->candidate_pairs = -1;
     out->fp64_rechecks = (int64_t)st[0];
+    return CM_OK;
+}
+
+int cm_job_launch_times(cm_job* J, int32_t cap, double* real_ms, double* random_ms, int32_t* n)
+{
+    if (!J || !n || (cap > 0 && (!real_ms || !random_ms))) return fail(CM_EINVAL, "null argument");
+    if (int r = prof_resolve(J)) return r;
+    *n = (int32_t)J->launch_ms.size();
+    for (int32_t k = 0; k < cap && k < *n; k++) {
+        real_ms[k] = J->launch_ms[k].first;
+        random_ms[k] = J->launch_ms[k].second;
+    }
     return CM_OK;
 }
 
@@ -3462,9 +3546,8 @@ void cm_job_destroy(cm_job* J)
                       &J->msorted, &J->nnear, &J->chunk_base};
     for (DevBuf* b : bufs) dev_put(*b);
     pin_put(J->hres);
-    if (J->e0) hipEventDestroy(J->e0);
-    if (J->e1) hipEventDestroy(J->e1);
-    if (J->e2) hipEventDestroy(J->e2);
+    for (auto& a : J->prof_ev)
+        for (auto& e : a) hipEventDestroy(e);
     for (auto& e : J->rb_ev)
         if (e) hipEventDestroy(e);
     if (J->stream) hipStreamDestroy(J->stream);
@@ -3525,9 +3608,10 @@ struct SrvReq {
     int32_t cols = 0;
     int64_t foot = 0;
     bool done = false;
+    bool waited = false;             // a cm_server_wait has taken the ticket
     int status = CM_OK;
     std::string err;
-    PinBuf res;                      // read-back accumulators (taken over from the slot's job)
+    PinBuf res;                     // read-back accumulators (taken over from the slot's job)
     int64_t acc_len = 0;
     double volume = 0.0;
     cm_request_times t{};
@@ -3730,10 +3814,14 @@ int cm_server_wait(cm_server* S, uint64_t ticket, cm_raw* out, cm_request_times*
     {
         std::unique_lock<std::mutex> lk(S->m);
         auto it = S->reqs.find(ticket);
-        if (it == S->reqs.end()) return fail(CM_EINVAL, "unknown ticket");
+        if (it == S->reqs.end() || it->second->waited) return fail(CM_EINVAL, "unknown ticket");
         q = it->second;
+        // a second waiter on the same ticket is refused while this one waits
+        q->waited = true;
+        // the map may rehash while the lock is released inside wait (an
+        // enqueue on another thread inserts): the entry is erased by key
         S->cv_done.wait(lk, [&] { return q->done; });
-        S->reqs.erase(it);
+        S->reqs.erase(ticket);
     }
     if (times) *times = q->t;
     const int st = q->status;

@@ -143,9 +143,15 @@ int cm_job_finish(cm_job *job, cm_raw *out);
  * holds at once).  Results do not depend on these. */
 int cm_job_set_tuning(cm_job *job, double cell_fraction, int32_t columns, int32_t eval_workgroups);
 
-/* Kernel timing of the job (HIP events on the job's stream). */
+/* Kernel timing of the job (HIP events on the job's stream, recorded around
+ * each batch's launches without a host wait and read when the statistics are
+ * asked for).  No counterpart in the reference. */
 int cm_job_set_profiling(cm_job *job, int32_t on);
 int cm_job_kernel_stats(cm_job *job, cm_kernel_stats *out);
+/* Per profiled batch, in submission order: duration of the real-pass and of
+ * the random-pass evaluation launch (ms; -1 when the batch had no random
+ * passes).  Writes min(cap, batches) entries and the number of batches to *n. */
+int cm_job_launch_times(cm_job *job, int32_t cap, double *real_ms, double *random_ms, int32_t *n);
 
 void cm_job_destroy(cm_job *job);
 

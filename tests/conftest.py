@@ -102,3 +102,17 @@ def assert_raw_equal(a, b, keys=RAW_KEYS, rtol=0.0):
             np.testing.assert_allclose(x, y, rtol=rtol, atol=0, err_msg=k)
     assert a["volume_total"] == b["volume_total"] or rtol and np.isclose(a["volume_total"], b["volume_total"],
                                                                          rtol=rtol)
+
+
+def assert_group_split(raw, autocorrelation):
+    """Default groups: every record adds its weight once to md_count and, over
+    the groups, once to the solute side (self: w/2 for atom i and w/2 for atom
+    j, src/update_counters.jl:48-53) and, for cross jobs, once to the solvent
+    side; the same for the ideal-gas arrays."""
+    for md, gu, gv in (("md_count", "solute_group_count", "solvent_group_count"),
+                       ("md_count_random", "solute_group_count_random", "solvent_group_count_random")):
+        assert np.array_equal(np.asarray(raw[gu]).sum(axis=0), raw[md]), gu
+        if autocorrelation:
+            assert not np.asarray(raw[gv]).any(), gv
+        else:
+            assert np.array_equal(np.asarray(raw[gv]).sum(axis=0), raw[md]), gv

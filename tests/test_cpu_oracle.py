@@ -356,3 +356,61 @@ def test_finalresults_reproduces_goldens(oracle, pkg, key):
     for k in ("md_count", "md_count_random", "coordination_number", "mddf", "kb", "rdf", "kb_rdf"):
         assert np.array_equal(getattr(R, k), fin[k]), k
     assert R.volume.domain == fin["volume_domain"] and R.density.solvent_bulk == fin["density_solvent_bulk"]
+
+
+# ---------------------------------------------------------------------------
+# group-count split (update_counters!, src/update_counters.jl:43-88)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("key", ["NAMD__tmao_tmao", "NAMD__water_tmao", "NAMD__water_water", "PDB__tmao_tmao",
+                                 "Gromacs__EMI_DCA", "Gromacs__EMI_EMI"])
+def test_group_split_against_goldens(key):
+    """Every record within the cutoff adds its weight w to md_count once and,
+    summed over the groups, w to the solute groups: cross MDDFs add w for
+    atom i to the solute groups and w for atom j to the solvent groups; self
+    MDDFs add w/2 for atom i and w/2 for atom j, both to the solute groups
+    (src/update_counters.jl:48-53, 72-77; finalresults! copies them to the
+    solvent side, src/results.jl:335-337).  The reference's own Result files
+    fix this: sum_g solute_group_count[g] == md_count bin by bin, and the same
+    for the ideal-gas arrays.  A rule with w instead of w/2 in self mode gives
+    twice md_count; one that sends atom j to the solvent groups gives half."""
+    meta, arr = _goldens()
+    m = meta[key]
+    assert not m["solute"]["custom_groups"] and not m["solvent"]["custom_groups"]
+    g = {k.split(".", 1)[1]: arr[k] for k in arr.files if k.startswith(key + ".")}
+    for md, gu, gv in (("md_count", "solute_group_count", "solvent_group_count"),
+                       ("md_count_random", "solute_group_count_random", "solvent_group_count_random")):
+        scale = np.max(g[md])
+        assert scale > 0
+        np.testing.assert_allclose(g[gu].sum(axis=0), g[md], rtol=0, atol=1e-12 * scale, err_msg=f"{key} {gu}")
+        np.testing.assert_allclose(g[gv].sum(axis=0), g[md], rtol=0, atol=1e-12 * scale, err_msg=f"{key} {gv}")
+        if m["autocorrelation"]:
+            assert np.array_equal(g[gu], g[gv])
+
+
+@pytest.mark.parametrize("auto", [False, True])
+def test_group_split_oracle_raw(oracle, pkg, auto):
+    """The oracle's raw accumulators satisfy the same split: with unit weights
+    sum_g gu == md_count exactly (self: half units for atom i and atom j),
+    sum_g gv == md_count for cross jobs and gv untouched for self jobs (the
+    reference copies the solute groups at finalisation)."""
+    import complexmixtures_amd.synthetic as S
+    rng = np.random.default_rng(61)
+    box = 24.0
+    x = S.place_molecules(rng, S.water_template(), rng.uniform(0, box, (300, 3)))
+    xv = np.stack([x + rng.normal(scale=0.2, size=x.shape) for _ in range(2)])
+    cells = np.repeat(np.diag([box] * 3)[None], 2, axis=0)
+    if auto:
+        job = oracle.Job(300, 3, 300, 3, autocorrelation=True, cutoff=8.0, dbulk=8.0, n_random_samples=3, seed=9)
+        raw = oracle.run(job, [1, 2], xv, xv, cells, np.ones(2))
+    else:
+        xu = xv[:, :30]
+        job = oracle.Job(10, 3, 290, 3, cutoff=8.0, dbulk=8.0, n_random_samples=3, seed=9)
+        raw = oracle.run(job, [1, 2], xu, xv[:, 30:], cells, np.ones(2))
+    for md, gu, gv in (("md_count", "gu", "gv"), ("md_count_random", "gu_random", "gv_random")):
+        assert raw[md].sum() > 0
+        assert np.array_equal(raw[gu].sum(axis=0), raw[md])
+        if auto:
+            assert not raw[gv].any()
+            assert np.all(raw[gu] * 2 == np.round(raw[gu] * 2))
+        else:
+            assert np.array_equal(raw[gv].sum(axis=0), raw[md])

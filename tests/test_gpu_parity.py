@@ -11,7 +11,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, assert_raw_equal, gpu_raw, oracle_raw
+from conftest import GOLDEN, assert_group_split, assert_raw_equal, gpu_raw, oracle_raw
 
 pytestmark = pytest.mark.gpu
 
@@ -165,6 +165,7 @@ def test_dupdcd_self_tmao_and_weights(pkg, oracle):
     a, _ = oracle_raw(pkg, oracle, traj, opts)
     b = gpu_raw(pkg, traj, opts)
     assert_raw_equal(a, b)
+    assert_group_split(b, True)
     R1 = pkg.mddf(traj, pkg.Options(silent=True, n_random_samples=4))
     R2 = pkg.mddf(traj, pkg.Options(silent=True, n_random_samples=4), frame_weights=[2.0, 0.0, 1.0])
     assert np.array_equal(R1.md_count, R2.md_count)
@@ -183,6 +184,7 @@ def test_parity_config1_reduced(pkg, oracle):
     a, _ = oracle_raw(pkg, oracle, traj, opts)
     b = gpu_raw(pkg, traj, opts)
     assert_raw_equal(a, b)
+    assert_group_split(b, False)
     assert b["hits_md"].sum() == round(a["md_count"].sum())
 
 
@@ -213,6 +215,7 @@ def test_parity_self_urea_triclinic(pkg, oracle):
     a, _ = oracle_raw(pkg, oracle, traj, opts)
     b = gpu_raw(pkg, traj, opts)
     assert_raw_equal(a, b)
+    assert_group_split(b, True)
 
 
 def test_parity_cross_many_solute_molecules(pkg, oracle):
@@ -232,6 +235,7 @@ def test_parity_cross_many_solute_molecules(pkg, oracle):
         a, _ = oracle_raw(pkg, oracle, traj, opts)
         b = gpu_raw(pkg, traj, opts)
         assert_raw_equal(a, b)
+        assert_group_split(b, False)
 
 
 def test_parity_custom_groups_and_usecutoff(pkg, oracle):
@@ -434,6 +438,7 @@ def test_molecule_pair_queue_self_many_groups(pkg, oracle):
     a, _ = oracle_raw(pkg, oracle, traj, opts)
     b = gpu_raw(pkg, traj, opts)
     assert_raw_equal(a, b)
+    assert_group_split(b, True)
 
 
 @pytest.mark.parametrize("nv", [64, 65])
@@ -488,6 +493,7 @@ def test_self_coordination_numbers(pkg, oracle):
     a, _ = oracle_raw(pkg, oracle, traj, opts, coordination_number_only=True)
     b = gpu_raw(pkg, traj, opts, coordination_number_only=True)
     assert_raw_equal(a, b)
+    assert_group_split(b, True)
 
 
 @pytest.mark.parametrize("irefatom", [2, 3])
@@ -537,6 +543,7 @@ def test_self_water_half_shell_with_random_passes(pkg, oracle):
     a, _ = oracle_raw(pkg, oracle, traj, opts)
     b = gpu_raw(pkg, traj, opts)
     assert_raw_equal(a, b)
+    assert_group_split(b, True)
 
 
 def test_self_water_walk_split_does_not_change_results():
