@@ -1,6 +1,8 @@
 """Geometry the molecule-pair kernels do not handle, detected on the device:
 rigid molecules so long that the cutoff plus two molecule radii exceed a plane
-spacing of the cell (while the cutoff itself is below half of it).  The
+spacing of the cell (while the cutoff itself is below half of it).  A 50 A box
+holds the same 22.5 A rods (whole under the minimum image, 10 + 2 x 11.25 A
+below the spacing) and is accepted.  The
 kernels still run within their buffers (the molecule grid degenerates to one
 cell per dimension); the job reports CM_EUNSUPPORTED when it is collected,
 from cm_job_finish (mddf) and from cm_server_wait (request server)."""
@@ -32,7 +34,7 @@ def test_long_molecules_rejected_by_finish(pkg):
     with pytest.raises(_lib.GPUError, match=MSG):
         pkg.mddf(traj, pkg.Options(dbulk=10.0, n_random_samples=2, seed=3, silent=True))
     # a cell with room for the molecules is accepted
-    xs2, cells2 = _rods(box=40.0)
+    xs2, cells2 = _rods(box=50.0)
     traj2 = pkg.MemoryTrajectory(xs2, xs2, cells2, sel, sel)
     R = pkg.mddf(traj2, pkg.Options(dbulk=10.0, n_random_samples=2, seed=3, silent=True))
     assert R.md_count.sum() > 0
@@ -52,7 +54,7 @@ def test_long_molecules_rejected_by_server(pkg):
         with pytest.raises(_lib.GPUError, match=MSG):
             srv.wait(tk)
         # the slot serves the next request normally
-        xs2, cells2 = _rods(box=40.0)
+        xs2, cells2 = _rods(box=50.0)
         d2 = _lib.DeviceArray(xs2.astype(np.float32))
         tk = srv.enqueue(np.arange(1, 3), d2, d2, 0, cells2, np.ones(2), **kw)
         raw, _ = srv.wait(tk)
