@@ -62,7 +62,8 @@ class RequestTimes(ctypes.Structure):
 EXPORTS = ["cm_job_create", "cm_job_submit", "cm_job_submit_device", "cm_job_finish", "cm_job_set_profiling",
            "cm_job_set_tuning", "cm_job_kernel_stats", "cm_job_launch_times", "cm_job_destroy", "cm_last_error", "cm_device_alloc", "cm_device_upload",
            "cm_device_free", "cm_device_synchronize", "cm_job_footprint", "cm_server_create", "cm_server_enqueue",
-           "cm_server_wait", "cm_server_destroy", "cm_release_cached_memory"]
+           "cm_server_wait", "cm_server_destroy", "cm_release_cached_memory", "cm_device_download",
+           "cm_mddf_noalloc"]
 
 _lib = None
 
@@ -106,10 +107,13 @@ def lib():
         L.cm_server_destroy.argtypes = [ctypes.c_void_p]
         L.cm_server_destroy.restype = None
         L.cm_release_cached_memory.argtypes = []
+        L.cm_device_download.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.cm_mddf_noalloc.argtypes = [_P(JobDesc), _i32, _P(_i64), ctypes.c_void_p, ctypes.c_void_p, _i32, _P(_d),
+                                      _P(_d), ctypes.c_void_p, _P(_d), ctypes.c_void_p]
         for f in ("cm_job_create", "cm_job_submit", "cm_job_submit_device", "cm_job_finish", "cm_job_set_profiling",
                   "cm_job_set_tuning", "cm_job_kernel_stats", "cm_job_launch_times", "cm_device_alloc", "cm_device_upload", "cm_device_free",
                   "cm_device_synchronize", "cm_job_footprint", "cm_server_create", "cm_server_enqueue",
-                  "cm_server_wait"):
+                  "cm_server_wait", "cm_device_download", "cm_mddf_noalloc"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -140,8 +144,17 @@ class DeviceArray:
         check(lib().cm_device_alloc(self.nbytes, ctypes.byref(self.ptr)))
         check(lib().cm_device_upload(self.ptr, host.ctypes.data_as(ctypes.c_void_p), self.nbytes))
 
+    @classmethod
+    def zeros(cls, shape, dtype=np.float64):
+        return cls(np.zeros(shape, dtype=dtype))
+
     def offset(self, nbytes):
         return ctypes.c_void_p(self.ptr.value + nbytes)
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(lib().cm_device_download(out.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes))
+        return out
 
     def free(self):
         if self.ptr:
@@ -337,3 +350,38 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def acc_len(nbins, n_groups_u, n_groups_v):
+    """Doubles of the FP64 accumulator block of cm_mddf_noalloc."""
+    return 4 * nbins + 2 * (n_groups_u + n_groups_v) * nbins
+
+
+def mddf_noalloc(frame_index, d_solute, d_solvent, frame0, cells, weights, d_acc, stream=None, **job_kw):
+    """cm_mddf_noalloc: one accumulation over device-resident frames whose
+    device buffers all come from the library's fixed scratch; d_acc is a
+    DeviceArray of acc_len(...) doubles (the output).  Returns (raw dict read
+    back from d_acc, volume_total)."""
+    d, _keep = job_desc(**job_kw)
+    nb, ngu, ngv = int(job_kw["nbins"]), int(job_kw["n_groups_u"]), int(job_kw["n_groups_v"])
+    if d_acc.dtype != np.float64 or d_acc.nbytes < 8 * acc_len(nb, ngu, ngv):
+        raise ArgumentError("d_acc must hold acc_len(nbins, n_groups_u, n_groups_v) float64 values")
+    dt = CM_F32 if d_solute.dtype == np.float32 else CM_F64
+    fi = np.ascontiguousarray(frame_index, dtype=np.int64)
+    c9 = _cells9(cells)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    fu = d_solute.nbytes // d_solute.shape[0]
+    fv = d_solvent.nbytes // d_solvent.shape[0]
+    vol = _d()
+    check(lib().cm_mddf_noalloc(ctypes.byref(d), len(fi), fi.ctypes.data_as(_P(_i64)), d_solute.offset(frame0 * fu),
+                                d_solvent.offset(frame0 * fv), dt, _dp(c9), _dp(w), d_acc.ptr, ctypes.byref(vol),
+                                stream))
+    a = d_acc.download().reshape(-1)[:acc_len(nb, ngu, ngv)]
+    out, o = {}, 0
+    for k, n in (("md_count", nb), ("md_count_random", nb), ("rdf_count", nb), ("rdf_count_random", nb),
+                 ("solute_group_count", ngu * nb), ("solvent_group_count", ngv * nb),
+                 ("solute_group_count_random", ngu * nb), ("solvent_group_count_random", ngv * nb)):
+        out[k] = a[o:o + n].reshape(-1, nb) if "group" in k else a[o:o + n].copy()
+        o += n
+    out["volume_total"] = vol.value
+    return out

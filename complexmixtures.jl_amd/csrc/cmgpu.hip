@@ -2525,26 +2525,6 @@ void pin_put(PinBuf& b)
     b.bytes = 0;
 }
 
-// Grows a job buffer.  A block being replaced may still be read by work
-// queued on the job's stream: the host waits for that stream only (not the
-// device, as hipFree would, which would stall the other slots of a request
-// server), then the block goes back to the cache for later jobs.
-int ensure(hipStream_t s, DevBuf& b, size_t bytes)
-{
-    if (bytes == 0) bytes = 16;
-    if (b.bytes >= bytes) return CM_OK;
-    if (b.p) {
-        if (s) {
-            const hipError_t e = hipStreamSynchronize(s);
-            if (e != hipSuccess) return fail(CM_EHIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
-        }
-        dev_put(b);
-    }
-    b.p = nullptr;
-    b.bytes = 0;
-    return dev_take(b, bytes);
-}
-
 // host memory the runtime knows as page-locked (hipHostMalloc'd or registered)
 bool page_locked(const void* p)
 {
@@ -2609,9 +2589,44 @@ struct cm_job {
     std::vector<int> huk, hdp, hdi;
     std::vector<int32_t> hgm;        // group maps (host copy of the upload)
     PinBuf hres;                     // accumulators and hit counts read back at finish (page-locked)
+    // buffers carved from a fixed device region instead of allocated (the
+    // entry point that allocates nothing); null for ordinary jobs
+    char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
 };
 
 namespace {
+
+// Grows a job buffer.  A block being replaced may still be read by work
+// queued on the job's stream: the host waits for that stream only (not the
+// device, as hipFree would, which would stall the other slots of a request
+// server), then the block goes back to the cache for later jobs.  Jobs with
+// an arena take the next 256-byte aligned piece of it instead (never given
+// back; the arena is reset per call).
+int ensure(cm_job* J, DevBuf& b, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    if (b.bytes >= bytes) return CM_OK;
+    if (J->arena) {
+        const size_t off = (J->arena_used + 255) & ~(size_t)255;
+        if (off > J->arena_cap || bytes > J->arena_cap - off)
+            return fail(CM_ENOMEM, "job does not fit the fixed device scratch of the no-allocation entry point");
+        b.p = J->arena + off;
+        b.bytes = bytes;
+        J->arena_used = off + bytes;
+        return CM_OK;
+    }
+    if (b.p) {
+        if (J->stream) {
+            const hipError_t e = hipStreamSynchronize(J->stream);
+            if (e != hipSuccess) return fail(CM_EHIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+        }
+        dev_put(b);
+    }
+    b.p = nullptr;
+    b.bytes = 0;
+    return dev_take(b, bytes);
+}
 
 int64_t per_frame_bytes(const cm_job* J, int dtype_size, bool host_input, int64_t cap, int64_t kcap)
 {
@@ -2930,44 +2945,44 @@ int alloc_batch(cm_job* J, int F, int64_t cap, int64_t kcap, int dtype_size, boo
     int r;
     int64_t Mu = J->d.nmols_u, Mv = J->d.nmols_v, NP = 1 + J->K;
     int64_t items = J->atoms_family ? J->d.natoms_per_mol_u : Mu;
-    if ((r = ensure(J->stream, J->geo, sizeof(Geo) * F))) return r;
-    if ((r = ensure(J->stream, J->cells_cnt, sizeof(int) * F * (cap + 1)))) return r;
-    if ((r = ensure(J->stream, J->cells_start, sizeof(int) * F * (cap + 1)))) return r;
-    if ((r = ensure(J->stream, J->tmp4, 16 * F * items))) return r;
-    if ((r = ensure(J->stream, J->tmpcell, 4 * F * items))) return r;
-    if ((r = ensure(J->stream, J->sorted, 16 * F * items))) return r;
-    if ((r = ensure(J->stream, J->vw, 16 * F * J->Nv))) return r;
+    if ((r = ensure(J, J->geo, sizeof(Geo) * F))) return r;
+    if ((r = ensure(J, J->cells_cnt, sizeof(int) * F * (cap + 1)))) return r;
+    if ((r = ensure(J, J->cells_start, sizeof(int) * F * (cap + 1)))) return r;
+    if ((r = ensure(J, J->tmp4, 16 * F * items))) return r;
+    if ((r = ensure(J, J->tmpcell, 4 * F * items))) return r;
+    if ((r = ensure(J, J->sorted, 16 * F * items))) return r;
+    if ((r = ensure(J, J->vw, 16 * F * J->Nv))) return r;
     if (!J->atoms_family) {
-        if ((r = ensure(J->stream, J->solw, 16 * F * J->Nu))) return r;
-        if ((r = ensure(J->stream, J->ucen, 16 * F * Mu))) return r;
-        if ((r = ensure(J->stream, J->drawn_ptr, 4 * F * (Mu + 1)))) return r;
-        if ((r = ensure(J->stream, J->drawn_idx, 4 * F * std::max(J->K, 1)))) return r;
-        if ((r = ensure(J->stream, J->vcell, 4 * F * Mv))) return r;
-        if ((r = ensure(J->stream, J->vcen, 16 * F * Mv))) return r;
+        if ((r = ensure(J, J->solw, 16 * F * J->Nu))) return r;
+        if ((r = ensure(J, J->ucen, 16 * F * Mu))) return r;
+        if ((r = ensure(J, J->drawn_ptr, 4 * F * (Mu + 1)))) return r;
+        if ((r = ensure(J, J->drawn_idx, 4 * F * std::max(J->K, 1)))) return r;
+        if ((r = ensure(J, J->vcell, 4 * F * Mv))) return r;
+        if ((r = ensure(J, J->vcen, 16 * F * Mv))) return r;
     } else {
-        if ((r = ensure(J->stream, J->near, (size_t)F * cap))) return r;
-        if ((r = ensure(J->stream, J->mkey, (size_t)4 * F * NP * Mv))) return r;
-        if ((r = ensure(J->stream, J->mcnt, (size_t)4 * F * NP * (kcap + 1)))) return r;
-        if ((r = ensure(J->stream, J->mstart, (size_t)4 * F * NP * (kcap + 1)))) return r;
-        if ((r = ensure(J->stream, J->msorted, (size_t)4 * F * NP * Mv))) return r;
-        if ((r = ensure(J->stream, J->nnear, (size_t)4 * F * NP))) return r;
-        if ((r = ensure(J->stream, J->chunk_base, (size_t)4 * (F * std::max(J->K, 1) + 1)))) return r;
+        if ((r = ensure(J, J->near, (size_t)F * cap))) return r;
+        if ((r = ensure(J, J->mkey, (size_t)4 * F * NP * Mv))) return r;
+        if ((r = ensure(J, J->mcnt, (size_t)4 * F * NP * (kcap + 1)))) return r;
+        if ((r = ensure(J, J->mstart, (size_t)4 * F * NP * (kcap + 1)))) return r;
+        if ((r = ensure(J, J->msorted, (size_t)4 * F * NP * Mv))) return r;
+        if ((r = ensure(J, J->nnear, (size_t)4 * F * NP))) return r;
+        if ((r = ensure(J, J->chunk_base, (size_t)4 * (F * std::max(J->K, 1) + 1)))) return r;
     }
     if (J->K > 0) {
-        if ((r = ensure(J->stream, J->P, (size_t)24 * F * J->K * J->Nv))) return r;
-        if ((r = ensure(J->stream, J->vw_r, (size_t)16 * F * J->K * J->Nv))) return r;
+        if ((r = ensure(J, J->P, (size_t)24 * F * J->K * J->Nv))) return r;
+        if ((r = ensure(J, J->vw_r, (size_t)16 * F * J->K * J->Nv))) return r;
         if (!J->atoms_family) {
-            if ((r = ensure(J->stream, J->vcell_r, (size_t)4 * F * J->K * Mv))) return r;
-            if ((r = ensure(J->stream, J->vcen_r, (size_t)16 * F * J->K * Mv))) return r;
+            if ((r = ensure(J, J->vcell_r, (size_t)4 * F * J->K * Mv))) return r;
+            if ((r = ensure(J, J->vcen_r, (size_t)16 * F * J->K * Mv))) return r;
         }
-        if ((r = ensure(J->stream, J->bflag, (size_t)F * J->kb * Mv))) return r;
-        if ((r = ensure(J->stream, J->blist, (size_t)4 * F * J->kb * Mv))) return r;
-        if ((r = ensure(J->stream, J->nbulk, (size_t)4 * F * J->kb))) return r;
-        if ((r = ensure(J->stream, J->uk, (size_t)4 * F * J->K))) return r;
+        if ((r = ensure(J, J->bflag, (size_t)F * J->kb * Mv))) return r;
+        if ((r = ensure(J, J->blist, (size_t)4 * F * J->kb * Mv))) return r;
+        if ((r = ensure(J, J->nbulk, (size_t)4 * F * J->kb))) return r;
+        if ((r = ensure(J, J->uk, (size_t)4 * F * J->K))) return r;
     }
     if (host_input) {
-        if ((r = ensure(J->stream, J->xin_u, (size_t)dtype_size * F * J->Nu * 3))) return r;
-        if ((r = ensure(J->stream, J->xin_v, (size_t)dtype_size * F * J->Nv * 3))) return r;
+        if ((r = ensure(J, J->xin_u, (size_t)dtype_size * F * J->Nu * 3))) return r;
+        if ((r = ensure(J, J->xin_v, (size_t)dtype_size * F * J->Nv * 3))) return r;
     }
     return CM_OK;
 }
@@ -3204,10 +3219,10 @@ int job_setup(cm_job* J, const cm_job_desc& d)
     const int64_t Nv = J->Nv;
     const int64_t nb = d.nbins;
     int r;
-    if ((r = ensure(J->stream, J->cnt, 8 * J->acc_len))) return r;
-    if ((r = ensure(J->stream, J->acc, 8 * J->acc_len))) return r;
-    if ((r = ensure(J->stream, J->hits, 8 * 4 * nb))) return r;
-    if ((r = ensure(J->stream, J->stats, 64))) return r;
+    if ((r = ensure(J, J->cnt, 8 * J->acc_len))) return r;
+    if ((r = ensure(J, J->acc, 8 * J->acc_len))) return r;
+    if ((r = ensure(J, J->hits, 8 * 4 * nb))) return r;
+    if ((r = ensure(J, J->stats, 64))) return r;
     if ((r = zero_async(J->cnt.p, 8 * J->acc_len, J->stream))) return r;
     if ((r = zero_async(J->acc.p, 8 * J->acc_len, J->stream))) return r;
     if ((r = zero_async(J->hits.p, 8 * 4 * nb, J->stream))) return r;
@@ -3227,7 +3242,7 @@ int job_setup(cm_job* J, const cm_job_desc& d)
         ov_i = gm.size(); gm.insert(gm.end(), d.groups_v_idx, d.groups_v_idx + m); gm.push_back(0);
     }
     if (!gm.empty()) {
-        if ((r = ensure(J->stream, J->gmap, 4 * gm.size()))) return r;
+        if ((r = ensure(J, J->gmap, 4 * gm.size()))) return r;
         HIPCHK(hipMemcpyAsync(J->gmap.p, gm.data(), 4 * gm.size(), hipMemcpyHostToDevice, J->stream));
         HIPCHK(hipStreamSynchronize(J->stream));
         int32_t* base = (int32_t*)J->gmap.p;
@@ -3846,6 +3861,137 @@ void cm_server_destroy(cm_server* S)
     for (cm_job* J : S->shells) cm_job_destroy(J);
     for (auto& kv : S->reqs) delete kv.second;
     delete S;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// entry point that allocates nothing (cm_mddf_noalloc)
+// ---------------------------------------------------------------------------
+// Fixed device scratch: every device buffer of such a call (integer hit
+// counters, group maps, the batch buffers of the cell lists, placements and
+// sorted molecules) is carved from it; the caller brings the coordinates and
+// the FP64 output.  One copy per device, held until the process ends.
+constexpr size_t SCRATCH_BYTES = (size_t)512 << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+namespace {
+
+constexpr int NA_MAX_DEVICES = 64;
+// room for the 256-byte alignment of the carved buffers and the few bytes
+// per frame (chunk offsets, drawn molecules) the per-frame estimate leaves out
+constexpr int64_t NA_SLACK = 65536;
+
+struct NoAllocDevice {
+    std::mutex m;          // serialises the calls on this device
+    bool ready = false;
+    void* ptr = nullptr;   // device address of `scratch` on this device
+    cm_job* job = nullptr; // host-side state reused by every call
+};
+
+NoAllocDevice* na_devices()
+{
+    static NoAllocDevice* d = new NoAllocDevice[NA_MAX_DEVICES];
+    return d;
+}
+
+// device bytes a call needs: the integer counters and fixed buffers, plus
+// the batch buffers of one frame
+int64_t noalloc_fixed_bytes(const cm_job* J)
+{
+    int64_t b = 8 * J->acc_len + 8 * 4 * (int64_t)J->d.nbins + 64 + 4 * 256;
+    if (J->d.groups_u_ptr) b += 4 * ((int64_t)J->d.natoms_per_mol_u + 2 + J->d.groups_u_ptr[J->d.natoms_per_mol_u]);
+    if (J->d.groups_v_ptr) b += 4 * (J->Nv + 2 + J->d.groups_v_ptr[J->Nv]);
+    return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cm_mddf_noalloc(const cm_job_desc* desc, int32_t nframes, const int64_t* frame_index, const void* d_x_solute,
+                    const void* d_x_solvent, int32_t dtype, const double* cells, const double* weights, double* d_acc,
+                    double* volume_total, void* stream)
+{
+    if (!desc || !d_acc || !volume_total) return fail(CM_EINVAL, "null argument");
+    if (nframes < 1 || !frame_index || !d_x_solute || !d_x_solvent || !cells || !weights)
+        return fail(CM_EINVAL, "invalid frame arrays");
+    if (dtype != CM_F64 && dtype != CM_F32) return fail(CM_EINVAL, "dtype must be CM_F64 or CM_F32");
+    if (int r = validate_desc(*desc)) return r;
+    for (int f = 0; f < nframes; f++) {
+        if (!(weights[f] > 0.0) || !std::isfinite(weights[f])) return fail(CM_EINVAL, "frame weights must be positive and finite");
+        if (frame_index[f] < 1 || frame_index[f] > 0xFFFFFFFFll) return fail(CM_EINVAL, "frame_index must be in [1, 2^32)");
+    }
+    // sizes first, on the host: a job that does not fit is refused before
+    // any device work
+    const int ds = dtype == CM_F32 ? 4 : 8;
+    int64_t need_fixed, capmax, kcapmax, Fmax, pfb;
+    {
+        cm_job plan;
+        job_params(&plan, *desc);
+        need_fixed = noalloc_fixed_bytes(&plan);
+        plan.d.memory_budget = (int64_t)SCRATCH_BYTES;
+        if (int r = batch_plan(&plan, nframes, cells, ds, false, &capmax, &kcapmax, &Fmax, &pfb)) return r;
+        if (need_fixed + pfb + NA_SLACK > (int64_t)SCRATCH_BYTES)
+            return fail(CM_ENOMEM, "job too large for the no-allocation entry point: it needs " +
+                                       std::to_string(need_fixed + pfb) + " bytes of device scratch for one frame, the "
+                                       "fixed scratch holds " + std::to_string(SCRATCH_BYTES));
+        Fmax = std::min<int64_t>(Fmax, std::max<int64_t>(1, ((int64_t)SCRATCH_BYTES - need_fixed - NA_SLACK) / pfb));
+    }
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return fail(CM_EUNSUPPORTED, "device index beyond the supported range");
+    NoAllocDevice& D = na_devices()[dev];
+    std::lock_guard<std::mutex> lk(D.m);
+    if (!D.ready) {
+        // once per device, with that device current: the device address of
+        // this device's copy of the scratch array
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(CM_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        D.ptr = p;
+        D.job = new cm_job();
+        D.ready = true;
+    }
+    cm_job* J = D.job;
+    // every buffer of the previous call is given up: the arena starts over
+    DevBuf* bufs[] = {&J->cnt, &J->acc, &J->hits, &J->stats, &J->gmap, &J->geo, &J->cells_cnt, &J->cells_start,
+                      &J->tmp4, &J->tmpcell, &J->sorted, &J->solw, &J->ucen, &J->vw, &J->vcell, &J->vcen, &J->P,
+                      &J->vw_r, &J->vcell_r, &J->vcen_r, &J->bflag, &J->blist, &J->nbulk, &J->uk, &J->drawn_ptr,
+                      &J->drawn_idx, &J->xin_u, &J->xin_v, &J->near, &J->mkey, &J->mcnt, &J->mstart,
+                      &J->msorted, &J->nnear, &J->chunk_base};
+    for (DevBuf* b : bufs) *b = DevBuf{};
+    J->arena = (char*)D.ptr;
+    J->arena_cap = SCRATCH_BYTES;
+    J->arena_used = 0;
+    J->stream = (hipStream_t)stream;
+    // the caller's output is the FP64 accumulator block (zeroed by the setup)
+    const int64_t acc_len = 4 * (int64_t)desc->nbins + 2 * (int64_t)desc->n_groups_u * desc->nbins +
+                            2 * (int64_t)desc->n_groups_v * desc->nbins;
+    J->acc = DevBuf{d_acc, 8 * (size_t)acc_len};
+    if (int r = job_setup(J, *desc)) return r;
+    J->d.max_batch_frames = Fmax;
+    J->d.memory_budget = (int64_t)SCRATCH_BYTES;
+    // the batch buffers for the largest batch, carved once
+    if (int r = alloc_batch(J, (int)Fmax, capmax, kcapmax, ds, false)) return r;
+    if (int r = submit_common(J, nframes, frame_index, d_x_solute, d_x_solvent, dtype, cells, weights, false)) return r;
+    if (J->pending) {
+        if (int r = flush(J, J->pending_w)) return r;
+        J->pending = false;
+    }
+    uint64_t gerr = 0;
+    HIPCHK(hipMemcpyAsync(&gerr, (char*)J->stats.p + 16, 8, hipMemcpyDeviceToHost, J->stream));
+    HIPCHK(hipStreamSynchronize(J->stream));
+    J->finished = true;
+    if (int r = geometry_error(gerr)) return r;
+    *volume_total = J->volume_total;
+    return CM_OK;
+}
+
+int cm_device_download(void* host, const void* dptr, size_t bytes)
+{
+    HIPCHK(hipMemcpy(host, dptr, bytes, hipMemcpyDeviceToHost));
+    return CM_OK;
 }
 
 }  // extern "C"

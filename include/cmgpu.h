@@ -170,6 +170,33 @@ int cm_device_upload(void *dptr, const void *host, size_t bytes);
 int cm_device_free(void *dptr);
 int cm_device_synchronize(void);
 
+/* Copies device memory back to the host (synchronous). */
+int cm_device_download(void *host, const void *dptr, size_t bytes);
+
+/* One mddf accumulation that allocates nothing: cm_job_create +
+ * cm_job_submit_device + cm_job_finish in one synchronous call, for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.
+ *   - Device buffers come from a fixed 512 MiB scratch array in the library's
+ *     device code (one per GPU, held until the process ends); a job whose
+ *     counters plus one frame of batch buffers do not fit is refused with
+ *     CM_ENOMEM before any device work.  Frames run in batches that fit.
+ *   - Inputs as cm_job_submit_device (device coordinates; host cells,
+ *     weights, frame numbers).  Output: d_acc, caller-allocated device memory
+ *     of 4*nbins + 2*(n_groups_u + n_groups_v)*nbins doubles, written as
+ *     [md_count | md_count_random | rdf_count | rdf_count_random |
+ *      solute_group_count | solvent_group_count | solute_group_count_random |
+ *      solvent_group_count_random] (group arrays [group][bin]); the values
+ *     equal those of cm_job_finish.  *volume_total as cm_raw.volume_total.
+ *   - stream: a hipStream_t, or NULL for the default stream.  The call
+ *     returns when the work is done.  Calls on one GPU share its scratch and
+ *     are serialised by the library; the scratch keeps the last call's
+ *     contents (everything a call reads is cleared or written by it first). */
+int cm_mddf_noalloc(const cm_job_desc *desc, int32_t nframes, const int64_t *frame_index, const void *d_x_solute,
+                    const void *d_x_solvent, int32_t dtype, const double *cells, const double *weights, double *d_acc,
+                    double *volume_total, void *stream);
+
 /* Device bytes a job would hold while it runs these frames (the admission
  * estimate of the request server).  No counterpart in the reference. */
 int cm_job_footprint(const cm_job_desc *desc, int32_t nframes, int32_t dtype, const double *cells, int64_t *bytes);
