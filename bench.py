@@ -214,6 +214,73 @@ def serve_capacity_once(args):
             "makespan_s": round(r["makespan_s"], 4), "p99_s": round(r["p99_s"], 4)}
 
 
+def batched_systems(args):
+    """Many uneven systems in one launch per stage: independent cubic systems
+    (solute 200-20,000 atoms log-uniform, water at 0.0334 molecules/A^3 in
+    boxes of 30-150 A), 50 jittered frames each, FP32, cutoff 10 A, binstep
+    0.02 A, n_random_samples 10.  Plain: every system as its own job, one after
+    the other (create, submit, finish).  Batched: all jobs in one
+    cm_jobs_submit_device (rounds of one batch of every job, each stage one
+    launch over all of them, replayed from a HIP graph), then every job
+    finished.  Both paths' counts must be identical."""
+    import complexmixtures_amd.serve as SV
+    from complexmixtures_amd import _lib
+    t0 = time.perf_counter()
+    pool = SV.make_pool(args.batch_systems, 20261208, args.batch_frames)
+    gen_s = time.perf_counter() - t0
+    nf = args.batch_frames
+    reqs = [SV.Request(0.0, k, nf, True, False, 40000 + k) for k in range(len(pool))]
+    specs = [SV.job_kwargs(pool[r.system], r) for r in reqs]
+
+    def plain():
+        out = []
+        for (kw, du, dv, _), r in zip(specs, reqs):
+            job = _lib.Job(**kw)
+            job.submit_device(np.arange(1, nf + 1), du, dv, 0, pool[r.system].cells[:nf], np.ones(nf))
+            out.append(job.finish())
+            job.close()
+        return out
+
+    def batched():
+        jobs = [_lib.Job(**kw) for kw, _, _, _ in specs]
+        _lib.submit_many([(j, np.arange(1, nf + 1), du, dv, 0, pool[r.system].cells[:nf], np.ones(nf))
+                          for j, (kw, du, dv, _), r in zip(jobs, specs, reqs)])
+        out = [j.finish() for j in jobs]
+        for j in jobs:
+            j.close()
+        return out
+
+    plain()
+    batched()
+    _lib_sync()
+    tp, tb, same = [], [], True
+    for _ in range(3):
+        t = time.perf_counter()
+        a = plain()
+        tp.append(time.perf_counter() - t)
+        s0 = _lib.multi_stats()
+        t = time.perf_counter()
+        b = batched()
+        tb.append(time.perf_counter() - t)
+        s1 = _lib.multi_stats()
+        same = same and all(SV.digest(x) == SV.digest(y) for x, y in zip(a, b))
+    mp, mb = sorted(tp)[1], sorted(tb)[1]
+    frames = nf * len(pool)
+    for s_ in pool:
+        s_.d_solute.free()
+        s_.d_water.free()
+    return {"workload": f"{len(pool)} cubic systems (solute 200-20,000 atoms log-uniform, water boxes 30-150 A), "
+                        f"{nf} frames each, FP32, n_random_samples 10",
+            "solute_atoms": [p.n_solute for p in pool], "waters": [p.n_water for p in pool],
+            "plain": {"jobs_per_s": round(len(pool) / mp, 3), "frames_per_s": round(frames / mp, 1),
+                      "seconds": round(mp, 4)},
+            "batched": {"jobs_per_s": round(len(pool) / mb, 3), "frames_per_s": round(frames / mb, 1),
+                        "seconds": round(mb, 4), "rounds": s1["rounds"] - s0["rounds"],
+                        "stage_launches": s1["launches"] - s0["launches"]},
+            "ratio": round(mp / mb, 3), "counts_identical": bool(same), "runs": "median of 3 per path",
+            "gen_s": round(gen_s, 2)}
+
+
 def _lib_sync():
     from complexmixtures_amd import _lib
     _lib.check(_lib.lib().cm_device_synchronize())
@@ -290,6 +357,10 @@ def main():
     ap.add_argument("--serve-capacity-path", choices=("served", "plain"), default=None,
                     help="only one capacity run of that serving path (for kernel traces)")
     ap.add_argument("--no-config3", action="store_true", help="skip the triclinic mixed-case rates")
+    ap.add_argument("--no-batched", action="store_true", help="skip the many-systems measurement")
+    ap.add_argument("--batched-only", action="store_true", help="only the many-systems measurement")
+    ap.add_argument("--batch-systems", type=int, default=64)
+    ap.add_argument("--batch-frames", type=int, default=50)
     ap.add_argument("--self-water-only", action="store_true", help="only the self-MDDF water rate")
     ap.add_argument("--config3-frames", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -320,6 +391,9 @@ def main():
         return
     if args.self_water_only:
         print(json.dumps({"self_water": self_water_rate(args)}), flush=True)
+        return
+    if args.batched_only:
+        print(json.dumps({"batched_systems": batched_systems(args)}), flush=True)
         return
     import complexmixtures_amd.synthetic as S
     from complexmixtures_amd import _lib
@@ -475,6 +549,8 @@ def main():
                                     "FP64 oracle's (the CPU baseline's own run)"}
     du.free()
     dv.free()
+    if not args.no_batched:
+        out["batched_systems"] = batched_systems(args)
     if not args.no_serve:
         out["served_stream"] = served_stream(args)
     if not args.no_config3:

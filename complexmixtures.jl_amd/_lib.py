@@ -45,6 +45,18 @@ class KernelStats(ctypes.Structure):
                 ("frames", _i64), ("candidate_pairs", _i64), ("fp64_rechecks", _i64)]
 
 
+class MDRecord(ctypes.Structure):
+    """cm_md_record: MinimumDistance, src/minimum_distances.jl:12-19 (0-based i, j)."""
+    _fields_ = [("within_cutoff", _i32), ("i", _i32), ("j", _i64), ("d", _d),
+                ("ref_atom_within_cutoff", _i32), ("pad", _i32), ("d_ref_atom", _d)]
+
+
+class Frames(ctypes.Structure):
+    """cm_frames: one job's frames for cm_jobs_submit_device."""
+    _fields_ = [("nframes", _i32), ("dtype", _i32), ("frame_index", _P(_i64)), ("d_x_solute", ctypes.c_void_p),
+                ("d_x_solvent", ctypes.c_void_p), ("cells", _P(_d)), ("weights", _P(_d))]
+
+
 class ServerDesc(ctypes.Structure):
     _fields_ = [("slots", _i32), ("pad", _i32), ("memory_budget", _i64)]
 
@@ -60,10 +72,11 @@ class RequestTimes(ctypes.Structure):
 
 
 EXPORTS = ["cm_job_create", "cm_job_submit", "cm_job_submit_device", "cm_job_finish", "cm_job_set_profiling",
-           "cm_job_set_tuning", "cm_job_kernel_stats", "cm_job_launch_times", "cm_job_destroy", "cm_last_error", "cm_device_alloc", "cm_device_upload",
+           "cm_job_set_tuning", "cm_job_kernel_stats", "cm_job_launch_times", "cm_job_set_verify", "cm_job_records",
+           "cm_job_placement", "cm_job_reference_draws", "cm_job_destroy", "cm_last_error", "cm_device_alloc", "cm_device_upload",
            "cm_device_free", "cm_device_synchronize", "cm_job_footprint", "cm_server_create", "cm_server_enqueue",
            "cm_server_wait", "cm_server_destroy", "cm_release_cached_memory", "cm_device_download",
-           "cm_mddf_noalloc"]
+           "cm_mddf_noalloc", "cm_jobs_submit_device", "cm_multi_set_graphs", "cm_multi_stats"]
 
 _lib = None
 
@@ -94,6 +107,10 @@ def lib():
         L.cm_job_set_tuning.argtypes = [ctypes.c_void_p, ctypes.c_double, _i32, _i32]
         L.cm_job_kernel_stats.argtypes = [ctypes.c_void_p, _P(KernelStats)]
         L.cm_job_launch_times.argtypes = [ctypes.c_void_p, _i32, _P(_d), _P(_d), _P(_i32)]
+        L.cm_job_set_verify.argtypes = [ctypes.c_void_p, _i32]
+        L.cm_job_records.argtypes = [ctypes.c_void_p, _i64, _i32, _P(MDRecord)]
+        L.cm_job_placement.argtypes = [ctypes.c_void_p, _i64, _i32, _P(_d), _P(ctypes.c_uint8)]
+        L.cm_job_reference_draws.argtypes = [ctypes.c_void_p, _i64, _P(_i32)]
         L.cm_job_destroy.argtypes = [ctypes.c_void_p]
         L.cm_job_destroy.restype = None
         L.cm_last_error.restype = ctypes.c_char_p
@@ -108,12 +125,17 @@ def lib():
         L.cm_server_destroy.restype = None
         L.cm_release_cached_memory.argtypes = []
         L.cm_device_download.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.cm_jobs_submit_device.argtypes = [_i32, _P(ctypes.c_void_p), _P(Frames)]
+        L.cm_multi_set_graphs.argtypes = [_i32]
+        L.cm_multi_stats.argtypes = [_P(_i64), _P(_i64)]
         L.cm_mddf_noalloc.argtypes = [_P(JobDesc), _i32, _P(_i64), ctypes.c_void_p, ctypes.c_void_p, _i32, _P(_d),
                                       _P(_d), ctypes.c_void_p, _P(_d), ctypes.c_void_p]
         for f in ("cm_job_create", "cm_job_submit", "cm_job_submit_device", "cm_job_finish", "cm_job_set_profiling",
-                  "cm_job_set_tuning", "cm_job_kernel_stats", "cm_job_launch_times", "cm_device_alloc", "cm_device_upload", "cm_device_free",
+                  "cm_job_set_tuning", "cm_job_kernel_stats", "cm_job_launch_times", "cm_job_set_verify",
+                  "cm_job_records", "cm_job_placement", "cm_job_reference_draws", "cm_device_alloc", "cm_device_upload", "cm_device_free",
                   "cm_device_synchronize", "cm_job_footprint", "cm_server_create", "cm_server_enqueue",
-                  "cm_server_wait", "cm_device_download", "cm_mddf_noalloc"):
+                  "cm_server_wait", "cm_device_download", "cm_mddf_noalloc", "cm_jobs_submit_device",
+                  "cm_multi_set_graphs", "cm_multi_stats"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -272,6 +294,30 @@ class Job:
         check(lib().cm_job_kernel_stats(self.h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in KernelStats._fields_}
 
+    def set_verify(self, on=True):
+        """Keep records, placements and reference draws of every frame (cm_job_set_verify)."""
+        check(lib().cm_job_set_verify(self.h, int(on)))
+        self.nmols_v = None
+
+    def records(self, frame, pass_, nmols_v):
+        """MinimumDistance records of one pass (0 real, 1..K random) of a frame:
+        list of (within, i, j, d, ref, d_ref)."""
+        out = (MDRecord * nmols_v)()
+        check(lib().cm_job_records(self.h, int(frame), int(pass_), out))
+        return [(m.within_cutoff, m.i, m.j, m.d, m.ref_atom_within_cutoff, m.d_ref_atom) for m in out]
+
+    def placement(self, frame, pass_, nmols_v, natoms_per_mol_v):
+        """(coordinates [nmols_v * natoms][3], placed flags [nmols_v]) of random pass pass_ of a frame."""
+        out = np.zeros((nmols_v * natoms_per_mol_v, 3))
+        fl = np.zeros(nmols_v, dtype=np.uint8)
+        check(lib().cm_job_placement(self.h, int(frame), int(pass_), _dp(out), fl.ctypes.data_as(_P(ctypes.c_uint8))))
+        return out, fl.astype(bool)
+
+    def reference_draws(self, frame, n_random_samples):
+        out = np.zeros(max(n_random_samples, 1), dtype=np.int32)
+        check(lib().cm_job_reference_draws(self.h, int(frame), out.ctypes.data_as(_P(_i32))))
+        return out[:n_random_samples]
+
     def launch_times(self):
         """Per profiled batch: (real-pass, random-pass) evaluation launch
         durations in ms (-1 without random passes)."""
@@ -385,3 +431,37 @@ def mddf_noalloc(frame_index, d_solute, d_solvent, frame0, cells, weights, d_acc
         o += n
     out["volume_total"] = vol.value
     return out
+
+
+def submit_many(items):
+    """cm_jobs_submit_device: items are (job, frame_index, d_solute, d_solvent,
+    frame0, cells, weights) as for Job.submit_device; the frames of all jobs
+    run together (one launch per stage per round of batches).  Each job keeps
+    references to its device arrays until it is closed."""
+    n = len(items)
+    handles = (ctypes.c_void_p * max(n, 1))()
+    fr = (Frames * max(n, 1))()
+    keep = []
+    for k, (job, frame_index, d_solute, d_solvent, frame0, cells, weights) in enumerate(items):
+        job._inputs = getattr(job, "_inputs", []) + [d_solute, d_solvent]
+        fi = np.ascontiguousarray(frame_index, dtype=np.int64)
+        c9 = _cells9(cells)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        keep += [fi, c9, w]
+        fu = d_solute.nbytes // d_solute.shape[0]
+        fv = d_solvent.nbytes // d_solvent.shape[0]
+        handles[k] = job.h
+        fr[k] = Frames(len(fi), CM_F32 if d_solute.dtype == np.float32 else CM_F64, fi.ctypes.data_as(_P(_i64)),
+                       d_solute.offset(frame0 * fu), d_solvent.offset(frame0 * fv), _dp(c9), _dp(w))
+    check(lib().cm_jobs_submit_device(n, handles, fr))
+
+
+def set_multi_graphs(on):
+    """Rounds of submit_many replay a captured HIP graph (on) or launch their stages directly."""
+    check(lib().cm_multi_set_graphs(int(bool(on))))
+
+
+def multi_stats():
+    r, l = _i64(), _i64()
+    check(lib().cm_multi_stats(ctypes.byref(r), ctypes.byref(l)))
+    return {"rounds": r.value, "launches": l.value}

@@ -117,6 +117,18 @@ struct KP {                  // job parameters, by value
     int kcap;                // molecule sort keys per (frame, pass): Morton codes of the fine cells
 };
 
+// MinimumDistance record as the verification getters return it
+// (src/minimum_distances.jl:12-19; 0-based i molecule-local, j global)
+struct VRec {
+    int within;
+    int i;
+    int64_t j;
+    double d;
+    int ref;
+    int pad;
+    double dref;
+};
+
 struct Bufs {
     Geo* geo;
     // cell list
@@ -156,6 +168,7 @@ struct Bufs {
     ull* cnt_md; ull* cnt_rdf; ull* cnt_md_r; ull* cnt_rdf_r;
     ull* cnt_gu; ull* cnt_gv; ull* cnt_gu_r; ull* cnt_gv_r;
     ull* stats;              // [0] exhaustive resolves, [2] geometry error bits of the job's frames
+    VRec* vrec;              // verification: [F][NP][Mv] records of every pass (null when off)
 };
 
 struct Rec {
@@ -268,13 +281,13 @@ __device__ __forceinline__ float block_max(float x)
 // molecule radii (bound on the distance of an atom from the centre of its
 // molecule, with a margin over FP32 rounding); blockIdx.y = frame
 template <class T>
-__global__ void k_radii(KP P, Bufs B, const T* xu, const T* xv, int do_solute)
+__device__ __forceinline__ void b_radii(KP P, Bufs B, const T* xu, const T* xv, int do_solute, int bx_, int nbx_, int by_, int nby_)
 {
-    const int f = blockIdx.y;
+    const int f = by_;
     Geo& g = B.geo[f];
     const int64_t nmol = (int64_t)P.Mv + (do_solute ? P.Mu : 0);
     float rv = 0.f, ru = 0.f;
-    for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < nmol; m += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t m = bx_ * (int64_t)blockDim.x + threadIdx.x; m < nmol; m += (int64_t)nbx_ * blockDim.x) {
         const bool solv = m < P.Mv;
         const T* x = solv ? xv + (int64_t)f * P.Nv * 3 : xu + (int64_t)f * P.Nu * 3;
         const int n = solv ? P.nv : P.nu;
@@ -310,6 +323,12 @@ __global__ void k_radii(KP P, Bufs B, const T* xu, const T* xv, int do_solute)
     }
 }
 
+template <class T>
+__global__ void k_radii(KP P, Bufs B, const T* xu, const T* xv, int do_solute)
+{
+    b_radii<T>(P, B, xu, xv, do_solute, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+}
+
 __device__ __forceinline__ double halfulp32(double x)
 {
     if (!(x > 1e-30)) return 0x1p-150;
@@ -319,7 +338,7 @@ __device__ __forceinline__ double halfulp32(double x)
 // geometry errors of a batch's frames into the job's error bits (bit 0:
 // molecules too large for the cell, bit 1: grid larger than its allocation),
 // read with the accumulators at the end of the job
-__global__ void k_geo_flags(KP P, Bufs B)
+__device__ __forceinline__ void b_geo_flags(KP P, Bufs B, int bx_, int nbx_, int by_, int nby_)
 {
     for (int f = threadIdx.x; f < P.F; f += blockDim.x) {
         const int e = B.geo[f].err;
@@ -327,9 +346,14 @@ __global__ void k_geo_flags(KP P, Bufs B)
     }
 }
 
-__global__ void k_grid(KP P, Bufs B, int mols_family)
+__global__ void k_geo_flags(KP P, Bufs B)
 {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    b_geo_flags(P, B, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+}
+
+__device__ __forceinline__ void b_grid(KP P, Bufs B, int mols_family, int bx_, int nbx_, int by_, int nby_)
+{
+    int f = bx_ * blockDim.x + threadIdx.x;
     if (f >= P.F) return;
     Geo& g = B.geo[f];
     double ru = mols_family ? (double)__uint_as_float(g.ru_bits) : 0.0;
@@ -365,14 +389,19 @@ __global__ void k_grid(KP P, Bufs B, int mols_family)
     g.rnear = (float)(P.cutsys + rv + 20.0 * E + 1e-2);
 }
 
+__global__ void k_grid(KP P, Bufs B, int mols_family)
+{
+    b_grid(P, B, mols_family, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+}
+
 // ---------------------------------------------------------------------------
 // cell list: counting sort by cell
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ void k_wrap_atoms(KP P, Bufs B, const T* xu)
+__device__ __forceinline__ void b_wrap_atoms(KP P, Bufs B, const T* xu, int bx_, int nbx_, int by_, int nby_)
 {
     int64_t total = (int64_t)P.F * P.nu;
-    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t it = bx_ * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)nbx_ * blockDim.x) {
         int f = (int)(it / P.nu);
         int i = (int)(it - (int64_t)f * P.nu);
         const Geo& g = B.geo[f];
@@ -391,6 +420,12 @@ __global__ void k_wrap_atoms(KP P, Bufs B, const T* xu)
         B.tmpcell[o] = cell;
         atomicAdd(&B.cnt[(int64_t)f * (P.cap + 1) + cell], 1);
     }
+}
+
+template <class T>
+__global__ void k_wrap_atoms(KP P, Bufs B, const T* xu)
+{
+    b_wrap_atoms<T>(P, B, xu, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
 }
 
 // Solute molecules (mols family): made whole, centre wrapped into the cell,
@@ -433,11 +468,11 @@ __global__ void k_prep_solute_mols(KP P, Bufs B, const T* xu)
 }
 
 // exclusive scan of the cell counts, one block per frame; leaves cnt zeroed
-__global__ void __launch_bounds__(1024) k_scan_cells(KP P, Bufs B)
+__device__ __forceinline__ void b_scan_cells(KP P, Bufs B, int bx_, int nbx_, int by_, int nby_)
 {
     __shared__ int wsum[16];
     __shared__ int carry;
-    int f = blockIdx.x;
+    int f = bx_;
     int ncell = B.geo[f].ncell;
     int* cnt = B.cnt + (int64_t)f * (P.cap + 1);
     int* st = B.start + (int64_t)f * (P.cap + 1);
@@ -466,10 +501,15 @@ __global__ void __launch_bounds__(1024) k_scan_cells(KP P, Bufs B)
     if (threadIdx.x == 0) st[ncell] = carry;
 }
 
-__global__ void k_scatter(KP P, Bufs B, int nper)
+__global__ void __launch_bounds__(1024) k_scan_cells(KP P, Bufs B)
+{
+    b_scan_cells(P, B, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+}
+
+__device__ __forceinline__ void b_scatter(KP P, Bufs B, int nper, int bx_, int nbx_, int by_, int nby_)
 {
     int64_t total = (int64_t)P.F * nper;
-    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t it = bx_ * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)nbx_ * blockDim.x) {
         int f = (int)(it / nper);
         int e = (int)(it - (int64_t)f * nper);
         int64_t o = (int64_t)f * B.nitems + e;
@@ -478,6 +518,11 @@ __global__ void k_scatter(KP P, Bufs B, int nper)
         int pos = B.start[cb + cell] + atomicAdd(&B.cnt[cb + cell], 1);
         B.sorted[(int64_t)f * B.nitems + pos] = B.tmp4[o];
     }
+}
+
+__global__ void k_scatter(KP P, Bufs B, int nper)
+{
+    b_scatter(P, B, nper, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
 }
 
 // solvent molecules of the real pass, in the frame of their cell
@@ -1438,10 +1483,10 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_rand(KP P, Bufs B, const TU
 // ---------------------------------------------------------------------------
 // bulk lists and random placement
 // ---------------------------------------------------------------------------
-__global__ void k_bulk_init(KP P, Bufs B)
+__device__ __forceinline__ void b_bulk_init(KP P, Bufs B, int bx_, int nbx_, int by_, int nby_)
 {
     int64_t total = (int64_t)P.F * P.kb * P.Mv;
-    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t it = bx_ * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)nbx_ * blockDim.x) {
         int64_t fk = it / P.Mv;
         int v = (int)(it - fk * P.Mv);
         unsigned char fl = P.usecutoff ? 0 : 1;
@@ -1450,12 +1495,17 @@ __global__ void k_bulk_init(KP P, Bufs B)
     }
 }
 
+__global__ void k_bulk_init(KP P, Bufs B)
+{
+    b_bulk_init(P, B, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+}
+
 // ordered compaction of the bulk flags, one block per (frame, list)
-__global__ void __launch_bounds__(1024) k_bulk_compact(KP P, Bufs B)
+__device__ __forceinline__ void b_bulk_compact(KP P, Bufs B, int bx_, int nbx_, int by_, int nby_)
 {
     __shared__ int wsum[16];
     __shared__ int carry;
-    int64_t fk = blockIdx.x;
+    int64_t fk = bx_;
     const unsigned char* fl = B.bflag + fk * P.Mv;
     int* out = B.blist + fk * P.Mv;
     if (threadIdx.x == 0) carry = 0;
@@ -1476,6 +1526,11 @@ __global__ void __launch_bounds__(1024) k_bulk_compact(KP P, Bufs B)
         __syncthreads();
     }
     if (threadIdx.x == 0) B.nbulk[fk] = carry;
+}
+
+__global__ void __launch_bounds__(1024) k_bulk_compact(KP P, Bufs B)
+{
+    b_bulk_compact(P, B, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
 }
 
 template <class TV>
@@ -1580,10 +1635,10 @@ __device__ __forceinline__ void cell_range(const Geo& g, float lx, float ly, flo
 
 
 // cells near the solute: every cell within rnear of a solute atom (index box)
-__global__ void k_near_mark(KP P, Bufs B)
+__device__ __forceinline__ void b_near_mark(KP P, Bufs B, int bx_, int nbx_, int by_, int nby_)
 {
     int64_t total = (int64_t)P.F * P.nu;
-    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t it = bx_ * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)nbx_ * blockDim.x) {
         int f = (int)(it / P.nu);
         const Geo& g = B.geo[f];
         float4 s = B.tmp4[it];
@@ -1613,13 +1668,18 @@ __global__ void k_near_mark(KP P, Bufs B)
     }
 }
 
+__global__ void k_near_mark(KP P, Bufs B)
+{
+    b_near_mark(P, B, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+}
+
 // real pass: centre of every solvent molecule (made whole around its first
 // atom) wrapped into the cell; near molecules get the Morton key of the cell
 template <class T>
-__global__ void k_prep_solvent_tiles(KP P, Bufs B, const T* xv)
+__device__ __forceinline__ void b_prep_solvent_tiles(KP P, Bufs B, const T* xv, int bx_, int nbx_, int by_, int nby_)
 {
     int64_t total = (int64_t)P.F * P.Mv;
-    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t it = bx_ * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)nbx_ * blockDim.x) {
         int f = (int)(it / P.Mv);
         int v = (int)(it - (int64_t)f * P.Mv);
         const Geo& g = B.geo[f];
@@ -1637,13 +1697,19 @@ __global__ void k_prep_solvent_tiles(KP P, Bufs B, const T* xv)
     }
 }
 
+template <class T>
+__global__ void k_prep_solvent_tiles(KP P, Bufs B, const T* xv)
+{
+    b_prep_solvent_tiles<T>(P, B, xv, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+}
+
 // real pass: atoms of the near molecules in sorted order, FP32, in the frame of
 // the centre's cell
 template <class T>
-__global__ void k_fill_real(KP P, Bufs B, const T* xv)
+__device__ __forceinline__ void b_fill_real(KP P, Bufs B, const T* xv, int bx_, int nbx_, int by_, int nby_)
 {
     int64_t total = (int64_t)P.F * P.Mv;
-    for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t it = bx_ * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)nbx_ * blockDim.x) {
         int f = (int)(it / P.Mv);
         int pos = (int)(it - (int64_t)f * P.Mv);
         const int64_t fp = (int64_t)f * P.NP;
@@ -1672,14 +1738,20 @@ __global__ void k_fill_real(KP P, Bufs B, const T* xv)
     }
 }
 
+template <class T>
+__global__ void k_fill_real(KP P, Bufs B, const T* xv)
+{
+    b_fill_real<T>(P, B, xv, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+}
+
 // exclusive scan of the cell counts of passes [p0, p0+np) of every frame (one
 // block per (frame, pass)); zeroes the counts and writes the number of near
 // molecules of the pass
-__global__ void __launch_bounds__(1024) k_scan_rows(KP P, Bufs B, int p0, int np)
+__device__ __forceinline__ void b_scan_rows(KP P, Bufs B, int p0, int np, int bx_, int nbx_, int by_, int nby_)
 {
     __shared__ int wsum[16];
     __shared__ int carry;
-    const int f = blockIdx.x / np, pp = p0 + (int)(blockIdx.x - (blockIdx.x / np) * np);
+    const int f = bx_ / np, pp = p0 + (int)(bx_ - (bx_ / np) * np);
     const int64_t row = (int64_t)f * P.NP + pp;
     int* c = B.mcnt + row * (P.kcap + 1);
     int* st = B.mstart + row * (P.kcap + 1);
@@ -1716,13 +1788,18 @@ __global__ void __launch_bounds__(1024) k_scan_rows(KP P, Bufs B, int p0, int np
     }
 }
 
+__global__ void __launch_bounds__(1024) k_scan_rows(KP P, Bufs B, int p0, int np)
+{
+    b_scan_rows(P, B, p0, np, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+}
+
 // molecules of passes [p0, p0+np) sorted by cell
-__global__ void k_mol_scatter(KP P, Bufs B, int p0, int np)
+__device__ __forceinline__ void b_mol_scatter(KP P, Bufs B, int p0, int np, int bx_, int nbx_, int by_, int nby_)
 {
     // grid: x over the molecules, y over the (frame, pass) pairs
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = bx_ * blockDim.x + threadIdx.x;
     if (v >= P.Mv) return;
-    for (int fq = blockIdx.y; fq < P.F * np; fq += gridDim.y) {
+    for (int fq = by_; fq < P.F * np; fq += nby_) {
         const int f = fq / np;
         const int p = p0 + fq - f * np;
         int64_t fp = (int64_t)f * P.NP + p;
@@ -1734,8 +1811,13 @@ __global__ void k_mol_scatter(KP P, Bufs B, int p0, int np)
     }
 }
 
+__global__ void k_mol_scatter(KP P, Bufs B, int p0, int np)
+{
+    b_mol_scatter(P, B, p0, np, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+}
+
 // chunk offsets of the passes [p0, p0+np) of every frame (one block)
-__global__ void __launch_bounds__(1024) k_chunk_table(KP P, Bufs B, int p0, int np)
+__device__ __forceinline__ void b_chunk_table(KP P, Bufs B, int p0, int np, int bx_, int nbx_, int by_, int nby_)
 {
     __shared__ int wsum[16];
     __shared__ int carry;
@@ -1768,15 +1850,20 @@ __global__ void __launch_bounds__(1024) k_chunk_table(KP P, Bufs B, int p0, int 
     if (threadIdx.x == 0) B.chunk_base[n] = carry;
 }
 
+__global__ void __launch_bounds__(1024) k_chunk_table(KP P, Bufs B, int p0, int np)
+{
+    b_chunk_table(P, B, p0, np, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+}
+
 // randomize_solvent! (src/mddf.jl:65-88), first half: the new centre of every
 // slot; slots whose centre falls in a near cell get the Morton key of the cell
 // (the others cannot have an atom within the cutoff of the solute)
-__global__ void k_place_keys(KP P, Bufs B)
+__device__ __forceinline__ void b_place_keys(KP P, Bufs B, int bx_, int nbx_, int by_, int nby_)
 {
     // grid: x over the slots, y over the (frame, pass) pairs
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = bx_ * blockDim.x + threadIdx.x;
     if (v >= P.Mv) return;
-    for (int fk = blockIdx.y; fk < P.F * P.K; fk += gridDim.y) {
+    for (int fk = by_; fk < P.F * P.K; fk += nby_) {
         const int f = fk / P.K;
         const int k = fk - f * P.K;
         const Geo& g = B.geo[f];
@@ -1791,15 +1878,20 @@ __global__ void k_place_keys(KP P, Bufs B)
     }
 }
 
+__global__ void k_place_keys(KP P, Bufs B)
+{
+    b_place_keys(P, B, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+}
+
 // second half: the placement of the near slots, written in sorted order (FP64
 // for the exact resolve, FP32 for the evaluation)
 template <class TV>
-__global__ void k_place_near(KP P, Bufs B, const TV* xv)
+__device__ __forceinline__ void b_place_near(KP P, Bufs B, const TV* xv, int bx_, int nbx_, int by_, int nby_)
 {
     // grid: x over the sorted positions, y over the (frame, pass) pairs
-    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    const int pos = bx_ * blockDim.x + threadIdx.x;
     if (pos >= P.Mv) return;
-    for (int fk = blockIdx.y; fk < P.F * P.K; fk += gridDim.y) {
+    for (int fk = by_; fk < P.F * P.K; fk += nby_) {
         const int f = fk / P.K;
         const int k = fk - f * P.K;
         int64_t fp = (int64_t)f * P.NP + 1 + k;
@@ -1817,6 +1909,12 @@ __global__ void k_place_near(KP P, Bufs B, const TV* xv)
         double ua[3] = {cm::u01(r2.x), cm::u01(r2.y), cm::u01(r2.z)};
         cm::random_move(src, dst, P.nv, P.iref, g.cell, uc, ua);
     }
+}
+
+template <class TV>
+__global__ void k_place_near(KP P, Bufs B, const TV* xv)
+{
+    b_place_near<TV>(P, B, xv, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
 }
 
 // lexicographic minimum of two exact records, OR of the reference flags
@@ -1878,17 +1976,14 @@ __device__ __forceinline__ void col_reduce(Best& b)
 // NVM atoms, e.g. water): a row is one molecule, each lane evaluates every
 // staged atom against all NVM atoms it holds in registers (one LDS read per
 // NVM pairs) and combines them in the lane.
-template <class TU, class TV, bool RANDOM, int C, int NVM>
-__global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu, const TV* xv)
+template <class TU, class TV, bool RANDOM, int C, int NVM, bool VERIFY>
+__device__ __forceinline__ void eval_tiles_chunks(const KP& P, const Bufs& B, const TU* xu, const TV* xv, int ch0, int ch1,
+                                                  int chs, float4* stg, float* stw, const int* cbs, bool cb_lds)
 {
-    // dynamic LDS: the staging buffer (P.tcap float4: atom, index; relative
-    // to the chunk centre when NVM > 0), with NVM > 0 the squared norms of the
-    // staged atoms (P.tcap float), then the chunk table of this launch when it
-    // has at most CBCAP entries
-    extern __shared__ float4 dyn[];
-    float4* stg = dyn;
-    float* stw = (float*)(dyn + P.tcap);
-    int* cbs = (int*)(stw + (NVM > 0 ? P.tcap : 0));
+    // chunks ch0, ch0 + chs, ... below ch1 of one job; LDS: the staging
+    // buffer stg (P.tcap float4: atom, index; relative to the chunk centre
+    // when NVM > 0), with NVM > 0 the squared norms stw of the staged atoms,
+    // and (cb_lds) the job's chunk table cbs
     const int tcap = P.tcap;
     // staging ranks per lane per step (fewer with a molecule per lane, whose
     // evaluation holds more registers)
@@ -1898,16 +1993,11 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
     const int p0 = RANDOM ? 1 : 0;
     const int npass = RANDOM ? P.K : 1;
     const int nfp = P.F * npass;
-    const bool cb_lds = nfp + 1 <= CBCAP;
-    if (cb_lds)
-        for (int k = lane; k <= nfp; k += 64) cbs[k] = B.chunk_base[k];
-    __syncthreads();
-    const int total = B.chunk_base[nfp];
     ull* gu = RANDOM ? B.cnt_gu_r : B.cnt_gu;
     ull* gv = RANDOM ? B.cnt_gv_r : B.cnt_gv;
     ull* gmd = RANDOM ? B.cnt_md_r : B.cnt_md;
     ull* grdf = RANDOM ? B.cnt_rdf_r : B.cnt_rdf;
-    for (int ch = blockIdx.x; ch < total; ch += gridDim.x) {
+    for (int ch = ch0; ch < ch1; ch += chs) {
         int lo_ = 0, hi_ = nfp;
         while (hi_ - lo_ > 1) {
             int mid = (lo_ + hi_) >> 1;
@@ -2200,13 +2290,14 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
         };
         Rec r = rec_zero();
         bool exh = false;
+        bool d32 = false, dref32 = false;   // r.d / r.dref hold FP32 squared distances (bins decided without them)
         if (leader) {
             if (mb <= cut2_hi_c) {
                 if (ms - mb > amb_c) {
                     int bn, bk;
                     const int dd = decide32(P, G32c, mb, bn, bk);
                     if (dd != 0) {
-                        if (dd == 2) { r.within = 1; r.i = mi; r.j = jbase + ma; r.d = mb; r.bin = bn; r.bulk = bk; }
+                        if (dd == 2) { r.within = 1; r.i = mi; r.j = jbase + ma; r.d = mb; r.bin = bn; r.bulk = bk; d32 = true; }
                     } else {
                         double x0, x1, x2, y0, y1, y2;
                         load3(xuf, mi, x0, x1, x2);
@@ -2221,7 +2312,7 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
                     int bn, bk;
                     const int dd = decide32(P, G32c, rb, bn, bk);
                     if (dd != 0) {
-                        if (dd == 2) { r.ref = 1; r.dref = rb; r.rbin = bn; }
+                        if (dd == 2) { r.ref = 1; r.dref = rb; r.rbin = bn; dref32 = true; }
                     } else {
                         double x0, x1, x2, y0, y1, y2;
                         load3(xuf, ri, x0, x1, x2);
@@ -2303,15 +2394,56 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
                 }
             }
             for (int o = 32; o >= 1; o >>= 1) rec_merge(loc, rec_shfl_xor(loc, o));
-            if (lane == L) r = loc;
+            if (lane == L) { r = loc; d32 = dref32 = false; }
         }
         if (leader) {
             rec_complete(P, r);
             hist_record_global(P, gmd, grdf, gu, gv, r);
             if (!RANDOM && P.K > 0) B.bflag[(int64_t)f * P.kb * P.Mv + v] = inbulk(P, r) ? 1 : 0;
+            if constexpr (VERIFY) {
+                // verification: the record with its FP64 distances (the
+                // argmin pairs are exact; the distances of records decided
+                // from FP32 bounds are formed here)
+                VRec o;
+                o.within = r.within; o.i = r.within ? r.i : -1; o.j = r.within ? r.j : -1;
+                o.d = r.within ? r.d : INFINITY;
+                o.ref = r.ref; o.pad = 0; o.dref = r.ref ? r.dref : INFINITY;
+                double x0, x1, x2, y0, y1, y2;
+                if (r.within && d32) {
+                    load3(xuf, r.i, x0, x1, x2);
+                    ysrc((int)(r.j - jbase), y0, y1, y2);
+                    o.d = sqrt(cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2));
+                }
+                if (r.ref && dref32) {
+                    load3(xuf, ri, x0, x1, x2);
+                    ysrc(P.iref, y0, y1, y2);
+                    o.dref = sqrt(cm::canon_d2(g.cell, x0, x1, x2, y0, y1, y2));
+                }
+                B.vrec[fp * P.Mv + v] = o;
+            }
         }
         __syncthreads();
     }
+}
+
+// Minimum distances of the solvent molecules of one chunk per wavefront
+// (64 threads), over every chunk of one job's launch (persistent grid).
+template <class TU, class TV, bool RANDOM, int C, int NVM, bool VERIFY>
+__global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu, const TV* xv)
+{
+    // dynamic LDS: staging buffer, squared norms (NVM > 0), then the chunk
+    // table of this launch when it has at most CBCAP entries
+    extern __shared__ float4 dyn[];
+    float4* stg = dyn;
+    float* stw = (float*)(dyn + P.tcap);
+    int* cbs = (int*)(stw + (NVM > 0 ? P.tcap : 0));
+    const int nfp = P.F * (RANDOM ? P.K : 1);
+    const bool cb_lds = nfp + 1 <= CBCAP;
+    if (cb_lds)
+        for (int k = threadIdx.x; k <= nfp; k += 64) cbs[k] = B.chunk_base[k];
+    __syncthreads();
+    eval_tiles_chunks<TU, TV, RANDOM, C, NVM, VERIFY>(P, B, xu, xv, blockIdx.x, B.chunk_base[nfp], gridDim.x, stg, stw,
+                                                     cbs, cb_lds);
 }
 
 // Fills n16 16-byte words and then ntail single bytes with zeros (a kernel,
@@ -2322,6 +2454,17 @@ __global__ void k_zero(uint4* p, int64_t n16, unsigned char* tail, int ntail)
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n16; k += stride) p[k] = make_uint4(0u, 0u, 0u, 0u);
     if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+// verification records of molecules no chunk evaluates: not within the
+// cutoff (zero(MinimumDistance), src/minimum_distances.jl:21-28)
+__global__ void k_vrec_init(VRec* p, int64_t n)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        VRec o;
+        o.within = 0; o.i = -1; o.j = -1; o.d = INFINITY; o.ref = 0; o.pad = 0; o.dref = INFINITY;
+        p[k] = o;
+    }
 }
 
 // integer hits -> FP64 accumulators
@@ -2335,6 +2478,235 @@ __global__ void k_flush(ull* cnt, double* acc, ull* hits, int64_t n, int64_t nhi
         acc[k] += (double)c * mult;
         if (k < nhist) hits[k] += c;
         cnt[k] = 0ull;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// many jobs per launch (cm_jobs_submit_device): one round runs one batch of
+// frames of every job, and each stage of the single-solute path is one launch
+// over all of them.  A device table holds each job's parameters and buffers
+// and, per stage, how many blocks of that stage's single-job grid the job has;
+// every block of a stage launch walks the concatenated block lists (a block
+// index maps to a job by a search over the prefix of the counts), so one
+// launch does exactly the work of the jobs' own launches.  The launches read
+// nothing but the table's address: a round is a table upload plus the same
+// launch sequence, which is captured once in a HIP graph and replayed.
+// ---------------------------------------------------------------------------
+constexpr int MS_MAX = 128;   // jobs per round
+enum MStage {
+    MS_ZERO, MS_RADII, MS_GRID, MS_WRAP, MS_SCAN_CELLS, MS_SCATTER, MS_NEAR, MS_PREP, MS_SCAN_ROWS0,
+    MS_MOL_SCATTER0, MS_FILL, MS_CHUNK0, MS_BULK_INIT, MS_BULK_COMPACT, MS_PLACE_KEYS, MS_SCAN_ROWSK,
+    MS_MOL_SCATTERK, MS_PLACE_NEAR, MS_CHUNKK, MS_GEO_FLAGS, MS_N
+};
+
+struct SysDev {
+    KP P;
+    Bufs B;
+    const void* xu;          // device coordinates of the round's frames
+    const void* xv;
+    uint4* zp[3];            // buffers cleared at the start of the round (cell counts, near flags, key counts)
+    int64_t z16[3];          // their sizes in 16-byte words
+    int nbx[MS_N];           // x extent (blocks) of each stage's grid for this job; y = blocks / nbx
+};
+
+struct MultiTab {
+    int nsys;
+    int pad;
+    int base[MS_N][MS_MAX + 1];   // per stage: prefix over the jobs of their block counts
+    SysDev sys[MS_MAX];
+};
+
+// Runs fn(job, bx, nbx, by, nby) for every (job, block) of stage ST, the
+// block indices of the launch striding over the concatenated lists.  SYNC:
+// the stage's body uses shared memory, so the block synchronises between two
+// of its virtual blocks.
+template <int ST, bool SYNC, class Fn>
+__device__ __forceinline__ void multi_blocks(const MultiTab* __restrict__ MT, Fn fn)
+{
+    const int n = MT->nsys;
+    const int* base = MT->base[ST];
+    const int total = base[n];
+    for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        int lo = 0, hi = n;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (base[mid] <= vb) lo = mid; else hi = mid;
+        }
+        const SysDev& S = MT->sys[lo];
+        const int local = vb - base[lo];
+        const int nbx = S.nbx[ST];
+        const int nby = (base[lo + 1] - base[lo]) / nbx;
+        fn(S, local - (local / nbx) * nbx, nbx, local / nbx, nby);
+        if (SYNC) __syncthreads();
+    }
+}
+
+__global__ void km_zero(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<MS_ZERO, false>(MT, [&](const SysDev& S, int bx, int nbx, int, int) {
+        const int64_t n0 = S.z16[0], n1 = S.z16[1], tot = n0 + n1 + S.z16[2];
+        for (int64_t k = bx * (int64_t)blockDim.x + threadIdx.x; k < tot; k += (int64_t)nbx * blockDim.x) {
+            uint4* q = k < n0 ? S.zp[0] + k : (k < n0 + n1 ? S.zp[1] + (k - n0) : S.zp[2] + (k - n0 - n1));
+            *q = make_uint4(0u, 0u, 0u, 0u);
+        }
+    });
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLOCK) km_radii(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<MS_RADII, true>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_radii<T>(S.P, S.B, (const T*)S.xu, (const T*)S.xv, 0, bx, nbx, by, nby);
+    });
+}
+
+__global__ void km_grid(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<MS_GRID, false>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_grid(S.P, S.B, 0, bx, nbx, by, nby);
+    });
+}
+
+template <class T>
+__global__ void km_wrap_atoms(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<MS_WRAP, false>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_wrap_atoms<T>(S.P, S.B, (const T*)S.xu, bx, nbx, by, nby);
+    });
+}
+
+__global__ void __launch_bounds__(1024) km_scan_cells(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<MS_SCAN_CELLS, true>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_scan_cells(S.P, S.B, bx, nbx, by, nby);
+    });
+}
+
+__global__ void km_scatter(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<MS_SCATTER, false>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_scatter(S.P, S.B, S.B.nitems, bx, nbx, by, nby);
+    });
+}
+
+__global__ void km_near_mark(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<MS_NEAR, false>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_near_mark(S.P, S.B, bx, nbx, by, nby);
+    });
+}
+
+template <class T>
+__global__ void km_prep_solvent_tiles(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<MS_PREP, false>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_prep_solvent_tiles<T>(S.P, S.B, (const T*)S.xv, bx, nbx, by, nby);
+    });
+}
+
+template <bool RANDOM>
+__global__ void __launch_bounds__(1024) km_scan_rows(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<RANDOM ? MS_SCAN_ROWSK : MS_SCAN_ROWS0, true>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_scan_rows(S.P, S.B, RANDOM ? 1 : 0, RANDOM ? S.P.K : 1, bx, nbx, by, nby);
+    });
+}
+
+template <bool RANDOM>
+__global__ void km_mol_scatter(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<RANDOM ? MS_MOL_SCATTERK : MS_MOL_SCATTER0, false>(MT, [&](const SysDev& S, int bx, int nbx, int by,
+                                                                            int nby) {
+        b_mol_scatter(S.P, S.B, RANDOM ? 1 : 0, RANDOM ? S.P.K : 1, bx, nbx, by, nby);
+    });
+}
+
+template <class T>
+__global__ void km_fill_real(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<MS_FILL, false>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_fill_real<T>(S.P, S.B, (const T*)S.xv, bx, nbx, by, nby);
+    });
+}
+
+template <bool RANDOM>
+__global__ void __launch_bounds__(1024) km_chunk_table(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<RANDOM ? MS_CHUNKK : MS_CHUNK0, true>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_chunk_table(S.P, S.B, RANDOM ? 1 : 0, RANDOM ? S.P.K : 1, bx, nbx, by, nby);
+    });
+}
+
+__global__ void km_bulk_init(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<MS_BULK_INIT, false>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_bulk_init(S.P, S.B, bx, nbx, by, nby);
+    });
+}
+
+__global__ void __launch_bounds__(1024) km_bulk_compact(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<MS_BULK_COMPACT, true>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_bulk_compact(S.P, S.B, bx, nbx, by, nby);
+    });
+}
+
+__global__ void km_place_keys(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<MS_PLACE_KEYS, false>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_place_keys(S.P, S.B, bx, nbx, by, nby);
+    });
+}
+
+template <class T>
+__global__ void km_place_near(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<MS_PLACE_NEAR, false>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_place_near<T>(S.P, S.B, (const T*)S.xv, bx, nbx, by, nby);
+    });
+}
+
+__global__ void km_geo_flags(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<MS_GEO_FLAGS, false>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_geo_flags(S.P, S.B, bx, nbx, by, nby);
+    });
+}
+
+// The evaluation over the chunks of every job of the round: chunk g of the
+// launch is chunk g - cb[j] of job j, cb the prefix of the jobs' chunk counts
+// (read from their chunk tables at the start of the launch; jobs without
+// random passes have none in the random launch).
+template <class TU, class TV, bool RANDOM, int C, int NVM>
+__global__ void __launch_bounds__(64, 3) km_eval_tiles(const MultiTab* __restrict__ MT)
+{
+    extern __shared__ float4 dyn[];
+    __shared__ int cb[MS_MAX + 1];
+    const int n = MT->nsys;
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int j = 0; j < n; j++) {
+            cb[j] = acc;
+            const KP& Pj = MT->sys[j].P;
+            const int nfp = Pj.F * (RANDOM ? Pj.K : 1);
+            acc += (RANDOM && Pj.K == 0) ? 0 : MT->sys[j].B.chunk_base[nfp];
+        }
+        cb[n] = acc;
+    }
+    __syncthreads();
+    const int total = cb[n];
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        int lo = 0, hi = n;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (cb[mid] <= g) lo = mid; else hi = mid;
+        }
+        const SysDev& S = MT->sys[lo];
+        const int ch = g - cb[lo];
+        float4* stg = dyn;
+        float* stw = (float*)(dyn + S.P.tcap);
+        eval_tiles_chunks<TU, TV, RANDOM, C, NVM, false>(S.P, S.B, (const TU*)S.xu, (const TV*)S.xv, ch, ch + 1, 1, stg,
+                                                         stw, nullptr, false);
     }
 }
 
@@ -2589,6 +2961,17 @@ struct cm_job {
     std::vector<int> huk, hdp, hdi;
     std::vector<int32_t> hgm;        // group maps (host copy of the upload)
     PinBuf hres;                     // accumulators and hit counts read back at finish (page-locked)
+    // verification mode: every batch's records, placements and reference
+    // draws are copied to the host, keyed by trajectory frame number
+    bool verify = false;
+    DevBuf vrec;
+    struct VerifyFrame {
+        std::vector<VRec> rec;            // [NP][Mv] (tiles family)
+        std::vector<double> placed;       // [K][Nv][3], NaN where a slot was not placed
+        std::vector<uint8_t> placed_flag; // [K][Mv]
+        std::vector<int32_t> draws;       // [K]
+    };
+    std::map<int64_t, VerifyFrame> vstore;
     // buffers carved from a fixed device region instead of allocated (the
     // entry point that allocates nothing); null for ordinary jobs
     char* arena = nullptr;
@@ -2674,7 +3057,10 @@ void launch_tiles_one(hipStream_t s, int grid, int cus, const KP& P, const Bufs&
     const size_t shm = (sizeof(float4) + (P.mol_lane ? sizeof(float) : 0)) * (size_t)P.tcap +
                        sizeof(int) * (size_t)std::min(nfp + 1, CBCAP);
     // 3-atom solvent molecules: one molecule per lane
-    auto kern = P.mol_lane ? k_eval_tiles<TU, TV, RANDOM, C, 3> : k_eval_tiles<TU, TV, RANDOM, C, 0>;
+    // verification (records kept) in its own instantiations, so that the
+    // ordinary kernels carry none of its registers
+    auto kern = B.vrec ? (P.mol_lane ? k_eval_tiles<TU, TV, RANDOM, C, 3, true> : k_eval_tiles<TU, TV, RANDOM, C, 0, true>)
+                       : (P.mol_lane ? k_eval_tiles<TU, TV, RANDOM, C, 3, false> : k_eval_tiles<TU, TV, RANDOM, C, 0, false>);
     if (grid <= 0) {
         int nb = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 64, shm) != hipSuccess || nb < 1) nb = 1;
@@ -2705,19 +3091,64 @@ void launch_tiles_c(int c, bool f32, bool random, hipStream_t s, int grid, int c
     else launch_tiles<1>(f32, random, s, grid, cus, P, B, xu, xv);
 }
 
-int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, const void* xv_dev, int dtype,
-              const double* cells, int64_t cap, int64_t kcap)
+// Verification mode: waits for the batch and keeps, per frame, the records
+// of every pass (tiles family), the placed solvent of every random pass (by
+// slot) and the drawn reference solute molecules.
+int verify_collect(cm_job* J, int F, const int64_t* frame_index)
 {
-    KP P = J->kp;
-    P.F = F;
-    P.cap = (int)cap;
-    P.kcap = (int)kcap;
-    const bool tiles = J->atoms_family;
-    // geometry on the host: cells, inverse, spacings (and the fine grid of the tiles family)
-    J->hgeo.assign(F, Geo());
-    const double cutsys = P.cutsys;
+    HIPCHK(hipStreamSynchronize(J->stream));
+    const int64_t Mv = J->d.nmols_v, Nv = J->Nv, K = J->K, NP = 1 + K, nv = J->d.natoms_per_mol_v;
+    std::vector<VRec> rec;
+    if (J->atoms_family) {
+        rec.resize((size_t)F * NP * Mv);
+        HIPCHK(hipMemcpy(rec.data(), J->vrec.p, sizeof(VRec) * rec.size(), hipMemcpyDeviceToHost));
+    }
+    std::vector<double> P;
+    std::vector<int> msorted, nnear;
+    if (K > 0) {
+        P.resize((size_t)F * K * Nv * 3);
+        HIPCHK(hipMemcpy(P.data(), J->P.p, sizeof(double) * P.size(), hipMemcpyDeviceToHost));
+        if (J->atoms_family) {
+            msorted.resize((size_t)F * NP * Mv);
+            nnear.resize((size_t)F * NP);
+            HIPCHK(hipMemcpy(msorted.data(), J->msorted.p, sizeof(int) * msorted.size(), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(nnear.data(), J->nnear.p, sizeof(int) * nnear.size(), hipMemcpyDeviceToHost));
+        }
+    }
     for (int f = 0; f < F; f++) {
-        Geo& g = J->hgeo[f];
+        cm_job::VerifyFrame& vf = J->vstore[frame_index[f]];
+        if (J->atoms_family) vf.rec.assign(rec.begin() + (size_t)f * NP * Mv, rec.begin() + (size_t)(f + 1) * NP * Mv);
+        vf.placed.assign((size_t)K * Nv * 3, NAN);
+        vf.placed_flag.assign((size_t)K * Mv, 0);
+        vf.draws.assign((size_t)K, 0);
+        for (int64_t k = 0; k < K; k++) {
+            const double* src = P.data() + ((size_t)f * K + k) * Nv * 3;
+            double* dst = vf.placed.data() + (size_t)k * Nv * 3;
+            if (J->atoms_family) {
+                // near slots only, stored at their sorted positions
+                const int64_t fp = (int64_t)f * NP + 1 + k;
+                for (int64_t pos = 0; pos < nnear[fp]; pos++) {
+                    const int64_t v = msorted[fp * Mv + pos];
+                    std::memcpy(dst + v * nv * 3, src + pos * nv * 3, sizeof(double) * 3 * nv);
+                    vf.placed_flag[k * Mv + v] = 1;
+                }
+            } else {
+                std::memcpy(dst, src, sizeof(double) * 3 * Nv);
+                std::fill(vf.placed_flag.begin() + k * Mv, vf.placed_flag.begin() + (k + 1) * Mv, (uint8_t)1);
+                vf.draws[k] = J->huk[(size_t)f * K + k];
+            }
+        }
+    }
+    return CM_OK;
+}
+
+// Host geometry of a batch's frames: cells, inverses, plane spacings and, for
+// the single-solute kernels, the fine cell grid.
+int batch_geo(const cm_job* J, int F, const int64_t* frame_index, const double* cells, int64_t cap, int64_t kcap, Geo* out)
+{
+    const double cutsys = J->kp.cutsys;
+    for (int f = 0; f < F; f++) {
+        Geo& g = out[f];
         memset(&g, 0, sizeof(Geo));
         cm::cell_setup(cells + 9 * f, J->d.triclinic, &g.cell);
         for (int k = 0; k < 9; k++) {
@@ -2731,7 +3162,7 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
         if (!(cutsys < 0.5 * spmin * (1.0 - 1e-9)))
             return fail(CM_EUNSUPPORTED, "cutoff must be smaller than half of the smallest plane spacing of the unit cell");
         g.frame = (uint32_t)frame_index[f];
-        if (tiles) {
+        if (J->atoms_family) {
             double side = cutsys * J->cell_frac;
             g.ncell = 1;
             for (int k = 0; k < 3; k++) {
@@ -2745,17 +3176,21 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
             if ((int64_t)1 << (3 * g.kbits) > kcap) return fail(CM_EHIP, "internal error: key space larger than its allocation");
         }
     }
-    hipStream_t s = J->stream;
-    HIPCHK(hipMemcpyAsync(J->geo.p, J->hgeo.data(), sizeof(Geo) * F, hipMemcpyHostToDevice, s));
+    return CM_OK;
+}
+
+// The kernels' view of a job's buffers (geometry array at geo)
+Bufs batch_bufs(const cm_job* J, Geo* geo)
+{
     Bufs B;
     memset(&B, 0, sizeof(B));
-    B.geo = (Geo*)J->geo.p;
+    B.geo = geo;
     B.cnt = (int*)J->cells_cnt.p;
     B.start = (int*)J->cells_start.p;
     B.tmp4 = (float4*)J->tmp4.p;
     B.tmpcell = (int*)J->tmpcell.p;
     B.sorted = (float4*)J->sorted.p;
-    B.nitems = tiles ? P.nu : P.Mu;
+    B.nitems = J->atoms_family ? J->kp.nu : J->kp.Mu;
     B.solw = (float4*)J->solw.p;
     B.ucen = (float4*)J->ucen.p;
     B.vw = (float4*)J->vw.p;
@@ -2779,10 +3214,32 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
     B.nnear = (int*)J->nnear.p;
     B.chunk_base = (int*)J->chunk_base.p;
     ull* c = (ull*)J->cnt.p;
-    int64_t nb = P.nbins, ngu = J->d.n_groups_u, ngv = J->d.n_groups_v;
+    const int64_t nb = J->d.nbins, ngu = J->d.n_groups_u, ngv = J->d.n_groups_v;
     B.cnt_md = c; B.cnt_md_r = c + nb; B.cnt_rdf = c + 2 * nb; B.cnt_rdf_r = c + 3 * nb;
     B.cnt_gu = c + 4 * nb; B.cnt_gv = B.cnt_gu + ngu * nb; B.cnt_gu_r = B.cnt_gv + ngv * nb; B.cnt_gv_r = B.cnt_gu_r + ngu * nb;
     B.stats = (ull*)J->stats.p;
+    return B;
+}
+
+int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, const void* xv_dev, int dtype,
+              const double* cells, int64_t cap, int64_t kcap)
+{
+    KP P = J->kp;
+    P.F = F;
+    P.cap = (int)cap;
+    P.kcap = (int)kcap;
+    const bool tiles = J->atoms_family;
+    // geometry on the host: cells, inverse, spacings (and the fine grid of the tiles family)
+    J->hgeo.assign(F, Geo());
+    if (int r = batch_geo(J, F, frame_index, cells, cap, kcap, J->hgeo.data())) return r;
+    hipStream_t s = J->stream;
+    HIPCHK(hipMemcpyAsync(J->geo.p, J->hgeo.data(), sizeof(Geo) * F, hipMemcpyHostToDevice, s));
+    Bufs B = batch_bufs(J, (Geo*)J->geo.p);
+    B.vrec = (J->verify && tiles) ? (VRec*)J->vrec.p : nullptr;
+    if (B.vrec) {
+        const int64_t nrec = (int64_t)F * P.NP * P.Mv;
+        hipLaunchKernelGGL(k_vrec_init, dim3(grid_for(nrec)), dim3(BLOCK), 0, s, B.vrec, nrec);
+    }
 
     // random draws of the reference solute molecules (mddf_frame!, src/mddf.jl:374-376);
     // with one solute molecule every draw is molecule 0
@@ -2937,8 +3394,13 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
     hipLaunchKernelGGL(k_geo_flags, dim3(1), dim3(64), 0, s, P, B);
     HIPCHK(hipGetLastError());
     J->frames += F;
+    if (J->verify) return verify_collect(J, F, frame_index);
     return CM_OK;
 }
+
+// sizes of the buffers cleared per batch, rounded up to whole 16-byte words
+// (the multi-job path clears them with 16-byte stores)
+size_t round16(size_t b) { return (b + 15) & ~(size_t)15; }
 
 int alloc_batch(cm_job* J, int F, int64_t cap, int64_t kcap, int dtype_size, bool host_input)
 {
@@ -2946,7 +3408,7 @@ int alloc_batch(cm_job* J, int F, int64_t cap, int64_t kcap, int dtype_size, boo
     int64_t Mu = J->d.nmols_u, Mv = J->d.nmols_v, NP = 1 + J->K;
     int64_t items = J->atoms_family ? J->d.natoms_per_mol_u : Mu;
     if ((r = ensure(J, J->geo, sizeof(Geo) * F))) return r;
-    if ((r = ensure(J, J->cells_cnt, sizeof(int) * F * (cap + 1)))) return r;
+    if ((r = ensure(J, J->cells_cnt, round16(sizeof(int) * F * (cap + 1))))) return r;
     if ((r = ensure(J, J->cells_start, sizeof(int) * F * (cap + 1)))) return r;
     if ((r = ensure(J, J->tmp4, 16 * F * items))) return r;
     if ((r = ensure(J, J->tmpcell, 4 * F * items))) return r;
@@ -2960,9 +3422,9 @@ int alloc_batch(cm_job* J, int F, int64_t cap, int64_t kcap, int dtype_size, boo
         if ((r = ensure(J, J->vcell, 4 * F * Mv))) return r;
         if ((r = ensure(J, J->vcen, 16 * F * Mv))) return r;
     } else {
-        if ((r = ensure(J, J->near, (size_t)F * cap))) return r;
+        if ((r = ensure(J, J->near, round16((size_t)F * cap)))) return r;
         if ((r = ensure(J, J->mkey, (size_t)4 * F * NP * Mv))) return r;
-        if ((r = ensure(J, J->mcnt, (size_t)4 * F * NP * (kcap + 1)))) return r;
+        if ((r = ensure(J, J->mcnt, round16((size_t)4 * F * NP * (kcap + 1))))) return r;
         if ((r = ensure(J, J->mstart, (size_t)4 * F * NP * (kcap + 1)))) return r;
         if ((r = ensure(J, J->msorted, (size_t)4 * F * NP * Mv))) return r;
         if ((r = ensure(J, J->nnear, (size_t)4 * F * NP))) return r;
@@ -2980,6 +3442,8 @@ int alloc_batch(cm_job* J, int F, int64_t cap, int64_t kcap, int dtype_size, boo
         if ((r = ensure(J, J->nbulk, (size_t)4 * F * J->kb))) return r;
         if ((r = ensure(J, J->uk, (size_t)4 * F * J->K))) return r;
     }
+    if (J->verify && J->atoms_family)
+        if ((r = ensure(J, J->vrec, sizeof(VRec) * (size_t)F * NP * Mv))) return r;
     if (host_input) {
         if ((r = ensure(J, J->xin_u, (size_t)dtype_size * F * J->Nu * 3))) return r;
         if ((r = ensure(J, J->xin_v, (size_t)dtype_size * F * J->Nv * 3))) return r;
@@ -2987,7 +3451,7 @@ int alloc_batch(cm_job* J, int F, int64_t cap, int64_t kcap, int dtype_size, boo
     return CM_OK;
 }
 
-int flush(cm_job* J, double w)
+int flush(cm_job* J, double w, hipStream_t stream = nullptr)
 {
     int64_t nb = J->d.nbins;
     int64_t half_lo = 0, half_hi = 0;
@@ -2995,7 +3459,8 @@ int flush(cm_job* J, double w)
         half_lo = 4 * nb;
         half_hi = J->acc_len;
     }
-    hipLaunchKernelGGL(k_flush, dim3(grid_for(J->acc_len)), dim3(BLOCK), 0, J->stream, (ull*)J->cnt.p, (double*)J->acc.p,
+    hipLaunchKernelGGL(k_flush, dim3(grid_for(J->acc_len)), dim3(BLOCK), 0, stream ? stream : J->stream, (ull*)J->cnt.p,
+                       (double*)J->acc.p,
                        (ull*)J->hits.p, J->acc_len, 4 * nb, half_lo, half_hi, w);
     HIPCHK(hipGetLastError());
     return CM_OK;
@@ -3151,6 +3616,8 @@ void job_params(cm_job* J, const cm_job_desc& d)
     J->eval_ms = J->total_ms = 0.0;
     J->eval_launches = J->total_launches = J->frames = 0;
     J->prof_used = J->prof_done = 0;
+    J->verify = false;
+    J->vstore.clear();
     J->prof_random.clear();
     J->launch_ms.clear();
     J->cell_frac = 0.5;
@@ -3550,6 +4017,52 @@ int cm_job_launch_times(cm_job* J, int32_t cap, double* real_ms, double* random_
     return CM_OK;
 }
 
+int cm_job_set_verify(cm_job* J, int32_t on)
+{
+    if (!J) return fail(CM_EINVAL, "null job");
+    if (J->frames > 0) return fail(CM_ESTATE, "verification must be set before the first submit");
+    if (J->arena) return fail(CM_EUNSUPPORTED, "no verification on the no-allocation path");
+    J->verify = on != 0;
+    return CM_OK;
+}
+
+int cm_job_records(cm_job* J, int64_t frame, int32_t pass, cm_md_record* out)
+{
+    if (!J || !out) return fail(CM_EINVAL, "null argument");
+    if (!J->verify) return fail(CM_ESTATE, "verification is off for this job");
+    if (!J->atoms_family)
+        return fail(CM_EUNSUPPORTED, "records are kept for jobs with one solute molecule (tile kernels) only");
+    if (pass < 0 || pass > J->K) return fail(CM_EINVAL, "pass must be 0 (real) or 1..n_random_samples");
+    auto it = J->vstore.find(frame);
+    if (it == J->vstore.end()) return fail(CM_EINVAL, "frame not processed by this job");
+    static_assert(sizeof(VRec) == sizeof(cm_md_record), "record layout");
+    std::memcpy(out, it->second.rec.data() + (size_t)pass * J->d.nmols_v, sizeof(VRec) * (size_t)J->d.nmols_v);
+    return CM_OK;
+}
+
+int cm_job_placement(cm_job* J, int64_t frame, int32_t pass, double* out, uint8_t* placed)
+{
+    if (!J || !out || !placed) return fail(CM_EINVAL, "null argument");
+    if (!J->verify) return fail(CM_ESTATE, "verification is off for this job");
+    if (pass < 1 || pass > J->K) return fail(CM_EINVAL, "pass must be 1..n_random_samples");
+    auto it = J->vstore.find(frame);
+    if (it == J->vstore.end()) return fail(CM_EINVAL, "frame not processed by this job");
+    const int64_t k = pass - 1;
+    std::memcpy(out, it->second.placed.data() + (size_t)k * J->Nv * 3, sizeof(double) * 3 * (size_t)J->Nv);
+    std::memcpy(placed, it->second.placed_flag.data() + (size_t)k * J->d.nmols_v, (size_t)J->d.nmols_v);
+    return CM_OK;
+}
+
+int cm_job_reference_draws(cm_job* J, int64_t frame, int32_t* out)
+{
+    if (!J || !out) return fail(CM_EINVAL, "null argument");
+    if (!J->verify) return fail(CM_ESTATE, "verification is off for this job");
+    auto it = J->vstore.find(frame);
+    if (it == J->vstore.end()) return fail(CM_EINVAL, "frame not processed by this job");
+    std::memcpy(out, it->second.draws.data(), sizeof(int32_t) * (size_t)J->K);
+    return CM_OK;
+}
+
 void cm_job_destroy(cm_job* J)
 {
     if (!J) return;
@@ -3558,7 +4071,7 @@ void cm_job_destroy(cm_job* J)
                       &J->tmp4, &J->tmpcell, &J->sorted, &J->solw, &J->ucen, &J->vw, &J->vcell, &J->vcen, &J->P,
                       &J->vw_r, &J->vcell_r, &J->vcen_r, &J->bflag, &J->blist, &J->nbulk, &J->uk, &J->drawn_ptr,
                       &J->drawn_idx, &J->xin_u, &J->xin_v, &J->near, &J->mkey, &J->mcnt, &J->mstart,
-                      &J->msorted, &J->nnear, &J->chunk_base};
+                      &J->msorted, &J->nnear, &J->chunk_base, &J->vrec};
     for (DevBuf* b : bufs) dev_put(*b);
     pin_put(J->hres);
     for (auto& a : J->prof_ev)
@@ -3861,6 +4374,379 @@ void cm_server_destroy(cm_server* S)
     for (cm_job* J : S->shells) cm_job_destroy(J);
     for (auto& kv : S->reqs) delete kv.second;
     delete S;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// many jobs per round (cm_jobs_submit_device)
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int MR_MAX_DEVICES = 64;
+constexpr int MS_FMAX = 48;        // frames of one job per round (the single-job batch cap)
+constexpr int MG_256 = 2048;       // blocks of the 256-thread stage launches
+constexpr int MG_1024 = 512;       // blocks of the 1024-thread stage launches
+constexpr int MG_64 = 256;         // blocks of the 64-thread stage launches
+constexpr int64_t MULTI_BUDGET = (int64_t)64 << 30;   // batch buffers of one submission's jobs together
+
+struct MultiRunner {
+    std::mutex m;
+    bool ready = false;
+    hipStream_t stream = nullptr;
+    void* tab = nullptr;                  // device table and geometry area (fixed address: graphs use it)
+    size_t tab_bytes = 0;
+    void* htab[2] = {nullptr, nullptr};   // page-locked staging of the table, used in turn
+    hipEvent_t up[2] = {nullptr, nullptr};
+    int cur = 0;
+    bool graphs = true;
+    std::map<int, hipGraphExec_t> exec;   // per kernel variant
+    hipEvent_t done = nullptr;
+    int cus = 256;
+    int64_t rounds = 0, launches = 0;
+};
+
+MultiRunner* multi_runners()
+{
+    static MultiRunner* r = new MultiRunner[MR_MAX_DEVICES];
+    return r;
+}
+
+constexpr size_t MT_GEO_OFF = (sizeof(MultiTab) + 255) & ~(size_t)255;
+
+int multi_init(MultiRunner& R)
+{
+    if (R.ready) return CM_OK;
+    HIPCHK(hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking));
+    R.tab_bytes = MT_GEO_OFF + sizeof(Geo) * (size_t)MS_MAX * MS_FMAX;
+    HIPCHK(hipMalloc(&R.tab, R.tab_bytes));
+    for (int k = 0; k < 2; k++) {
+        HIPCHK(hipHostMalloc(&R.htab[k], R.tab_bytes, hipHostMallocDefault));
+        HIPCHK(hipEventCreateWithFlags(&R.up[k], hipEventDisableTiming));
+    }
+    HIPCHK(hipEventCreateWithFlags(&R.done, hipEventDisableTiming));
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess)
+        if (hipDeviceGetAttribute(&R.cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || R.cus < 1) R.cus = 256;
+    R.ready = true;
+    return CM_OK;
+}
+
+template <class KERN>
+int persistent_grid(KERN kern, size_t shm, int cus)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 64, shm) != hipSuccess || nb < 1) nb = 1;
+    return nb * cus;
+}
+
+// the stage launches of one round (every argument but the table address is
+// fixed per kernel variant, so the sequence can be captured and replayed)
+template <class T, int C, int NVM>
+void multi_launch(hipStream_t s, const MultiTab* tab, int tcap, int cus)
+{
+    const size_t shm = (sizeof(float4) + (NVM > 0 ? sizeof(float) : 0)) * (size_t)tcap;
+    auto ev_real = km_eval_tiles<T, T, false, C, NVM>;
+    auto ev_rand = km_eval_tiles<T, double, true, C, NVM>;
+    // persistent grids of the two evaluations (occupancy queried once per
+    // variant, before any capture)
+    static const int g_real = persistent_grid(ev_real, shm, cus);
+    static const int g_rand = persistent_grid(ev_rand, shm, cus);
+    hipLaunchKernelGGL(km_zero, dim3(MG_256), dim3(BLOCK), 0, s, tab);
+    hipLaunchKernelGGL(km_radii<T>, dim3(MG_256), dim3(BLOCK), 0, s, tab);
+    hipLaunchKernelGGL(km_grid, dim3(MG_64), dim3(64), 0, s, tab);
+    hipLaunchKernelGGL(km_wrap_atoms<T>, dim3(MG_256), dim3(BLOCK), 0, s, tab);
+    hipLaunchKernelGGL(km_scan_cells, dim3(MG_1024), dim3(1024), 0, s, tab);
+    hipLaunchKernelGGL(km_scatter, dim3(MG_256), dim3(BLOCK), 0, s, tab);
+    hipLaunchKernelGGL(km_near_mark, dim3(MG_256), dim3(BLOCK), 0, s, tab);
+    hipLaunchKernelGGL(km_prep_solvent_tiles<T>, dim3(MG_256), dim3(BLOCK), 0, s, tab);
+    hipLaunchKernelGGL(km_scan_rows<false>, dim3(MG_1024), dim3(1024), 0, s, tab);
+    hipLaunchKernelGGL(km_mol_scatter<false>, dim3(MG_256), dim3(BLOCK), 0, s, tab);
+    hipLaunchKernelGGL(km_fill_real<T>, dim3(MG_256), dim3(BLOCK), 0, s, tab);
+    hipLaunchKernelGGL(km_chunk_table<false>, dim3(MG_1024), dim3(1024), 0, s, tab);
+    hipLaunchKernelGGL(km_bulk_init, dim3(MG_256), dim3(BLOCK), 0, s, tab);
+    hipLaunchKernelGGL(ev_real, dim3(g_real), dim3(64), shm, s, tab);
+    hipLaunchKernelGGL(km_bulk_compact, dim3(MG_1024), dim3(1024), 0, s, tab);
+    hipLaunchKernelGGL(km_place_keys, dim3(MG_256), dim3(BLOCK), 0, s, tab);
+    hipLaunchKernelGGL(km_scan_rows<true>, dim3(MG_1024), dim3(1024), 0, s, tab);
+    hipLaunchKernelGGL(km_mol_scatter<true>, dim3(MG_256), dim3(BLOCK), 0, s, tab);
+    hipLaunchKernelGGL(km_place_near<T>, dim3(MG_256), dim3(BLOCK), 0, s, tab);
+    hipLaunchKernelGGL(km_chunk_table<true>, dim3(MG_1024), dim3(1024), 0, s, tab);
+    hipLaunchKernelGGL(ev_rand, dim3(g_rand), dim3(64), shm, s, tab);
+    hipLaunchKernelGGL(km_geo_flags, dim3(MG_64), dim3(64), 0, s, tab);
+}
+
+template <class T, int NVM>
+void multi_launch_c(int c, hipStream_t s, const MultiTab* tab, int tcap, int cus)
+{
+    if (c == 8) multi_launch<T, 8, NVM>(s, tab, tcap, cus);
+    else if (c == 4) multi_launch<T, 4, NVM>(s, tab, tcap, cus);
+    else if (c == 2) multi_launch<T, 2, NVM>(s, tab, tcap, cus);
+    else multi_launch<T, 1, NVM>(s, tab, tcap, cus);
+}
+
+void multi_launch_any(bool f32, int mol_lane, int c, hipStream_t s, const MultiTab* tab, int tcap, int cus)
+{
+    if (f32) {
+        if (mol_lane) multi_launch_c<float, 3>(c, s, tab, tcap, cus);
+        else multi_launch_c<float, 0>(c, s, tab, tcap, cus);
+    } else {
+        if (mol_lane) multi_launch_c<double, 3>(c, s, tab, tcap, cus);
+        else multi_launch_c<double, 0>(c, s, tab, tcap, cus);
+    }
+}
+
+// one job's part of a multi-job submission
+struct MJob {
+    cm_job* J = nullptr;
+    const cm_frames* fr = nullptr;
+    int64_t cap = 0, kcap = 0;
+    std::vector<std::pair<int, int>> batches;   // (first frame, frames)
+};
+
+// blocks of each stage's grid for one job's batch of F frames, as the
+// single-job launches size them: (x extent, total)
+void stage_blocks(const KP& P, int F, const int64_t z16[3], int nbx[MS_N], int cnt[MS_N])
+{
+    auto set = [&](int st, int64_t x, int64_t y) {
+        x = std::max<int64_t>(x, 1);
+        nbx[st] = (int)x;
+        cnt[st] = (int)(x * std::max<int64_t>(y, 0));
+    };
+    const int64_t mvb = (P.Mv + BLOCK - 1) / BLOCK;
+    const int K = P.K;
+    set(MS_ZERO, std::min<int64_t>(1024, std::max<int64_t>(1, (z16[0] + z16[1] + z16[2] + BLOCK - 1) / BLOCK)), 1);
+    set(MS_RADII, std::min<int64_t>(mvb, 64), F);
+    set(MS_GRID, (F + 63) / 64, 1);
+    set(MS_WRAP, grid_for((int64_t)F * P.nu), 1);
+    set(MS_SCAN_CELLS, F, 1);
+    set(MS_SCATTER, grid_for((int64_t)F * P.nu), 1);
+    set(MS_NEAR, grid_for((int64_t)F * P.nu), 1);
+    set(MS_PREP, grid_for((int64_t)F * P.Mv), 1);
+    set(MS_SCAN_ROWS0, F, 1);
+    set(MS_MOL_SCATTER0, mvb, std::min(F, 65535));
+    set(MS_FILL, grid_for((int64_t)F * P.Mv), 1);
+    set(MS_CHUNK0, 1, 1);
+    set(MS_BULK_INIT, grid_for((int64_t)F * P.kb * P.Mv), K > 0 ? 1 : 0);
+    set(MS_BULK_COMPACT, (int64_t)F * P.kb, K > 0 ? 1 : 0);
+    set(MS_PLACE_KEYS, mvb, K > 0 ? std::min(F * K, 65535) : 0);
+    set(MS_SCAN_ROWSK, (int64_t)F * K, K > 0 ? 1 : 0);
+    set(MS_MOL_SCATTERK, mvb, K > 0 ? std::min(F * K, 65535) : 0);
+    set(MS_PLACE_NEAR, mvb, K > 0 ? std::min(F * K, 65535) : 0);
+    set(MS_CHUNKK, 1, K > 0 ? 1 : 0);
+    set(MS_GEO_FLAGS, 1, 1);
+}
+
+// the rounds of one group of jobs that share the kernel variant
+int multi_run_group(MultiRunner& R, std::vector<MJob>& grp, int dtype)
+{
+    const int ds = dtype == CM_F32 ? 4 : 8;
+    const cm_job* J0 = grp[0].J;
+    const int variant = (dtype == CM_F32 ? 1 : 0) | (J0->kp.mol_lane ? 2 : 0) | (J0->tile_c << 2);
+    size_t maxr = 0;
+    for (auto& m : grp) maxr = std::max(maxr, m.batches.size());
+    for (size_t r = 0; r < maxr; r++) {
+        std::vector<MJob*> act;
+        for (auto& m : grp)
+            if (r < m.batches.size()) act.push_back(&m);
+        for (size_t a0 = 0; a0 < act.size(); a0 += MS_MAX) {
+            const int n = (int)std::min<size_t>(MS_MAX, act.size() - a0);
+            // the staging buffer's previous upload must have been read
+            HIPCHK(hipEventSynchronize(R.up[R.cur]));
+            MultiTab* T = (MultiTab*)R.htab[R.cur];
+            Geo* gh = (Geo*)((char*)R.htab[R.cur] + MT_GEO_OFF);
+            Geo* gd = (Geo*)((char*)R.tab + MT_GEO_OFF);
+            T->nsys = n;
+            T->pad = 0;
+            int64_t goff = 0;
+            int cnt[MS_N];
+            for (int st = 0; st < MS_N; st++) T->base[st][0] = 0;
+            for (int i = 0; i < n; i++) {
+                MJob& m = *act[a0 + i];
+                cm_job* J = m.J;
+                const cm_frames& fr = *m.fr;
+                const int f0 = m.batches[r].first, F = m.batches[r].second;
+                const double w = fr.weights[f0];
+                if (J->pending && J->pending_w != w)
+                    if (int e = flush(J, J->pending_w, R.stream)) return e;
+                J->pending = true;
+                J->pending_w = w;
+                for (int f = 0; f < F; f++) {
+                    cm::Cell64 c;
+                    cm::cell_setup(fr.cells + 9 * (f0 + f), J->d.triclinic, &c);
+                    J->volume_total += w * cm::cell_volume(c.H);
+                }
+                if (int e = batch_geo(J, F, fr.frame_index + f0, fr.cells + 9 * f0, m.cap, m.kcap, gh + goff)) return e;
+                SysDev& S = T->sys[i];
+                memset(&S, 0, sizeof(SysDev));
+                S.P = J->kp;
+                S.P.F = F;
+                S.P.cap = (int)m.cap;
+                S.P.kcap = (int)m.kcap;
+                S.B = batch_bufs(J, gd + goff);
+                S.xu = (const char*)fr.d_x_solute + (size_t)f0 * J->Nu * 3 * ds;
+                S.xv = (const char*)fr.d_x_solvent + (size_t)f0 * J->Nv * 3 * ds;
+                S.zp[0] = (uint4*)J->cells_cnt.p;
+                S.zp[1] = (uint4*)J->near.p;
+                S.zp[2] = (uint4*)J->mcnt.p;
+                S.z16[0] = (int64_t)(round16(sizeof(int) * (size_t)F * (m.cap + 1)) / 16);
+                S.z16[1] = (int64_t)(round16((size_t)F * m.cap) / 16);
+                S.z16[2] = (int64_t)(round16((size_t)4 * F * S.P.NP * (m.kcap + 1)) / 16);
+                stage_blocks(S.P, F, S.z16, S.nbx, cnt);
+                for (int st = 0; st < MS_N; st++) T->base[st][i + 1] = T->base[st][i] + cnt[st];
+                goff += F;
+                J->frames += F;
+            }
+            HIPCHK(hipMemcpyAsync(R.tab, R.htab[R.cur], MT_GEO_OFF + sizeof(Geo) * (size_t)goff, hipMemcpyHostToDevice,
+                                  R.stream));
+            HIPCHK(hipEventRecord(R.up[R.cur], R.stream));
+            R.cur ^= 1;
+            const bool f32 = dtype == CM_F32;
+            if (R.graphs) {
+                auto it = R.exec.find(variant);
+                if (it == R.exec.end()) {
+                    // first round of this variant: launched directly (which also
+                    // settles the occupancy queries), captured from the next one
+                    multi_launch_any(f32, J0->kp.mol_lane, J0->tile_c, R.stream, (const MultiTab*)R.tab, J0->kp.tcap,
+                                     R.cus);
+                    HIPCHK(hipGetLastError());
+                    R.exec.emplace(variant, nullptr);
+                    R.rounds++;
+                    R.launches += MS_N + 2;
+                    continue;
+                }
+                if (!it->second) {
+                    hipGraph_t g = nullptr;
+                    hipGraphExec_t x = nullptr;
+                    HIPCHK(hipStreamBeginCapture(R.stream, hipStreamCaptureModeThreadLocal));
+                    multi_launch_any(f32, J0->kp.mol_lane, J0->tile_c, R.stream, (const MultiTab*)R.tab, J0->kp.tcap, R.cus);
+                    HIPCHK(hipStreamEndCapture(R.stream, &g));
+                    HIPCHK(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+                    HIPCHK(hipGraphDestroy(g));
+                    it->second = x;
+                }
+                HIPCHK(hipGraphLaunch(it->second, R.stream));
+            } else {
+                multi_launch_any(f32, J0->kp.mol_lane, J0->tile_c, R.stream, (const MultiTab*)R.tab, J0->kp.tcap, R.cus);
+                HIPCHK(hipGetLastError());
+            }
+            R.rounds++;
+            R.launches += MS_N + 2;
+        }
+    }
+    return CM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cm_jobs_submit_device(int32_t njobs, cm_job* const* jobs, const cm_frames* frames)
+{
+    if (njobs < 0 || (njobs > 0 && (!jobs || !frames))) return fail(CM_EINVAL, "null argument");
+    for (int k = 0; k < njobs; k++) {
+        cm_job* J = jobs[k];
+        const cm_frames& fr = frames[k];
+        if (!J) return fail(CM_EINVAL, "null job");
+        if (J->finished) return fail(CM_ESTATE, "job already finished");
+        for (int q = 0; q < k; q++)
+            if (jobs[q] == J) return fail(CM_EINVAL, "a job appears twice");
+        if (fr.nframes < 0 || (fr.nframes > 0 && (!fr.frame_index || !fr.d_x_solute || !fr.d_x_solvent || !fr.cells ||
+                                                  !fr.weights)))
+            return fail(CM_EINVAL, "invalid frame arrays");
+        if (fr.dtype != CM_F64 && fr.dtype != CM_F32) return fail(CM_EINVAL, "dtype must be CM_F64 or CM_F32");
+        for (int f = 0; f < fr.nframes; f++) {
+            if (!(fr.weights[f] > 0.0) || !std::isfinite(fr.weights[f])) return fail(CM_EINVAL, "frame weights must be positive and finite");
+            if (fr.frame_index[f] < 1 || fr.frame_index[f] > 0xFFFFFFFFll) return fail(CM_EINVAL, "frame_index must be in [1, 2^32)");
+        }
+    }
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MR_MAX_DEVICES) return fail(CM_EUNSUPPORTED, "device index beyond the supported range");
+    MultiRunner& R = multi_runners()[dev];
+    std::lock_guard<std::mutex> lk(R.m);
+    if (int r = multi_init(R)) return r;
+    // the previous submission's rounds may still use buffers that are about
+    // to grow
+    HIPCHK(hipStreamSynchronize(R.stream));
+    // groups of jobs sharing the kernel variant; the others run on their own.
+    // The jobs of a submission hold their batch buffers at once: each gets an
+    // equal share of MULTI_BUDGET for its frames per round (at least one).
+    int nmulti = 0;
+    for (int k = 0; k < njobs; k++)
+        if (frames[k].nframes > 0 && jobs[k]->atoms_family && !jobs[k]->verify && !jobs[k]->arena && !jobs[k]->profiling)
+            nmulti++;
+    const int64_t share = MULTI_BUDGET / std::max(1, nmulti);
+    std::map<int, std::vector<MJob>> groups;
+    std::map<int, int> group_dtype;
+    for (int k = 0; k < njobs; k++) {
+        cm_job* J = jobs[k];
+        const cm_frames& fr = frames[k];
+        if (fr.nframes == 0) continue;
+        if (!J->atoms_family || J->verify || J->arena || J->profiling) {
+            if (int r = submit_common(J, fr.nframes, fr.frame_index, fr.d_x_solute, fr.d_x_solvent, fr.dtype, fr.cells,
+                                      fr.weights, false))
+                return r;
+            continue;
+        }
+        const int ds = fr.dtype == CM_F32 ? 4 : 8;
+        MJob m;
+        m.J = J;
+        m.fr = &fr;
+        int64_t Fmax, pfb;
+        if (int r = batch_plan(J, fr.nframes, fr.cells, ds, false, &m.cap, &m.kcap, &Fmax, &pfb)) return r;
+        Fmax = std::max<int64_t>(1, std::min<int64_t>({Fmax, (int64_t)MS_FMAX, share / std::max<int64_t>(pfb, 1)}));
+        // batches as the single-job path cuts them: equal batches over each
+        // run of one frame weight
+        int Fbig = 1;
+        for (int f0 = 0; f0 < fr.nframes;) {
+            const double w = fr.weights[f0];
+            int run = 1;
+            while (f0 + run < fr.nframes && fr.weights[f0 + run] == w) run++;
+            const int nb = (int)((run + Fmax - 1) / Fmax);
+            const int F = (run + nb - 1) / nb;
+            for (int b = 0; b < run; b += F) m.batches.emplace_back(f0 + b, std::min(F, run - b));
+            Fbig = std::max(Fbig, F);
+            f0 += run;
+        }
+        // the job's own stream may still run earlier work on these buffers
+        HIPCHK(hipStreamSynchronize(J->stream));
+        if (int r = alloc_batch(J, Fbig, m.cap, m.kcap, ds, false)) return r;
+        const int key = (fr.dtype == CM_F32 ? 1 : 0) | (J->kp.mol_lane ? 2 : 0) | (J->tile_c << 2);
+        groups[key].push_back(m);
+        group_dtype[key] = fr.dtype;
+    }
+    for (auto& kv : groups)
+        if (int r = multi_run_group(R, kv.second, group_dtype[kv.first])) return r;
+    // the jobs' own streams (cm_job_finish, later submits) wait for the rounds
+    HIPCHK(hipEventRecord(R.done, R.stream));
+    for (auto& kv : groups)
+        for (auto& m : kv.second) HIPCHK(hipStreamWaitEvent(m.J->stream, R.done, 0));
+    return CM_OK;
+}
+
+int cm_multi_set_graphs(int32_t on)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MR_MAX_DEVICES) return fail(CM_EUNSUPPORTED, "device index beyond the supported range");
+    MultiRunner& R = multi_runners()[dev];
+    std::lock_guard<std::mutex> lk(R.m);
+    R.graphs = on != 0;
+    return CM_OK;
+}
+
+int cm_multi_stats(int64_t* rounds, int64_t* launches)
+{
+    if (!rounds || !launches) return fail(CM_EINVAL, "null argument");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MR_MAX_DEVICES) return fail(CM_EUNSUPPORTED, "device index beyond the supported range");
+    MultiRunner& R = multi_runners()[dev];
+    std::lock_guard<std::mutex> lk(R.m);
+    *rounds = R.rounds;
+    *launches = R.launches;
+    return CM_OK;
 }
 
 }  // extern "C"

@@ -105,6 +105,22 @@ typedef struct cm_kernel_stats {
     int64_t fp64_rechecks;      /* molecules resolved with the exhaustive FP64 pass */
 } cm_kernel_stats;
 
+/* MinimumDistance record (src/minimum_distances.jl:12-19), as the
+ * verification getters return it: within_cutoff, i (molecule-local solute
+ * atom, 0-based), j (global solvent atom, 0-based; in a random pass, of the
+ * placed slot), d, ref_atom_within_cutoff, d_ref_atom.  A molecule with no
+ * solute atom within the cutoff has zero(MinimumDistance): 0, -1, -1, Inf,
+ * 0, Inf (src/minimum_distances.jl:21-28). */
+typedef struct cm_md_record {
+    int32_t within_cutoff;
+    int32_t i;
+    int64_t j;
+    double d;
+    int32_t ref_atom_within_cutoff;
+    int32_t pad;
+    double d_ref_atom;
+} cm_md_record;
+
 /* Creates a job: validates the description, copies the group maps, allocates
  * device accumulators.  Replaces Result(trajectory, options) +
  * build_particle_system (src/results.jl:124-193, src/minimum_distances.jl:157-176)
@@ -134,6 +150,39 @@ int cm_job_submit_device(cm_job *job, int32_t nframes, const int64_t *frame_inde
  * of src/mddf.jl:336).  The job may not be submitted to afterwards. */
 int cm_job_finish(cm_job *job, cm_raw *out);
 
+/* Frames of one job for cm_jobs_submit_device: the arguments of
+ * cm_job_submit_device. */
+typedef struct cm_frames {
+    int32_t nframes;
+    int32_t dtype;
+    const int64_t *frame_index;
+    const void *d_x_solute;
+    const void *d_x_solvent;
+    const double *cells;
+    const double *weights;
+} cm_frames;
+
+/* cm_job_submit_device for many jobs at once: the results of every job are
+ * those of its own cm_job_submit_device, but the work of all of them runs
+ * together.  Jobs with one solute molecule that share the coordinate type
+ * and the tile tuning run in rounds: a round takes one batch of frames of
+ * every job (batches cut as cm_job_submit_device cuts them), and each stage
+ * of the path is one launch over all of them, captured once in a HIP graph
+ * and replayed per round.  Other jobs (many solute molecules, verification or
+ * profiling on) are submitted one by one.  The jobs of a call hold their
+ * batch buffers at the same time: each takes an equal share of 64 GiB for its
+ * frames per round (at least one frame).  Returns once the work is queued;
+ * each job's cm_job_finish waits for its part.  The device coordinates must
+ * stay valid until then.  A job may appear once per call.  No counterpart in
+ * the reference (whose unit of work is one mddf call). */
+int cm_jobs_submit_device(int32_t njobs, cm_job *const *jobs, const cm_frames *frames);
+/* Rounds of cm_jobs_submit_device replay a captured graph (on, the default)
+ * or launch their stages one by one (off); results are the same. */
+int cm_multi_set_graphs(int32_t on);
+/* Rounds run and stage launches issued so far by cm_jobs_submit_device on
+ * the calling thread's device. */
+int cm_multi_stats(int64_t *rounds, int64_t *launches);
+
 /* Tuning of the single-solute kernels (no counterpart in the reference; the
  * reference's Options.lcell plays this role for CellListMap): fine cell side
  * as a fraction of the cutoff (0.2-1.0), columns per row of a wavefront (1, 2,
@@ -152,6 +201,32 @@ int cm_job_kernel_stats(cm_job *job, cm_kernel_stats *out);
  * the random-pass evaluation launch (ms; -1 when the batch had no random
  * passes).  Writes min(cap, batches) entries and the number of batches to *n. */
 int cm_job_launch_times(cm_job *job, int32_t cap, double *real_ms, double *random_ms, int32_t *n);
+
+/* Verification mode (before the first submit): the job keeps, for every
+ * frame it processes, the per-molecule records of the real pass and of each
+ * random pass, the placed solvent coordinates of each random pass and the
+ * drawn reference solute molecules, so they can be compared with an FP64
+ * reference record by record.  Records are FP64 exact (the distances of
+ * records that the FP32 stage decided are recomputed).  Host memory grows with
+ * frames x passes x molecules: meant for small checks.  No counterpart in the
+ * reference (its per-pass state is system.list, src/mddf.jl:380-426). */
+int cm_job_set_verify(cm_job *job, int32_t on);
+/* Records of frame `frame` (1-based trajectory number), pass 0 (real) or
+ * 1..n_random_samples: out[nmols_v], one per solvent molecule (slot), the
+ * list that update_counters! reads (src/update_counters.jl:43-88; the
+ * records of minimum_distances!, src/minimum_distances.jl:129-148).  Jobs with
+ * one solute molecule (the tile kernels) only: CM_EUNSUPPORTED otherwise. */
+int cm_job_records(cm_job *job, int64_t frame, int32_t pass, cm_md_record *out);
+/* Placed solvent of random pass `pass` (1..n_random_samples) of a frame, as
+ * randomize_solvent! leaves system.ypositions (src/mddf.jl:65-88):
+ * out[nmols_v * natoms_per_mol_v][3]; placed[nmols_v] is 1 for the slots the
+ * device placed (jobs with one solute molecule place only the slots whose
+ * centre lies near the solute, the others cannot reach the cutoff; their
+ * coordinates are NaN). */
+int cm_job_placement(cm_job *job, int64_t frame, int32_t pass, double *out, uint8_t *placed);
+/* The solute molecule (0-based) each random pass of a frame was drawn for
+ * (src/mddf.jl:374-376): out[n_random_samples]. */
+int cm_job_reference_draws(cm_job *job, int64_t frame, int32_t *out);
 
 void cm_job_destroy(cm_job *job);
 
