@@ -1589,7 +1589,8 @@ constexpr int TCAP_DEFAULT = 640;   // staged solute atoms per round (10 KiB of 
 constexpr int RB_PIECES = 8;        // read-back pieces of cm_job_finish
 constexpr int TCAP_MOL = 512;       // the same with the squared norms (20 B each: 10 KiB)
 constexpr int CBCAP = 1024;         // chunk-table entries kept in LDS
-constexpr int SU = 4;               // staging: ranks per lane per step
+constexpr int SU = 4;               // staging: atoms per lane per step
+constexpr int SEG = 8;              // staging: atoms of a row per lane (segment)
 
 __device__ __forceinline__ int wrap_index(int i, int n, int& m)
 {
@@ -1619,6 +1620,19 @@ __device__ __forceinline__ void cell_range(const Geo& g, float lx, float ly, flo
 {
     const double* Hi = g.cell.Hi;
     double fmn[3] = {INFINITY, INFINITY, INFINITY}, fmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (g.ortho) {
+        // diagonal cell: the corners' fractional coordinates are the two
+        // products per dimension (the off-diagonal terms are exact zeros, so
+        // the values equal the general form's)
+        const double lo3[3] = {(double)lx - r, (double)ly - r, (double)lz - r};
+        const double hi3[3] = {(double)hx + r, (double)hy + r, (double)hz + r};
+        for (int k = 0; k < 3; k++) {
+            const double a = Hi[4 * k] * lo3[k], b = Hi[4 * k] * hi3[k];
+            ilo[k] = (int)floor(fmin(a, b) * g.nc[k] - 1e-6);
+            ihi[k] = (int)floor(fmax(a, b) * g.nc[k] + 1e-6);
+        }
+        return;
+    }
     for (int c = 0; c < 8; c++) {
         double x = (c & 1) ? (double)hx + r : (double)lx - r;
         double y = (c & 2) ? (double)hy + r : (double)ly - r;
@@ -2178,58 +2192,62 @@ __device__ __forceinline__ void eval_tiles_chunks(const KP& P, const Bufs& B, co
                     shift_of(g, mx, my, mz, tx, ty, tz);
                 }
             }
-            const int incl = wave_incl_scan(cnt);
-            const int excl = incl - cnt;
-            const int T = __builtin_amdgcn_readlane(incl, 63);
-            for (int t0 = 0; t0 < T; t0 += 64 * SUk) {
-                // owners of the ranks (largest j with excl[j] <= t), then all
-                // loads in flight before any is used
-                int sidx[SUk];
-                float sx[SUk], sy[SUk], sz[SUk];
+            // The atoms of the rows are staged in segments of up to SEG
+            // consecutive atoms: each lane takes one segment per step (owner
+            // row found by a search over the wave's segment prefix, once per
+            // SEG atoms), loads its atoms SUk at a time with the loads in
+            // flight together, and appends the ones within rs of the AABB at
+            // ballot/popcount offsets.  The staged set does not depend on the
+            // order (FP32 ties between candidates go to the FP64 resolve).
+            const int nseg = (cnt + SEG - 1) / SEG;
+            const int sincl = wave_incl_scan(nseg);
+            const int sexcl = sincl - nseg;
+            const int NS = __builtin_amdgcn_readlane(sincl, 63);
+            for (int s0 = 0; s0 < NS; s0 += 64) {
+                const int sg = s0 + lane;
+                int j = 0;
 #pragma unroll
-                for (int u = 0; u < SUk; u++) {
-                    const int t = t0 + 64 * u + lane;
-                    int j = 0;
-#pragma unroll
-                    for (int step = 32; step >= 1; step >>= 1) {
-                        const int e = __shfl(excl, (j + step) & 63, 64);
-                        j = (j + step < 64 && e <= t) ? j + step : j;
-                    }
-                    const int sj = __shfl(cstart, j, 64), ej = __shfl(excl, j, 64);
-                    sx[u] = __shfl(tx, j, 64);
-                    sy[u] = __shfl(ty, j, 64);
-                    sz[u] = __shfl(tz, j, 64);
-                    sidx[u] = t < T ? sj + (t - ej) : 0;
+                for (int step = 32; step >= 1; step >>= 1) {
+                    const int e = __shfl(sexcl, (j + step) & 63, 64);
+                    j = (j + step < 64 && e <= sg) ? j + step : j;
                 }
-                float4 q[SUk];
+                const int sj = __shfl(cstart, j, 64), cj = __shfl(cnt, j, 64), ej = __shfl(sexcl, j, 64);
+                const float ox = __shfl(tx, j, 64), oy = __shfl(ty, j, 64), oz = __shfl(tz, j, 64);
+                const int k0 = (sg - ej) * SEG;
+                const int n = sg < NS ? min(SEG, cj - k0) : 0;
+                const int abase = sj + k0;
+                const int nmax = wave_max_i(n);
+                for (int kk = 0; kk < nmax; kk += SUk) {
+                    float4 q[SUk];
 #pragma unroll
-                for (int u = 0; u < SUk; u++) q[u] = srt[sidx[u]];
+                    for (int u = 0; u < SUk; u++) q[u] = srt[kk + u < n ? abase + kk + u : 0];
 #pragma unroll
-                for (int u = 0; u < SUk; u++) q[u] = mk4(q[u].x + sx[u], q[u].y + sy[u], q[u].z + sz[u], q[u].w);
+                    for (int u = 0; u < SUk; u++) q[u] = mk4(q[u].x + ox, q[u].y + oy, q[u].z + oz, q[u].w);
 #pragma unroll
-                for (int u = 0; u < SUk; u++) {
-                    if (t0 + 64 * u >= T) break;
-                    if (nst + 64 > tcap) {
-                        evaluate(nst);
-                        rounds++;
-                        nst = 0;
-                    }
-                    const float dx = fmaxf(fmaxf(lx - q[u].x, q[u].x - hx), 0.f);
-                    const float dy = fmaxf(fmaxf(ly - q[u].y, q[u].y - hy), 0.f);
-                    const float dz = fmaxf(fmaxf(lz - q[u].z, q[u].z - hz), 0.f);
-                    const bool pass = t0 + 64 * u + lane < T && dx * dx + dy * dy + dz * dz <= rs2;
-                    const unsigned long long m = __ballot(pass);
-                    if (pass) {
-                        const int slot = nst + __popcll(m & lt_mask);
-                        if constexpr (NVM > 0) {
-                            const float sx_ = q[u].x - ccx, sy_ = q[u].y - ccy, sz_ = q[u].z - ccz;
-                            stg[slot] = mk4(sx_, sy_, sz_, q[u].w);
-                            stw[slot] = fmaf(sx_, sx_, fmaf(sy_, sy_, sz_ * sz_));
-                        } else {
-                            stg[slot] = q[u];
+                    for (int u = 0; u < SUk; u++) {
+                        if (kk + u >= nmax) break;
+                        if (nst + 64 > tcap) {
+                            evaluate(nst);
+                            rounds++;
+                            nst = 0;
                         }
+                        const float dx = fmaxf(fmaxf(lx - q[u].x, q[u].x - hx), 0.f);
+                        const float dy = fmaxf(fmaxf(ly - q[u].y, q[u].y - hy), 0.f);
+                        const float dz = fmaxf(fmaxf(lz - q[u].z, q[u].z - hz), 0.f);
+                        const bool pass = kk + u < n && dx * dx + dy * dy + dz * dz <= rs2;
+                        const unsigned long long m = __ballot(pass);
+                        if (pass) {
+                            const int slot = nst + __popcll(m & lt_mask);
+                            if constexpr (NVM > 0) {
+                                const float sx_ = q[u].x - ccx, sy_ = q[u].y - ccy, sz_ = q[u].z - ccz;
+                                stg[slot] = mk4(sx_, sy_, sz_, q[u].w);
+                                stw[slot] = fmaf(sx_, sx_, fmaf(sy_, sy_, sz_ * sz_));
+                            } else {
+                                stg[slot] = q[u];
+                            }
+                        }
+                        nst += __popcll(m);
                     }
-                    nst += __popcll(m);
                 }
             }
         }
@@ -3694,6 +3712,17 @@ int job_setup(cm_job* J, const cm_job_desc& d)
     if ((r = zero_async(J->acc.p, 8 * J->acc_len, J->stream))) return r;
     if ((r = zero_async(J->hits.p, 8 * 4 * nb, J->stream))) return r;
     if ((r = zero_async(J->stats.p, 64, J->stream))) return r;
+    // The page-locked read-back buffer is taken now, while the device is idle
+    // for this job: page-locking tens of MB (hipHostMalloc) while the job's
+    // kernels run stalled one of them about fourfold (a 2.4 ms evaluation took
+    // 9.6 ms in the first job of a process, where the cache has no block yet).
+    if (!J->arena) {
+        const size_t need = 8 * (size_t)J->acc_len + 8 * 4 * (size_t)nb + 8;
+        if (J->hres.bytes < need) {
+            pin_put(J->hres);
+            if ((r = pin_take(J->hres, need))) return r;
+        }
+    }
     // group maps
     std::vector<int32_t>& gm = J->hgm;
     gm.clear();
