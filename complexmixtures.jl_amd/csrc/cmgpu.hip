@@ -115,6 +115,7 @@ struct KP {                  // job parameters, by value
     int gnu, gnv;            // group counts of the solute / solvent side
     int NP;                  // passes per frame, 1 + K (tiles family)
     int kcap;                // molecule sort keys per (frame, pass): Morton codes of the fine cells
+    int place_all;           // molecule-pair family: place every slot of a random pass (verification)
 };
 
 // MinimumDistance record as the verification getters return it
@@ -148,7 +149,6 @@ struct Bufs {
     // random passes
     double* P;               // [F][K][Nv][3]
     float4* vw_r;            // [F][K][Nv]
-    int* vcell_r;            // [F][K][Mv]
     float4* vcen_r;          // [F][K][Mv]
     unsigned char* bflag;    // [F][kb][Mv]
     int* blist;              // [F][kb][Mv]
@@ -1433,6 +1433,28 @@ __global__ void __launch_bounds__(BLOCK, 3) k_eval_mols_real(KP P, Bufs B, const
     if (shg) groups_flush(P, shg, B.cnt_gu, B.cnt_gv);
 }
 
+// Whether placed slot v (centre cv) can have an atom pair with the drawn
+// solute molecule (centre cu) within the cutoff under some lattice shift:
+// the centre pre-check of the random passes of the molecule-pair family.
+// k_place and k_eval_mols_rand evaluate it with the same FP32 operations on
+// the same values, so a slot skipped by the placement is one the evaluation
+// skips.
+__device__ __forceinline__ bool mol_pair_near(const Geo& g, float4 cv, float4 cu)
+{
+    bool any = false;
+    for (int mz = -1; mz <= 1; mz++)
+    for (int my = -1; my <= 1; my++)
+    for (int mx = -1; mx <= 1; mx++) {
+        float fx = (float)mx, fy = (float)my, fz = (float)mz;
+        float tx = g.H32[0] * fx + g.H32[3] * fy + g.H32[6] * fz;
+        float ty = g.H32[1] * fx + g.H32[4] * fy + g.H32[7] * fz;
+        float tz = g.H32[2] * fx + g.H32[5] * fy + g.H32[8] * fz;
+        float dx = cv.x - (cu.x + tx), dy = cv.y - (cu.y + ty), dz = cv.z - (cu.z + tz);
+        any |= !(dx * dx + dy * dy + dz * dz > g.rpre2);
+    }
+    return any;
+}
+
 template <class TU>
 __global__ void __launch_bounds__(BLOCK) k_eval_mols_rand(KP P, Bufs B, const TU* xu)
 {
@@ -1454,7 +1476,8 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_rand(KP P, Bufs B, const TU
         const float4* xs = B.solw + (int64_t)f * P.Nu + (int64_t)u * P.nu;
         Best bm, br;
         bm.init(); br.init();
-        bool any = false;
+        // the slots k_place skipped fail this test (the same one)
+        if (!mol_pair_near(g, cv, cu)) continue;
         for (int mz = -1; mz <= 1; mz++)
         for (int my = -1; my <= 1; my++)
         for (int mx = -1; mx <= 1; mx++) {
@@ -1464,10 +1487,8 @@ __global__ void __launch_bounds__(BLOCK) k_eval_mols_rand(KP P, Bufs B, const TU
             float tz = g.H32[2] * fx + g.H32[5] * fy + g.H32[8] * fz;
             float dx = cv.x - (cu.x + tx), dy = cv.y - (cu.y + ty), dz = cv.z - (cu.z + tz);
             if (dx * dx + dy * dy + dz * dz > g.rpre2) continue;
-            any = true;
             pairs_mol(P, xs, yw, tx, ty, tz, bm, br);
         }
-        if (!any) continue;
         const TU* xuf = xu + (int64_t)f * P.Nu * 3;
         int64_t jbase = (int64_t)v * P.nv;
         auto ysrc = [&](int a, double& y0, double& y1, double& y2) {
@@ -1561,24 +1582,29 @@ __global__ void k_place(KP P, Bufs B, const TV* xv)
         const int k = fk - f * P.K;
         const Geo& g = B.geo[f];
         cm::u32x4 r = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT}, P.key0, P.key1);
+        double uc[3] = {cm::u01(r.y), cm::u01(r.z), cm::u01(r.w)};
+        // the new centre H*uc (inside the cell)
+        const double* H = g.cell.H;
+        double c0 = H[0] * uc[0] + H[3] * uc[1] + H[6] * uc[2];
+        double c1 = H[1] * uc[0] + H[4] * uc[1] + H[7] * uc[2];
+        double c2 = H[2] * uc[0] + H[5] * uc[1] + H[8] * uc[2];
+        const float4 cv = mk4((float)c0, (float)c1, (float)c2, 0.f);
+        B.vcen_r[(int64_t)fk * P.Mv + v] = cv;
+        // only slots the evaluation can reach are placed (all of them when
+        // the placements are kept for verification)
+        if (!P.place_all) {
+            const int u = B.uk[fk];
+            if (P.self_ && u == v) continue;
+            if (!mol_pair_near(g, cv, B.ucen[(int64_t)f * P.Mu + u])) continue;
+        }
         cm::u32x4 r2 = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT + 1u}, P.key0, P.key1);
         int64_t lk = (int64_t)f * P.kb + (P.kb == 1 ? 0 : k);
         int nb = B.nbulk[lk];
         int64_t jmol = nb > 0 ? (int64_t)B.blist[lk * P.Mv + cm::draw_index(r.x, nb)] : cm::draw_index(r.x, P.Mv);
         MolSrc<TV> src{xv + (int64_t)f * P.Nv * 3 + jmol * P.nv * 3};
         MolDst dst{B.P + (fk * P.Nv + (int64_t)v * P.nv) * 3, B.vw_r + fk * P.Nv + (int64_t)v * P.nv};
-        double uc[3] = {cm::u01(r.y), cm::u01(r.z), cm::u01(r.w)};
         double ua[3] = {cm::u01(r2.x), cm::u01(r2.y), cm::u01(r2.z)};
         cm::random_move(src, dst, P.nv, P.iref, g.cell, uc, ua);
-        // the new centre H*uc lies inside the cell: its cell follows from uc
-        int cell;
-        cell_of_frac(g, uc[0], uc[1], uc[2], cell);
-        B.vcell_r[(int64_t)fk * P.Mv + v] = cell;
-        const double* H = g.cell.H;
-        double c0 = H[0] * uc[0] + H[3] * uc[1] + H[6] * uc[2];
-        double c1 = H[1] * uc[0] + H[4] * uc[1] + H[7] * uc[2];
-        double c2 = H[2] * uc[0] + H[5] * uc[1] + H[8] * uc[2];
-        B.vcen_r[(int64_t)fk * P.Mv + v] = mk4((float)c0, (float)c1, (float)c2, 0.f);
     }
 }
 
@@ -2954,7 +2980,7 @@ struct cm_job {
     DevBuf gmap;   // group maps
     // batch buffers
     DevBuf geo, cells_cnt, cells_start, tmp4, tmpcell, sorted, solw, ucen, vw, vcell, vcen;
-    DevBuf P, vw_r, vcell_r, vcen_r, bflag, blist, nbulk, uk, drawn_ptr, drawn_idx;
+    DevBuf P, vw_r, vcen_r, bflag, blist, nbulk, uk, drawn_ptr, drawn_idx;
     DevBuf near, mkey, mcnt, mstart, msorted, nnear, chunk_base;
     DevBuf xin_u, xin_v;
     double cell_frac = 0.5;   // tiles family: fine cell side as a fraction of the cutoff
@@ -3039,7 +3065,7 @@ int64_t per_frame_bytes(const cm_job* J, int dtype_size, bool host_input, int64_
     if (!J->atoms_family) {
         b += J->Nu * 16 + Mu * 16 + (Mu + 1) * 4 + J->K * 8;
         b += J->Nv * 16 + Mv * 20;
-        b += (int64_t)J->K * (J->Nv * (24 + 16) + Mv * 20);
+        b += (int64_t)J->K * (J->Nv * (24 + 16) + Mv * 16);
     } else {
         b += cap + NP * ((kcap + 1) * 8 + Mv * 8 + 4) + 8;
         b += J->Nv * 16 + (int64_t)J->K * J->Nv * (24 + 16);
@@ -3216,7 +3242,6 @@ Bufs batch_bufs(const cm_job* J, Geo* geo)
     B.vcen = (float4*)J->vcen.p;
     B.P = (double*)J->P.p;
     B.vw_r = (float4*)J->vw_r.p;
-    B.vcell_r = (int*)J->vcell_r.p;
     B.vcen_r = (float4*)J->vcen_r.p;
     B.bflag = (unsigned char*)J->bflag.p;
     B.blist = (int*)J->blist.p;
@@ -3452,7 +3477,6 @@ int alloc_batch(cm_job* J, int F, int64_t cap, int64_t kcap, int dtype_size, boo
         if ((r = ensure(J, J->P, (size_t)24 * F * J->K * J->Nv))) return r;
         if ((r = ensure(J, J->vw_r, (size_t)16 * F * J->K * J->Nv))) return r;
         if (!J->atoms_family) {
-            if ((r = ensure(J, J->vcell_r, (size_t)4 * F * J->K * Mv))) return r;
             if ((r = ensure(J, J->vcen_r, (size_t)16 * F * J->K * Mv))) return r;
         }
         if ((r = ensure(J, J->bflag, (size_t)F * J->kb * Mv))) return r;
@@ -4052,6 +4076,7 @@ int cm_job_set_verify(cm_job* J, int32_t on)
     if (J->frames > 0) return fail(CM_ESTATE, "verification must be set before the first submit");
     if (J->arena) return fail(CM_EUNSUPPORTED, "no verification on the no-allocation path");
     J->verify = on != 0;
+    J->kp.place_all = J->verify ? 1 : 0;
     return CM_OK;
 }
 
@@ -4098,7 +4123,7 @@ void cm_job_destroy(cm_job* J)
     if (J->stream) hipStreamSynchronize(J->stream);
     DevBuf* bufs[] = {&J->cnt, &J->acc, &J->hits, &J->stats, &J->gmap, &J->geo, &J->cells_cnt, &J->cells_start,
                       &J->tmp4, &J->tmpcell, &J->sorted, &J->solw, &J->ucen, &J->vw, &J->vcell, &J->vcen, &J->P,
-                      &J->vw_r, &J->vcell_r, &J->vcen_r, &J->bflag, &J->blist, &J->nbulk, &J->uk, &J->drawn_ptr,
+                      &J->vw_r, &J->vcen_r, &J->bflag, &J->blist, &J->nbulk, &J->uk, &J->drawn_ptr,
                       &J->drawn_idx, &J->xin_u, &J->xin_v, &J->near, &J->mkey, &J->mcnt, &J->mstart,
                       &J->msorted, &J->nnear, &J->chunk_base, &J->vrec};
     for (DevBuf* b : bufs) dev_put(*b);
@@ -4872,7 +4897,7 @@ int cm_mddf_noalloc(const cm_job_desc* desc, int32_t nframes, const int64_t* fra
     // every buffer of the previous call is given up: the arena starts over
     DevBuf* bufs[] = {&J->cnt, &J->acc, &J->hits, &J->stats, &J->gmap, &J->geo, &J->cells_cnt, &J->cells_start,
                       &J->tmp4, &J->tmpcell, &J->sorted, &J->solw, &J->ucen, &J->vw, &J->vcell, &J->vcen, &J->P,
-                      &J->vw_r, &J->vcell_r, &J->vcen_r, &J->bflag, &J->blist, &J->nbulk, &J->uk, &J->drawn_ptr,
+                      &J->vw_r, &J->vcen_r, &J->bflag, &J->blist, &J->nbulk, &J->uk, &J->drawn_ptr,
                       &J->drawn_idx, &J->xin_u, &J->xin_v, &J->near, &J->mkey, &J->mcnt, &J->mstart,
                       &J->msorted, &J->nnear, &J->chunk_base};
     for (DevBuf* b : bufs) *b = DevBuf{};
