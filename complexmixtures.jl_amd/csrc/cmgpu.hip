@@ -2567,15 +2567,16 @@ struct MultiTab {
 template <int ST, bool SYNC, class Fn>
 __device__ __forceinline__ void multi_blocks(const MultiTab* __restrict__ MT, Fn fn)
 {
+    // the stage's prefix row in LDS; a block's virtual blocks increase, so
+    // its job index only moves forward
+    __shared__ int base[MS_MAX + 1];
     const int n = MT->nsys;
-    const int* base = MT->base[ST];
+    for (int k = threadIdx.x; k <= n; k += blockDim.x) base[k] = MT->base[ST][k];
+    __syncthreads();
     const int total = base[n];
+    int lo = 0;
     for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
-        int lo = 0, hi = n;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (base[mid] <= vb) lo = mid; else hi = mid;
-        }
+        while (base[lo + 1] <= vb) lo++;
         const SysDev& S = MT->sys[lo];
         const int local = vb - base[lo];
         const int nbx = S.nbx[ST];
@@ -2739,12 +2740,9 @@ __global__ void __launch_bounds__(64, 3) km_eval_tiles(const MultiTab* __restric
     }
     __syncthreads();
     const int total = cb[n];
+    int lo = 0;
     for (int g = blockIdx.x; g < total; g += gridDim.x) {
-        int lo = 0, hi = n;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (cb[mid] <= g) lo = mid; else hi = mid;
-        }
+        while (cb[lo + 1] <= g) lo++;
         const SysDev& S = MT->sys[lo];
         const int ch = g - cb[lo];
         float4* stg = dyn;
@@ -4439,9 +4437,14 @@ namespace {
 
 constexpr int MR_MAX_DEVICES = 64;
 constexpr int MS_FMAX = 48;        // frames of one job per round (the single-job batch cap)
-constexpr int MG_256 = 2048;       // blocks of the 256-thread stage launches
-constexpr int MG_1024 = 512;       // blocks of the 1024-thread stage launches
-constexpr int MG_64 = 256;         // blocks of the 64-thread stage launches
+// Blocks of the stage launches.  The slot stages have one virtual block per
+// (256 slots, frame, pass) of every job, most of which exit at once (their
+// slots are not near); a block walks its virtual blocks one after another,
+// so the grids are large enough that each walks a few dozen at most (a grid
+// of 2048 took 1.8x the single-job launches' time in the near placement).
+constexpr int MG_256 = 32768;      // blocks of the 256-thread stage launches
+constexpr int MG_1024 = 4096;      // blocks of the 1024-thread stage launches
+constexpr int MG_64 = 1024;        // blocks of the 64-thread stage launches
 constexpr int64_t MULTI_BUDGET = (int64_t)64 << 30;   // batch buffers of one submission's jobs together
 
 struct MultiRunner {
