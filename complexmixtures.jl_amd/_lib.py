@@ -76,7 +76,8 @@ EXPORTS = ["cm_job_create", "cm_job_submit", "cm_job_submit_device", "cm_job_fin
            "cm_job_placement", "cm_job_reference_draws", "cm_job_destroy", "cm_last_error", "cm_device_alloc", "cm_device_upload",
            "cm_device_free", "cm_device_synchronize", "cm_job_footprint", "cm_server_create", "cm_server_enqueue",
            "cm_server_wait", "cm_server_destroy", "cm_release_cached_memory", "cm_device_download",
-           "cm_mddf_noalloc", "cm_jobs_submit_device", "cm_multi_set_graphs", "cm_multi_stats"]
+           "cm_mddf_noalloc", "cm_jobs_submit_device", "cm_multi_set_graphs", "cm_multi_stats",
+           "cm_set_readback_pack_limit"]
 
 _lib = None
 
@@ -85,6 +86,12 @@ def release_cached_memory():
     """Returns the device and page-locked host blocks that finished jobs left
     cached for later jobs (cm_release_cached_memory)."""
     check(lib().cm_release_cached_memory())
+
+
+def set_readback_pack_limit(limit=0xffffffff):
+    """Largest hit count cm_job_finish reads back as a 32-bit word; a lower
+    limit sends jobs down the FP64 read-back (cm_set_readback_pack_limit)."""
+    check(lib().cm_set_readback_pack_limit(int(limit)))
 
 
 class GPUError(RuntimeError):
@@ -124,6 +131,7 @@ def lib():
         L.cm_server_destroy.argtypes = [ctypes.c_void_p]
         L.cm_server_destroy.restype = None
         L.cm_release_cached_memory.argtypes = []
+        L.cm_set_readback_pack_limit.argtypes = [_u64]
         L.cm_device_download.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         L.cm_jobs_submit_device.argtypes = [_i32, _P(ctypes.c_void_p), _P(Frames)]
         L.cm_multi_set_graphs.argtypes = [_i32]
@@ -135,7 +143,7 @@ def lib():
                   "cm_job_records", "cm_job_placement", "cm_job_reference_draws", "cm_device_alloc", "cm_device_upload", "cm_device_free",
                   "cm_device_synchronize", "cm_job_footprint", "cm_server_create", "cm_server_enqueue",
                   "cm_server_wait", "cm_device_download", "cm_mddf_noalloc", "cm_jobs_submit_device",
-                  "cm_multi_set_graphs", "cm_multi_stats"):
+                  "cm_multi_set_graphs", "cm_multi_stats", "cm_set_readback_pack_limit"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib

@@ -45,6 +45,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -2511,18 +2512,39 @@ __global__ void k_vrec_init(VRec* p, int64_t n)
     }
 }
 
-// integer hits -> FP64 accumulators
+// integer hits -> FP64 accumulators (assign: the first run of the job, every
+// entry written, so the accumulators need not hold zeros)
 __global__ void k_flush(ull* cnt, double* acc, ull* hits, int64_t n, int64_t nhist, int64_t half_lo, int64_t half_hi,
-                        double w)
+                        double w, int assign)
 {
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
         ull c = cnt[k];
-        if (!c) continue;
         double mult = (k >= half_lo && k < half_hi) ? w / 2 : w;
-        acc[k] += (double)c * mult;
+        if (assign) {
+            acc[k] = (double)c * mult;
+        } else {
+            if (!c) continue;
+            acc[k] += (double)c * mult;
+        }
         if (k < nhist) hits[k] += c;
         cnt[k] = 0ull;
     }
+}
+
+// Read-back of a job whose frames all had one weight: the hits as 32-bit
+// words (over[0] set when one exceeds limit, at most 2^32 - 1), the md/rdf
+// hits added to the cumulative ones.  The counts stay in cnt for the FP64
+// path.
+__global__ void k_pack32(const ull* cnt, uint32_t* out, ull* hits, int64_t n, int64_t nhist, ull limit, ull* over)
+{
+    bool big = false;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        const ull c = cnt[k];
+        out[k] = (uint32_t)c;
+        big |= c > limit;
+        if (k < nhist) hits[k] += c;
+    }
+    if (big) atomicOr(over, 1ull);
 }
 
 // ---------------------------------------------------------------------------
@@ -2939,6 +2961,62 @@ void pin_put(PinBuf& b)
     b.bytes = 0;
 }
 
+// Streams of destroyed jobs, kept per device for the next ones: a process
+// that runs one mddf call after another would otherwise create and destroy a
+// stream per call.  A stream comes back idle (cm_job_destroy waits for it).
+struct StreamPool {
+    std::mutex m;
+    std::vector<std::pair<int, hipStream_t>> s;   // (device, stream)
+};
+
+StreamPool& stream_pool()
+{
+    static StreamPool* p = new StreamPool;
+    return *p;
+}
+
+constexpr size_t STREAM_POOL_MAX = 64;
+
+int stream_take(hipStream_t* out)
+{
+    const int dev = current_device();
+    {
+        StreamPool& p = stream_pool();
+        std::lock_guard<std::mutex> lk(p.m);
+        for (size_t k = p.s.size(); k-- > 0;)
+            if (p.s[k].first == dev) {
+                *out = p.s[k].second;
+                p.s.erase(p.s.begin() + k);
+                return CM_OK;
+            }
+    }
+    if (hipStreamCreateWithFlags(out, hipStreamNonBlocking) != hipSuccess) {
+        *out = nullptr;
+        return fail(CM_EHIP, "hipStreamCreate failed");
+    }
+    return CM_OK;
+}
+
+void stream_put(hipStream_t st)
+{
+    if (!st) return;
+    int dev = 0;
+    if (hipStreamGetDevice(st, &dev) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipStreamDestroy(st);
+        return;
+    }
+    StreamPool& p = stream_pool();
+    {
+        std::lock_guard<std::mutex> lk(p.m);
+        if (p.s.size() < STREAM_POOL_MAX) {
+            p.s.emplace_back(dev, st);
+            return;
+        }
+    }
+    (void)hipStreamDestroy(st);
+}
+
 // host memory the runtime knows as page-locked (hipHostMalloc'd or registered)
 bool page_locked(const void* p)
 {
@@ -2974,6 +3052,7 @@ struct cm_job {
     DevBuf cnt, acc, hits, stats;
     double pending_w = 0.0;
     bool pending = false;
+    int64_t flushes = 0;   // weight runs already added to the FP64 accumulators
     double volume_total = 0.0;
     DevBuf gmap;   // group maps
     // batch buffers
@@ -3491,18 +3570,31 @@ int alloc_batch(cm_job* J, int F, int64_t cap, int64_t kcap, int dtype_size, boo
     return CM_OK;
 }
 
-int flush(cm_job* J, double w, hipStream_t stream = nullptr)
+// the self-MDDF half range of the accumulators: both group hits of a record
+// count w/2 (update_counters.jl:48-53)
+void half_range(const cm_job* J, int64_t* lo, int64_t* hi)
+{
+    *lo = *hi = 0;
+    if (J->d.autocorrelation) {
+        *lo = 4 * (int64_t)J->d.nbins;
+        *hi = J->acc_len;
+    }
+}
+
+// assign: the hits of the job's first weight run go into accumulators that
+// may hold anything (the packed read-back reuses their storage); with
+// count_hits == false the md/rdf hits are not added to the cumulative ones
+// (they already were)
+int flush(cm_job* J, double w, hipStream_t stream = nullptr, bool assign = false, bool count_hits = true)
 {
     int64_t nb = J->d.nbins;
-    int64_t half_lo = 0, half_hi = 0;
-    if (J->d.autocorrelation) {   // self: both group hits of a record are w/2 (update_counters.jl:48-53)
-        half_lo = 4 * nb;
-        half_hi = J->acc_len;
-    }
+    int64_t half_lo, half_hi;
+    half_range(J, &half_lo, &half_hi);
     hipLaunchKernelGGL(k_flush, dim3(grid_for(J->acc_len)), dim3(BLOCK), 0, stream ? stream : J->stream, (ull*)J->cnt.p,
                        (double*)J->acc.p,
-                       (ull*)J->hits.p, J->acc_len, 4 * nb, half_lo, half_hi, w);
+                       (ull*)J->hits.p, J->acc_len, count_hits ? 4 * nb : 0, half_lo, half_hi, w, assign ? 1 : 0);
     HIPCHK(hipGetLastError());
+    J->flushes++;
     return CM_OK;
 }
 
@@ -3650,6 +3742,7 @@ void job_params(cm_job* J, const cm_job_desc& d)
     J->Nv = Nv;
     J->pending = false;
     J->pending_w = 0.0;
+    J->flushes = 0;
     J->volume_total = 0.0;
     J->finished = false;
     J->profiling = false;
@@ -3784,50 +3877,30 @@ int geometry_error(uint64_t gerr)
     return CM_OK;
 }
 
-// Read-back of cm_job_finish, pipelined: the accumulators come back in
-// RB_PIECES pieces, each with an event, and worker threads copy 2 MB tasks
-// into the caller's arrays as soon as the piece holding a task has landed, so
-// the copies into (often freshly allocated, page-faulting) caller memory
-// overlap the transfer.  The geometry error bits come back first.
-int job_finish_pipelined(cm_job* J, cm_raw* out)
+// Copies the read-back in h into the caller's arrays: FP64 accumulators, or
+// 32-bit hits (packed) turned into (double)hits * w (w/2 in the self-MDDF
+// group range), the product k_flush forms.  Worker threads take tasks of
+// 256K entries; with ev, each first waits for the piece holding its task.
+std::atomic<uint64_t>& pack_limit()
 {
-    if (J->pending) {
-        int r = flush(J, J->pending_w);
-        if (r) return r;
-        J->pending = false;
-    }
+    static std::atomic<uint64_t> v{0xffffffffull};
+    return v;
+}
+
+int finish_copy(cm_job* J, cm_raw* out, const char* h, bool packed, double w, size_t piece, const hipEvent_t* ev)
+{
     const cm_job_desc& d = J->d;
     const int64_t nb = d.nbins, ngu = d.n_groups_u, ngv = d.n_groups_v;
-    const size_t acc_bytes = 8 * (size_t)J->acc_len;
-    const size_t off_hits = acc_bytes, off_err = acc_bytes + 8 * 4 * (size_t)nb;
-    const size_t need = off_err + 8;
-    if (J->hres.bytes < need) {
-        pin_put(J->hres);
-        int r = pin_take(J->hres, need);
-        if (r) return r;
-    }
-    for (auto& e : J->rb_ev)
-        if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    char* h = (char*)J->hres.p;
-    hipStream_t s = J->stream;
-    HIPCHK(hipMemcpyAsync(h + off_err, (char*)J->stats.p + 16, 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(h + off_hits, J->hits.p, 8 * 4 * nb, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipEventRecord(J->rb_ev[RB_PIECES], s));
-    const size_t piece = ((acc_bytes + RB_PIECES - 1) / RB_PIECES + 4095) & ~(size_t)4095;
-    for (int k = 0; k < RB_PIECES; k++) {
-        const size_t o = std::min(acc_bytes, k * piece), n = std::min(acc_bytes, o + piece) - o;
-        if (n) HIPCHK(hipMemcpyAsync(h + o, (char*)J->acc.p + o, n, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipEventRecord(J->rb_ev[k], s));
-    }
-    HIPCHK(hipEventSynchronize(J->rb_ev[RB_PIECES]));
-    uint64_t gerr;
-    memcpy(&gerr, h + off_err, 8);
-    if (int r = geometry_error(gerr)) return r;
-    // the caller's arrays as byte ranges of the accumulator block, in order
-    struct Seg { size_t o, n; char* dst; };
+    const int64_t n = J->acc_len;
+    const size_t es = packed ? 4 : 8;
+    int64_t half_lo, half_hi;
+    half_range(J, &half_lo, &half_hi);
+    const double wh = w / 2;
+    // the caller's arrays as entry ranges of the accumulator block, in order
+    struct Seg { int64_t o, n; double* dst; };
     std::vector<Seg> segs;
-    size_t o = 0;
-    auto add = [&](double* dst, int64_t n) { if (dst && n) segs.push_back({o, 8 * (size_t)n, (char*)dst}); o += 8 * (size_t)n; };
+    int64_t o = 0;
+    auto add = [&](double* dst, int64_t m) { if (dst && m) segs.push_back({o, m, dst}); o += m; };
     add(out->md_count, nb);
     add(out->md_count_random, nb);
     add(out->rdf_count, nb);
@@ -3836,35 +3909,44 @@ int job_finish_pipelined(cm_job* J, cm_raw* out)
     add(out->solvent_group_count, ngv * nb);
     add(out->solute_group_count_random, ngu * nb);
     add(out->solvent_group_count_random, ngv * nb);
-    constexpr size_t TASK = (size_t)2 << 20;
-    const size_t ntask = (acc_bytes + TASK - 1) / TASK;
+    constexpr int64_t TASK = (int64_t)1 << 18;
+    const int64_t ntask = (n + TASK - 1) / TASK;
     hipError_t werr = hipSuccess;
     std::mutex werr_mu;
-    auto work = [&](size_t t0, size_t step) {
-        for (size_t t = t0; t < ntask; t += step) {
-            const size_t a = t * TASK, b = std::min(acc_bytes, a + TASK);
-            // pieces land in stream order: the piece holding the task's last byte
-            const hipError_t e = hipEventSynchronize(J->rb_ev[(b - 1) / piece]);
-            if (e != hipSuccess) {
-                std::lock_guard<std::mutex> lk(werr_mu);
-                werr = e;
-                return;
+    auto work = [&](int64_t t0, int64_t step) {
+        for (int64_t t = t0; t < ntask; t += step) {
+            const int64_t a = t * TASK, b = std::min(n, a + TASK);
+            if (ev) {
+                // pieces land in stream order: the piece holding the task's last byte
+                const hipError_t e = hipEventSynchronize(ev[(es * (size_t)b - 1) / piece]);
+                if (e != hipSuccess) {
+                    std::lock_guard<std::mutex> lk(werr_mu);
+                    werr = e;
+                    return;
+                }
             }
             for (const Seg& sg : segs) {
-                const size_t lo = std::max(a, sg.o), hi = std::min(b, sg.o + sg.n);
-                if (lo < hi) memcpy(sg.dst + (lo - sg.o), h + lo, hi - lo);
+                const int64_t lo = std::max(a, sg.o), hi = std::min(b, sg.o + sg.n);
+                if (lo >= hi) continue;
+                double* dst = sg.dst + (lo - sg.o);
+                if (!packed) {
+                    memcpy(dst, h + 8 * lo, 8 * (size_t)(hi - lo));
+                    continue;
+                }
+                const uint32_t* src = (const uint32_t*)h;
+                for (int64_t k = lo; k < hi; k++) dst[k - lo] = (double)src[k] * ((k >= half_lo && k < half_hi) ? wh : w);
             }
         }
     };
-    const size_t nt = std::min<size_t>(8, ntask);
+    const int64_t nt = std::min<int64_t>(8, ntask);
     std::vector<std::thread> th;
-    for (size_t w = 1; w < nt; w++) th.emplace_back(work, w, nt);
-    if (nt > 0) work(0, std::max<size_t>(nt, 1));
+    for (int64_t q = 1; q < nt; q++) th.emplace_back(work, q, nt);
+    if (nt > 0) work(0, std::max<int64_t>(nt, 1));
     for (auto& x : th) x.join();
     if (werr != hipSuccess) return fail(CM_EHIP, std::string("read-back: ") + hipGetErrorString(werr));
-    HIPCHK(hipEventSynchronize(J->rb_ev[RB_PIECES - 1]));
+    if (ev) HIPCHK(hipEventSynchronize(ev[RB_PIECES - 1]));
     out->volume_total = J->volume_total;
-    const ull* hits = (const ull*)(h + off_hits);
+    const ull* hits = (const ull*)(h + 8 * (size_t)n);
     auto cpu = [&](uint64_t* dst, const ull* src) { if (dst) memcpy(dst, src, 8 * nb); };
     cpu(out->hits_md, hits);
     cpu(out->hits_md_random, hits + nb);
@@ -3872,6 +3954,73 @@ int job_finish_pipelined(cm_job* J, cm_raw* out)
     cpu(out->hits_rdf_random, hits + 3 * nb);
     J->finished = true;
     return CM_OK;
+}
+
+// Read-back of cm_job_finish, pipelined: the accumulators come back in
+// RB_PIECES pieces, each with an event, and worker threads copy 1M-entry tasks
+// into the caller's arrays as soon as the piece holding a task has landed, so
+// the copies into (often freshly allocated, page-faulting) caller memory
+// overlap the transfer.  The geometry error bits come back first.
+//
+// When every frame of the job had one weight w (no weight change flushed
+// yet), each FP64 result is (double)hits * w (w/2 in the self-MDDF group
+// range) with nothing added before it, so the hits cross PCIe as 32-bit words
+// (half the bytes of the accumulators) and the host forms the same products,
+// rounded the same way.  A count that needs more than 32 bits sends the job
+// down the FP64 path.
+int job_finish_pipelined(cm_job* J, cm_raw* out)
+{
+    const cm_job_desc& d = J->d;
+    const int64_t nb = d.nbins, ngu = d.n_groups_u, ngv = d.n_groups_v;
+    const int64_t n = J->acc_len;
+    hipStream_t s = J->stream;
+    const bool packed = J->pending && J->flushes == 0;
+    const double w = J->pending_w;
+    if (packed) {
+        hipLaunchKernelGGL(k_pack32, dim3(grid_for(n)), dim3(BLOCK), 0, s, (const ull*)J->cnt.p, (uint32_t*)J->acc.p,
+                           (ull*)J->hits.p, n, 4 * nb, (ull)pack_limit().load(), (ull*)J->stats.p + 3);
+        HIPCHK(hipGetLastError());
+    } else if (J->pending) {
+        if (int r = flush(J, J->pending_w)) return r;
+    }
+    J->pending = false;
+    const size_t acc_bytes = 8 * (size_t)n;
+    const size_t off_hits = acc_bytes, off_err = acc_bytes + 8 * 4 * (size_t)nb;
+    const size_t need = off_err + 16;
+    if (J->hres.bytes < need) {
+        pin_put(J->hres);
+        int r = pin_take(J->hres, need);
+        if (r) return r;
+    }
+    for (auto& e : J->rb_ev)
+        if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    char* h = (char*)J->hres.p;
+    // error bits and the packing overflow flag, then the md/rdf hits
+    HIPCHK(hipMemcpyAsync(h + off_err, (char*)J->stats.p + 16, 16, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(h + off_hits, J->hits.p, 8 * 4 * nb, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(J->rb_ev[RB_PIECES], s));
+    const size_t es = packed ? 4 : 8;   // bytes per entry on the wire
+    const size_t wire = es * (size_t)n;
+    const size_t piece = ((wire + RB_PIECES - 1) / RB_PIECES + 4095) & ~(size_t)4095;
+    for (int k = 0; k < RB_PIECES; k++) {
+        const size_t o = std::min(wire, k * piece), m = std::min(wire, o + piece) - o;
+        if (m) HIPCHK(hipMemcpyAsync(h + o, (char*)J->acc.p + o, m, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipEventRecord(J->rb_ev[k], s));
+    }
+    HIPCHK(hipEventSynchronize(J->rb_ev[RB_PIECES]));
+    uint64_t flags[2];
+    memcpy(flags, h + off_err, 16);
+    if (int r = geometry_error(flags[0])) return r;
+    if (packed && flags[1]) {
+        // a count beyond 32 bits: the FP64 accumulators after all (the hits
+        // were already added to the cumulative md/rdf ones)
+        HIPCHK(hipStreamSynchronize(s));
+        if (int r = flush(J, w, nullptr, true, false)) return r;
+        HIPCHK(hipMemcpyAsync(h, J->acc.p, acc_bytes, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return finish_copy(J, out, h, false, w, piece, nullptr);
+    }
+    return finish_copy(J, out, h, packed, w, piece, J->rb_ev);
 }
 
 // flushes the pending weight and reads the accumulators and hits back into J->hres
@@ -3984,9 +4133,9 @@ int cm_job_create(const cm_job_desc* desc, cm_job** out)
     int r = validate_desc(*desc);
     if (r) return r;
     cm_job* J = new cm_job();
-    if (hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking) != hipSuccess) {
+    if ((r = stream_take(&J->stream))) {
         cm_job_destroy(J);
-        return fail(CM_EHIP, "hipStreamCreate failed");
+        return r;
     }
     if ((r = job_setup(J, *desc))) {
         cm_job_destroy(J);
@@ -4130,8 +4279,15 @@ void cm_job_destroy(cm_job* J)
         for (auto& e : a) hipEventDestroy(e);
     for (auto& e : J->rb_ev)
         if (e) hipEventDestroy(e);
-    if (J->stream) hipStreamDestroy(J->stream);
+    stream_put(J->stream);
     delete J;
+}
+
+int cm_set_readback_pack_limit(uint64_t limit)
+{
+    if (limit > 0xffffffffull) return fail(CM_EINVAL, "the pack limit is at most 2^32 - 1");
+    pack_limit().store(limit);
+    return CM_OK;
 }
 
 int cm_release_cached_memory(void)

@@ -52,9 +52,9 @@ def job_kwargs(R, options, tmeta, coordination_number_only):
     solute, solvent = R.solute, R.solvent
     nu = solute.natomspermol
     Nv = solvent.nmols * solvent.natomspermol
-    groups_u = group_csr(solute, list(range(nu)), solute)
+    groups_u = group_csr(solute, range(nu), solute)
     # self: the solvent-side atom j counts into the solute groups (update_counters.jl:48-53)
-    groups_v = group_csr(solvent, list(range(Nv)), solute if R.autocorrelation else solvent)
+    groups_v = group_csr(solvent, range(Nv), solute if R.autocorrelation else solvent)
     seed = options.seed if options.seed > 0 else secrets.randbits(63) + 1
     return dict(nmols_u=solute.nmols, natoms_per_mol_u=nu, nmols_v=solvent.nmols,
                 natoms_per_mol_v=solvent.natomspermol, autocorrelation=R.autocorrelation,

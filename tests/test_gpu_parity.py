@@ -596,3 +596,39 @@ def test_cached_memory_reuse_and_release(pkg):
     c = gpu_raw(pkg, traj, opts)
     assert_raw_equal(a, b)
     assert_raw_equal(a, c)
+
+
+@pytest.mark.parametrize("self_mddf", [False, True])
+def test_packed_and_fp64_readback_agree(pkg, oracle, self_mddf):
+    """cm_job_finish sends the hits of a job whose frames share one weight as
+    32-bit words and forms (double)hits * w on the host (w/2 in the self-MDDF
+    group range); a pack limit of 0 sends the same job down the FP64 path
+    after the packing (the md/rdf hits added once).  Both read-backs agree bit
+    for bit, and with the oracle (exactly for a dyadic weight, to 1e-13 for
+    0.3, whose per-record sums round differently from the product)."""
+    from complexmixtures_amd import _lib
+    S = syn()
+    if self_mddf:
+        s = S.config3(nframes=2, n_solute=10, n_urea=600, n_water=10, solvent="self", seed=4, a=36.0)
+        opts = pkg.Options(dbulk=9.0, n_random_samples=3, seed=10, silent=True)
+    else:
+        s = S.config1(nframes=3, n_solute=300, n_water=900, box=30.0, seed=5)
+        opts = pkg.Options(dbulk=8.0, n_random_samples=6, seed=77, silent=True)
+    traj = mem(pkg, s)
+    nf = s["x_solute"].shape[0]
+    for w in (0.75, 0.3):
+        fw = np.full(nf, w)
+        packed = gpu_raw(pkg, traj, opts, frame_weights=fw)
+        _lib.set_readback_pack_limit(0)
+        try:
+            wide = gpu_raw(pkg, traj, opts, frame_weights=fw)
+        finally:
+            _lib.set_readback_pack_limit()
+        assert_raw_equal(packed, wide)
+        for k in ("hits_md", "hits_md_random", "hits_rdf", "hits_rdf_random"):
+            assert np.array_equal(packed[k], wide[k]), k
+        a, _ = oracle_raw(pkg, oracle, traj, opts, frame_weights=fw)
+        assert_raw_equal(a, packed, rtol=0.0 if w == 0.75 else 1e-13)
+        if w == 0.75:
+            assert_group_split(packed, self_mddf)
+        assert packed["md_count"].sum() > 0
