@@ -4023,30 +4023,43 @@ int job_finish_pipelined(cm_job* J, cm_raw* out)
     return finish_copy(J, out, h, packed, w, piece, J->rb_ev);
 }
 
-// flushes the pending weight and reads the accumulators and hits back into J->hres
-int job_collect(cm_job* J)
+// Reads a finished request's results back into J->hres: the accumulators
+// (or, when its frames shared one weight, the 32-bit hits: *packed set, as in
+// cm_job_finish), the md/rdf hits, then the geometry error bits.
+int job_collect(cm_job* J, bool* packed)
 {
-    if (J->pending) {
-        int r = flush(J, J->pending_w);
-        if (r) return r;
-        J->pending = false;
+    const int64_t n = J->acc_len, nb = J->d.nbins;
+    *packed = J->pending && J->flushes == 0;
+    if (*packed) {
+        hipLaunchKernelGGL(k_pack32, dim3(grid_for(n)), dim3(BLOCK), 0, J->stream, (const ull*)J->cnt.p,
+                           (uint32_t*)J->acc.p, (ull*)J->hits.p, n, 4 * nb, (ull)pack_limit().load(),
+                           (ull*)J->stats.p + 3);
+        HIPCHK(hipGetLastError());
+    } else if (J->pending) {
+        if (int r = flush(J, J->pending_w)) return r;
     }
-    // accumulators, hits, then the geometry error bits
-    const size_t off_err = 8 * (size_t)J->acc_len + 8 * 4 * (size_t)J->d.nbins;
-    const size_t need = off_err + 8;
+    J->pending = false;
+    const size_t off_err = 8 * (size_t)n + 8 * 4 * (size_t)nb;
+    const size_t need = off_err + 16;
     if (J->hres.bytes < need) {
         pin_put(J->hres);
         int r = pin_take(J->hres, need);
         if (r) return r;
     }
-    HIPCHK(hipMemcpyAsync(J->hres.p, J->acc.p, 8 * J->acc_len, hipMemcpyDeviceToHost, J->stream));
-    HIPCHK(hipMemcpyAsync((char*)J->hres.p + 8 * J->acc_len, J->hits.p, 8 * 4 * J->d.nbins, hipMemcpyDeviceToHost,
-                          J->stream));
-    HIPCHK(hipMemcpyAsync((char*)J->hres.p + off_err, (char*)J->stats.p + 16, 8, hipMemcpyDeviceToHost, J->stream));
+    HIPCHK(hipMemcpyAsync(J->hres.p, J->acc.p, (*packed ? 4 : 8) * (size_t)n, hipMemcpyDeviceToHost, J->stream));
+    HIPCHK(hipMemcpyAsync((char*)J->hres.p + 8 * n, J->hits.p, 8 * 4 * nb, hipMemcpyDeviceToHost, J->stream));
+    HIPCHK(hipMemcpyAsync((char*)J->hres.p + off_err, (char*)J->stats.p + 16, 16, hipMemcpyDeviceToHost, J->stream));
     HIPCHK(hipStreamSynchronize(J->stream));
-    uint64_t gerr;
-    memcpy(&gerr, (const char*)J->hres.p + off_err, 8);
-    if (int r = geometry_error(gerr)) return r;
+    uint64_t flags[2];
+    memcpy(flags, (const char*)J->hres.p + off_err, 16);
+    if (int r = geometry_error(flags[0])) return r;
+    if (*packed && flags[1]) {
+        // a count beyond 32 bits: FP64 accumulators (md/rdf hits already added)
+        if (int r = flush(J, J->pending_w, nullptr, true, false)) return r;
+        HIPCHK(hipMemcpyAsync(J->hres.p, J->acc.p, 8 * (size_t)n, hipMemcpyDeviceToHost, J->stream));
+        HIPCHK(hipStreamSynchronize(J->stream));
+        *packed = false;
+    }
     J->finished = true;
     return CM_OK;
 }
@@ -4074,17 +4087,49 @@ void copy_par(void* dst, const void* src, size_t bytes)
     for (auto& x : th) x.join();
 }
 
-void fill_raw(const cm_job_desc& d, int64_t acc_len, const PinBuf& res, double volume, cm_raw* out)
+// dst[k] = (double)src[o + k] * w (w/2 for entries o + k in [half_lo,
+// half_hi)), split over a few threads like copy_par
+void conv_par(double* dst, const uint32_t* src, int64_t o, int64_t n, int64_t half_lo, int64_t half_hi, double w)
+{
+    const double wh = w / 2;
+    auto run = [=](int64_t a, int64_t b) {
+        for (int64_t k = a; k < b; k++) dst[k] = (double)src[o + k] * ((o + k >= half_lo && o + k < half_hi) ? wh : w);
+    };
+    constexpr int64_t PIECE = (int64_t)1 << 19;
+    const int nt = (int)std::min<int64_t>(8, n / PIECE);
+    if (nt < 2) {
+        run(0, n);
+        return;
+    }
+    const int64_t step = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) {
+        const int64_t a = t * step;
+        if (a >= n) break;
+        th.emplace_back(run, a, std::min(n, a + step));
+    }
+    run(0, std::min(n, step));
+    for (auto& x : th) x.join();
+}
+
+// raw accumulators of a collected request into the caller's cm_raw: FP64
+// copies, or the products (double)hits * w of a packed read-back
+void fill_raw(const cm_job_desc& d, int64_t acc_len, const PinBuf& res, double volume, bool packed, double w,
+              cm_raw* out)
 {
     const int64_t nb = d.nbins, ngu = d.n_groups_u, ngv = d.n_groups_v;
-    const double* a = (const double*)res.p;
+    const int64_t half_lo = d.autocorrelation ? 4 * nb : 0, half_hi = d.autocorrelation ? acc_len : 0;
     const ull* hits = (const ull*)((const char*)res.p + 8 * acc_len);
-    auto cp = [&](double* dst, const double* src, int64_t n) { if (dst) copy_par(dst, src, 8 * (size_t)n); };
-    cp(out->md_count, a, nb);
-    cp(out->md_count_random, a + nb, nb);
-    cp(out->rdf_count, a + 2 * nb, nb);
-    cp(out->rdf_count_random, a + 3 * nb, nb);
-    const double* g = a + 4 * nb;
+    auto cp = [&](double* dst, int64_t o, int64_t n) {
+        if (!dst) return;
+        if (packed) conv_par(dst, (const uint32_t*)res.p, o, n, half_lo, half_hi, w);
+        else copy_par(dst, (const double*)res.p + o, 8 * (size_t)n);
+    };
+    cp(out->md_count, 0, nb);
+    cp(out->md_count_random, nb, nb);
+    cp(out->rdf_count, 2 * nb, nb);
+    cp(out->rdf_count_random, 3 * nb, nb);
+    const int64_t g = 4 * nb;
     cp(out->solute_group_count, g, ngu * nb);
     cp(out->solvent_group_count, g + ngu * nb, ngv * nb);
     cp(out->solute_group_count_random, g + (ngu + ngv) * nb, ngu * nb);
@@ -4267,7 +4312,7 @@ int cm_job_reference_draws(cm_job* J, int64_t frame, int32_t* out)
 void cm_job_destroy(cm_job* J)
 {
     if (!J) return;
-    if (J->stream) hipStreamSynchronize(J->stream);
+    if (J->stream) (void)hipStreamSynchronize(J->stream);
     DevBuf* bufs[] = {&J->cnt, &J->acc, &J->hits, &J->stats, &J->gmap, &J->geo, &J->cells_cnt, &J->cells_start,
                       &J->tmp4, &J->tmpcell, &J->sorted, &J->solw, &J->ucen, &J->vw, &J->vcell, &J->vcen, &J->P,
                       &J->vw_r, &J->vcen_r, &J->bflag, &J->blist, &J->nbulk, &J->uk, &J->drawn_ptr,
@@ -4276,9 +4321,9 @@ void cm_job_destroy(cm_job* J)
     for (DevBuf* b : bufs) dev_put(*b);
     pin_put(J->hres);
     for (auto& a : J->prof_ev)
-        for (auto& e : a) hipEventDestroy(e);
+        for (auto& e : a) (void)hipEventDestroy(e);
     for (auto& e : J->rb_ev)
-        if (e) hipEventDestroy(e);
+        if (e) (void)hipEventDestroy(e);
     stream_put(J->stream);
     delete J;
 }
@@ -4349,6 +4394,8 @@ struct SrvReq {
     std::string err;
     PinBuf res;                     // read-back accumulators (taken over from the slot's job)
     int64_t acc_len = 0;
+    bool packed = false;            // res holds 32-bit hits of frames of weight wt
+    double wt = 0.0;
     double volume = 0.0;
     cm_request_times t{};
     ~SrvReq() { pin_put(res); }
@@ -4402,7 +4449,11 @@ int run_request(cm_job* J, SrvReq* q)
         if ((r = cm_job_set_tuning(J, q->cf > 0.0 ? q->cf : 0.5, q->cols, 0))) return r;
     if ((r = submit_common(J, q->nframes, q->fi.data(), q->xu, q->xv, q->dtype, q->cells.data(), q->w.data(), false)))
         return r;
-    if ((r = job_collect(J))) return r;
+    const double w = J->pending_w;
+    bool packed = false;
+    if ((r = job_collect(J, &packed))) return r;
+    q->packed = packed;
+    q->wt = w;
     q->res = J->hres;
     J->hres = PinBuf{};
     q->acc_len = J->acc_len;
@@ -4412,7 +4463,8 @@ int run_request(cm_job* J, SrvReq* q)
 
 void serve_slot(cm_server* S, int k)
 {
-    hipSetDevice(S->device);
+    // the device current when the server was created (hipGetDevice returned it)
+    (void)hipSetDevice(S->device);
     cm_job* J = S->shells[k];
     for (;;) {
         SrvReq* q;
@@ -4562,7 +4614,7 @@ int cm_server_wait(cm_server* S, uint64_t ticket, cm_raw* out, cm_request_times*
     if (times) *times = q->t;
     const int st = q->status;
     if (st == CM_OK) {
-        if (out) fill_raw(q->d, q->acc_len, q->res, q->volume, out);
+        if (out) fill_raw(q->d, q->acc_len, q->res, q->volume, q->packed, q->wt, out);
     } else {
         g_err = q->err;
     }

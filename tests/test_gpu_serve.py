@@ -37,6 +37,24 @@ def test_served_equals_plain(pkg, pool):
     assert served["digests"] == plain["digests"]
 
 
+def test_served_fp64_readback_equals_packed(pkg, pool):
+    """A request whose frames share one weight comes back as 32-bit hits;
+    with a pack limit of 0 every request falls back to the FP64 accumulators.
+    The counts are the same."""
+    from complexmixtures_amd import _lib
+    S = _serve()
+    reqs = _requests(S, 6, pool, 5)
+    packed = S.run_served(pool, reqs, slots=2, keep_raw=True)
+    _lib.set_readback_pack_limit(0)
+    try:
+        wide = S.run_served(pool, reqs, slots=2, keep_raw=True)
+    finally:
+        _lib.set_readback_pack_limit()
+    for a, b in zip(packed["raw"], wide["raw"]):
+        assert_raw_equal(a, b)
+    assert packed["digests"] == wide["digests"]
+
+
 def test_served_matches_oracle(pkg, oracle, pool):
     """One cross and one self request of the mix against the FP64 oracle."""
     S = _serve()
