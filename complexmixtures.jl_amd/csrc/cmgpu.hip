@@ -2595,14 +2595,18 @@ __device__ __forceinline__ void multi_blocks(const MultiTab* __restrict__ MT, Fn
     const int n = MT->nsys;
     for (int k = threadIdx.x; k <= n; k += blockDim.x) base[k] = MT->base[ST][k];
     __syncthreads();
-    const int total = base[n];
+    // values read from LDS are the same in every lane; made uniform, the
+    // job's parameters below are scalar loads, as kernel arguments would be
+    const int total = __builtin_amdgcn_readfirstlane(base[n]);
     int lo = 0;
     for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
         while (base[lo + 1] <= vb) lo++;
+        lo = __builtin_amdgcn_readfirstlane(lo);
+        const int b0 = __builtin_amdgcn_readfirstlane(base[lo]), b1 = __builtin_amdgcn_readfirstlane(base[lo + 1]);
         const SysDev& S = MT->sys[lo];
-        const int local = vb - base[lo];
+        const int local = vb - b0;
         const int nbx = S.nbx[ST];
-        const int nby = (base[lo + 1] - base[lo]) / nbx;
+        const int nby = (b1 - b0) / nbx;
         fn(S, local - (local / nbx) * nbx, nbx, local / nbx, nby);
         if (SYNC) __syncthreads();
     }
@@ -2761,12 +2765,14 @@ __global__ void __launch_bounds__(64, 3) km_eval_tiles(const MultiTab* __restric
         cb[n] = acc;
     }
     __syncthreads();
-    const int total = cb[n];
+    const int total = __builtin_amdgcn_readfirstlane(cb[n]);
     int lo = 0;
     for (int g = blockIdx.x; g < total; g += gridDim.x) {
         while (cb[lo + 1] <= g) lo++;
+        // uniform (the same in every lane): the job's parameters are scalar loads
+        lo = __builtin_amdgcn_readfirstlane(lo);
         const SysDev& S = MT->sys[lo];
-        const int ch = g - cb[lo];
+        const int ch = g - __builtin_amdgcn_readfirstlane(cb[lo]);
         float4* stg = dyn;
         float* stw = (float*)(dyn + S.P.tcap);
         eval_tiles_chunks<TU, TV, RANDOM, C, NVM, false>(S.P, S.B, (const TU*)S.xu, (const TV*)S.xv, ch, ch + 1, 1, stg,
