@@ -159,6 +159,8 @@ struct Bufs {
     int* drawn_idx;          // [F][K]
     // tiles family: near cells and molecules sorted by cell per (frame, pass)
     unsigned char* near;     // [F][cap]
+    unsigned* dfield;        // [F][cap] 0xffffffff - bits of the distance from a near cell's centre to the
+                             // nearest solute atom image of its own and face-neighbour cells (0: none)
     int* mkey;               // [F][NP][Mv] Morton key of the centre's cell, -1 when far
     int* mcnt;               // [F][NP][kcap+1]
     int* mstart;             // [F][NP][kcap+1]
@@ -1714,6 +1716,95 @@ __global__ void k_near_mark(KP P, Bufs B)
     b_near_mark(P, B, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
 }
 
+// Every near cell: the distance from its centre q to the nearest image of a
+// solute atom in the cell itself or one of its six face neighbours, encoded
+// as 0xffffffff - bits (0, the cleared value, when they hold none).  It is
+// the distance to one particular atom image, so for any point p
+// min_a |p - a| <= |p - q| + that distance: each molecule of the cell gets
+// from it an upper bound on its atoms' nearest solute distance
+// (molecule_bound).  Eight lanes per cell, lane j < 7 scanning one cell.
+__device__ __forceinline__ void b_dfield(KP P, Bufs B, int bx_, int nbx_, int by_, int nby_)
+{
+    const int64_t total = (int64_t)P.F * P.cap;
+    const int j = threadIdx.x & 7;
+    const int64_t step = ((int64_t)nbx_ * blockDim.x) >> 3;
+    // the eight lanes of a group share it (blockDim is a multiple of 8)
+    for (int64_t it = (bx_ * (int64_t)blockDim.x + threadIdx.x) >> 3; it < total; it += step) {
+        const bool nearc = B.near[it] != 0;
+        float best = INFINITY;
+        if (nearc && j < 7) {
+            const int f = (int)(it / P.cap);
+            const int cell = (int)(it - (int64_t)f * P.cap);
+            const Geo& g = B.geo[f];
+            const int cx = cell % g.nc[0], cy = (cell / g.nc[0]) % g.nc[1], cz = cell / (g.nc[0] * g.nc[1]);
+            const int ax = j == 1 ? 1 : (j == 2 ? -1 : 0), ay = j == 3 ? 1 : (j == 4 ? -1 : 0),
+                      az = j == 5 ? 1 : (j == 6 ? -1 : 0);
+            int mx, my, mz;
+            const int ix = wrap_index(cx + ax, g.nc[0], mx), iy = wrap_index(cy + ay, g.nc[1], my),
+                      iz = wrap_index(cz + az, g.nc[2], mz);
+            float tx, ty, tz;
+            shift_of(g, mx, my, mz, tx, ty, tz);
+            const float fx = (cx + 0.5f) / g.nc[0], fy = (cy + 0.5f) / g.nc[1], fz = (cz + 0.5f) / g.nc[2];
+            // q minus the image shift, against the wrapped atoms
+            const float rx = g.H32[0] * fx + g.H32[3] * fy + g.H32[6] * fz - tx;
+            const float ry = g.H32[1] * fx + g.H32[4] * fy + g.H32[7] * fz - ty;
+            const float rz = g.H32[2] * fx + g.H32[5] * fy + g.H32[8] * fz - tz;
+            const int* st = B.start + (int64_t)f * (P.cap + 1);
+            const float4* srt = B.sorted + (int64_t)f * B.nitems;
+            const int c2 = (iz * g.nc[1] + iy) * g.nc[0] + ix;
+            const int k0 = st[c2], k1 = st[c2 + 1];
+            int k = k0;
+            for (; k + 3 < k1; k += 4) {
+                float4 a[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) a[u] = srt[k + u];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const float ex = a[u].x - rx, ey = a[u].y - ry, ez = a[u].z - rz;
+                    best = fminf(best, ex * ex + ey * ey + ez * ez);
+                }
+            }
+            for (; k < k1; k++) {
+                const float4 a = srt[k];
+                const float ex = a.x - rx, ey = a.y - ry, ez = a.z - rz;
+                best = fminf(best, ex * ex + ey * ey + ez * ez);
+            }
+        }
+        best = fminf(best, __shfl_xor(best, 1, 64));
+        best = fminf(best, __shfl_xor(best, 2, 64));
+        best = fminf(best, __shfl_xor(best, 4, 64));
+        if (nearc && j == 0 && best < INFINITY) B.dfield[it] = 0xffffffffu - __float_as_uint(sqrtf(best));
+    }
+}
+
+__global__ void k_dfield(KP P, Bufs B)
+{
+    b_dfield(P, B, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+}
+
+// Upper bound (FP32, rounded outward by a wide margin) on the distance from
+// any atom of a molecule whose centre has fractional coordinates fr (in the
+// near cell `cell`) to its nearest solute atom: dfield of the cell, plus the
+// distance from the centre to the cell centre, plus the molecule radius.
+// +inf when the cell found no solute atom.
+// FP32 throughout: every rounding is far below the 1e-5 relative + 0.01 A
+// margin.
+// e: the cell's dfield entry, read by the caller before its stores (they may
+// alias it as far as the compiler knows, which would hold the load back)
+__device__ __forceinline__ float molecule_bound(const Geo& g, unsigned e, int cell, const double fr[3])
+{
+    if (e == 0u) return INFINITY;
+    const float D = __uint_as_float(0xffffffffu - e);
+    const int c0 = cell % g.nc[0], c1 = (cell / g.nc[0]) % g.nc[1], c2 = cell / (g.nc[0] * g.nc[1]);
+    const float d0 = (float)fr[0] - (c0 + 0.5f) / g.nc[0], d1 = (float)fr[1] - (c1 + 0.5f) / g.nc[1],
+                d2 = (float)fr[2] - (c2 + 0.5f) / g.nc[2];
+    const float* H = g.H32;
+    const float ex = H[0] * d0 + H[3] * d1 + H[6] * d2;
+    const float ey = H[1] * d0 + H[4] * d1 + H[7] * d2;
+    const float ez = H[2] * d0 + H[5] * d1 + H[8] * d2;
+    return (D + sqrtf(ex * ex + ey * ey + ez * ez) + __uint_as_float(g.rv_bits)) * (1.0f + 1e-5f) + 1e-2f;
+}
+
 // real pass: centre of every solvent molecule (made whole around its first
 // atom) wrapped into the cell; near molecules get the Morton key of the cell
 template <class T>
@@ -1769,12 +1860,16 @@ __device__ __forceinline__ void b_fill_real(KP P, Bufs B, const T* xv, int bx_, 
         double tz = H[2] * n[0] + H[5] * n[1] + H[8] * n[2];
         double x0, y0, z0;
         load3(x, a0, x0, y0, z0);
+        // the first atom's w carries the molecule's nearest-solute bound
+        int cell;
+        cell_of_frac(g, fr[0], fr[1], fr[2], cell);
+        const float ub = molecule_bound(g, B.dfield[(int64_t)f * P.cap + cell], cell, fr);
         float4* dst = B.vw + (int64_t)f * P.Nv + (int64_t)pos * P.nv;
         for (int a = 0; a < P.nv; a++) {
             double xa, ya, za, ox, oy, oz;
             load3(x, a0 + a, xa, ya, za);
             cm::minimage(g.cell, xa - x0, ya - y0, za - z0, &ox, &oy, &oz);
-            dst[a] = mk4((float)(x0 + ox - tx), (float)(y0 + oy - ty), (float)(z0 + oz - tz), 0.f);
+            dst[a] = mk4((float)(x0 + ox - tx), (float)(y0 + oy - ty), (float)(z0 + oz - tz), a == 0 ? ub : 0.f);
         }
     }
 }
@@ -1948,7 +2043,13 @@ __device__ __forceinline__ void b_place_near(KP P, Bufs B, const TV* xv, int bx_
         MolDst dst{B.P + (fk * P.Nv + (int64_t)pos * P.nv) * 3, B.vw_r + fk * P.Nv + (int64_t)pos * P.nv};
         double uc[3] = {cm::u01(r.y), cm::u01(r.z), cm::u01(r.w)};
         double ua[3] = {cm::u01(r2.x), cm::u01(r2.y), cm::u01(r2.z)};
+        // the placed centre is H uc (a rigid copy of a molecule of the frame):
+        // the first atom's w carries the molecule's nearest-solute bound
+        int cell;
+        cell_of_frac(g, uc[0], uc[1], uc[2], cell);
+        const unsigned de = B.dfield[(int64_t)f * P.cap + cell];
         cm::random_move(src, dst, P.nv, P.iref, g.cell, uc, ua);
+        dst.w[0].w = molecule_bound(g, de, cell, uc);
     }
 }
 
@@ -2049,19 +2150,30 @@ __device__ __forceinline__ void eval_tiles_chunks(const KP& P, const Bufs& B, co
         const int p = p0 + pidx;
         const int64_t fp = (int64_t)f * P.NP + p;
         const Geo& g = B.geo[f];
-        const int m0 = (ch - B.chunk_base[fq]) * P.mpt;
-        const int nm = min(P.mpt, B.nnear[fp] - m0);
+        const int m0 = (ch - (cb_lds ? cbs[fq] : B.chunk_base[fq])) * P.mpt;
         const int ml = NVM ? row : row / P.nv, a = NVM ? 0 : row - (row / P.nv) * P.nv;
-        const bool active = ml < nm;
-        const int v = active ? B.msorted[fp * P.Mv + m0 + ml] : 0;
+        // the molecule's loads are issued together with the near count that
+        // decides whether the lane has a molecule (positions past the count
+        // read a valid slot and are discarded)
+        const int mpos = min(m0 + ml, P.Mv - 1);
+        const int nm = min(P.mpt, B.nnear[fp] - m0);
+        const int vld = B.msorted[fp * P.Mv + mpos];
         const float4* vwb = RANDOM ? B.vw_r + ((int64_t)f * P.K + pidx) * P.Nv : B.vw + (int64_t)f * P.Nv;
-        const int64_t pbase = (int64_t)(m0 + ml) * P.nv;   // sorted position of the molecule
+        const int64_t pbase = (int64_t)mpos * P.nv;   // sorted position of the molecule
         float4 y = mk4(0.f, 0.f, 0.f, 0.f);
         float4 ym[NVM > 0 ? NVM : 1];   // molecule per lane: its atoms
+        if constexpr (NVM > 0) {
+#pragma unroll
+            for (int q = 0; q < NVM; q++) ym[q] = vwb[pbase + q];
+        } else {
+            y = vwb[pbase + a];
+        }
+        const bool active = ml < nm;
+        const int v = active ? vld : 0;
         float alx, aly, alz, ahx, ahy, ahz;   // box of the lane's atoms
         if constexpr (NVM > 0) {
 #pragma unroll
-            for (int q = 0; q < NVM; q++) ym[q] = active ? vwb[pbase + q] : mk4(0.f, 0.f, 0.f, 0.f);
+            for (int q = 0; q < NVM; q++) if (!active) ym[q] = mk4(0.f, 0.f, 0.f, 0.f);
             alx = ahx = ym[0].x; aly = ahy = ym[0].y; alz = ahz = ym[0].z;
 #pragma unroll
             for (int q = 1; q < NVM; q++) {
@@ -2069,14 +2181,14 @@ __device__ __forceinline__ void eval_tiles_chunks(const KP& P, const Bufs& B, co
                 ahx = fmaxf(ahx, ym[q].x); ahy = fmaxf(ahy, ym[q].y); ahz = fmaxf(ahz, ym[q].z);
             }
         } else {
-            y = active ? vwb[pbase + a] : mk4(0.f, 0.f, 0.f, 0.f);
+            if (!active) y = mk4(0.f, 0.f, 0.f, 0.f);
             alx = ahx = y.x; aly = ahy = y.y; alz = ahz = y.z;
         }
         const float lx = wave_min(active ? alx : INFINITY), ly = wave_min(active ? aly : INFINITY),
                     lz = wave_min(active ? alz : INFINITY);
         const float hx = wave_max(active ? ahx : -INFINITY), hy = wave_max(active ? ahy : -INFINITY),
                     hz = wave_max(active ? ahz : -INFINITY);
-        const float rs = g.rstage, rs2 = rs * rs;
+        const float rs_full = g.rstage;
         // NVM > 0: distances relative to the chunk centre c.  With y' = y - c
         // and s' = s - c, |y' - s'|^2 = (|s'|^2 - 2 y'.s') + |y'|^2: three FMAs
         // per pair against the staged (s', |s'|^2), |y'|^2 added once per atom.
@@ -2096,7 +2208,7 @@ __device__ __forceinline__ void eval_tiles_chunks(const KP& P, const Bufs& B, co
             // (+ the rounding of c itself, at most 2^-24 |c| per component)
             const double Yb = 0.5 * sqrt(ex * ex + ey * ey + ez * ez) * (1.0 + 1e-6) +
                               1e-6 * ((double)fabsf(ccx) + fabsf(ccy) + fabsf(ccz)) + 1e-5;
-            const double Sb = Yb + (double)rs * (1.0 + 1e-5) + 1e-4;
+            const double Sb = Yb + (double)rs_full * (1.0 + 1e-5) + 1e-4;
             const double Gc = g.guard + 0x1p-24 * (Yb + Sb) * (2.0 * (P.cutsys + 0.5) + 7.0 * (Yb + Sb)) + 1e-9;
             Gcf = sgpr_f(nextafterf((float)Gc, INFINITY));
             cut2_hi_c = sgpr_f(nextafterf((float)(P.cut2 + Gc), INFINITY));
@@ -2106,6 +2218,20 @@ __device__ __forceinline__ void eval_tiles_chunks(const KP& P, const Bufs& B, co
             for (int q = 0; q < NVM; q++)
                 ym[q] = mk4(-2.0f * (ym[q].x - ccx), -2.0f * (ym[q].y - ccy), -2.0f * (ym[q].z - ccz), ym[q].w);
         }
+        // Staging radius.  Each molecule carries an upper bound U on its
+        // atoms' nearest solute distance (molecule_bound).  A solute atom that
+        // can decide anything for an atom p of the chunk (its FP64 nearest,
+        // or within the FP32 decision band of it) is within
+        // sqrt(U^2 + 2 G + amb) of p, hence of the AABB; staging out to
+        // sqrt(U^2 + 2 amb + 4 G) plus a slack over FP32 rounding keeps every
+        // such atom, and the farther ones cannot change a decision.
+        float rs = rs_full;
+        {
+            const float um = wave_max(active ? (NVM > 0 ? ym[0].w : y.w) : 0.f);
+            if (um < rs_full)
+                rs = sgpr_f(fminf(rs_full, sqrtf(fmaf(um, um, 2.0f * amb_c + 4.0f * Gcf)) * (1.0f + 1e-5f) + 1e-2f));
+        }
+        const float rs2 = rs * rs;
         int ilo[3], ihi[3];
         cell_range(g, lx, ly, lz, hx, hy, hz, rs, ilo, ihi);
         const int nx = ihi[0] - ilo[0] + 1, ny = ihi[1] - ilo[1] + 1, nz = ihi[2] - ilo[2] + 1;
@@ -2560,7 +2686,7 @@ __global__ void k_pack32(const ull* cnt, uint32_t* out, ull* hits, int64_t n, in
 // ---------------------------------------------------------------------------
 constexpr int MS_MAX = 128;   // jobs per round
 enum MStage {
-    MS_ZERO, MS_RADII, MS_GRID, MS_WRAP, MS_SCAN_CELLS, MS_SCATTER, MS_NEAR, MS_PREP, MS_SCAN_ROWS0,
+    MS_ZERO, MS_RADII, MS_GRID, MS_WRAP, MS_SCAN_CELLS, MS_SCATTER, MS_NEAR, MS_DFIELD, MS_PREP, MS_SCAN_ROWS0,
     MS_MOL_SCATTER0, MS_FILL, MS_CHUNK0, MS_BULK_INIT, MS_BULK_COMPACT, MS_PLACE_KEYS, MS_SCAN_ROWSK,
     MS_MOL_SCATTERK, MS_PLACE_NEAR, MS_CHUNKK, MS_GEO_FLAGS, MS_N
 };
@@ -2664,6 +2790,13 @@ __global__ void km_near_mark(const MultiTab* __restrict__ MT)
 {
     multi_blocks<MS_NEAR, false>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
         b_near_mark(S.P, S.B, bx, nbx, by, nby);
+    });
+}
+
+__global__ void km_dfield(const MultiTab* __restrict__ MT)
+{
+    multi_blocks<MS_DFIELD, false>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+        b_dfield(S.P, S.B, bx, nbx, by, nby);
     });
 }
 
@@ -3150,7 +3283,7 @@ int64_t per_frame_bytes(const cm_job* J, int dtype_size, bool host_input, int64_
         b += J->Nv * 16 + Mv * 20;
         b += (int64_t)J->K * (J->Nv * (24 + 16) + Mv * 16);
     } else {
-        b += cap + NP * ((kcap + 1) * 8 + Mv * 8 + 4) + 8;
+        b += cap * 5 + NP * ((kcap + 1) * 8 + Mv * 8 + 4) + 8;
         b += J->Nv * 16 + (int64_t)J->K * J->Nv * (24 + 16);
     }
     b += (int64_t)J->kb * Mv * 5 + J->kb * 4;
@@ -3307,6 +3440,11 @@ int batch_geo(const cm_job* J, int F, const int64_t* frame_index, const double* 
 }
 
 // The kernels' view of a job's buffers (geometry array at geo)
+// The near flags [F][cap] (bytes) and, after them, the encoded distance
+// bounds [F][cap] of the tiles family: one block, cleared together per batch.
+size_t near_flags_bytes(int64_t F, int64_t cap) { return ((size_t)F * cap + 15) & ~(size_t)15; }
+size_t near_bytes(int64_t F, int64_t cap) { return near_flags_bytes(F, cap) + (((size_t)4 * F * cap + 15) & ~(size_t)15); }
+
 Bufs batch_bufs(const cm_job* J, Geo* geo)
 {
     Bufs B;
@@ -3361,6 +3499,7 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
     hipStream_t s = J->stream;
     HIPCHK(hipMemcpyAsync(J->geo.p, J->hgeo.data(), sizeof(Geo) * F, hipMemcpyHostToDevice, s));
     Bufs B = batch_bufs(J, (Geo*)J->geo.p);
+    B.dfield = (unsigned*)((char*)J->near.p + near_flags_bytes(F, cap));
     B.vrec = (J->verify && tiles) ? (VRec*)J->vrec.p : nullptr;
     if (B.vrec) {
         const int64_t nrec = (int64_t)F * P.NP * P.Mv;
@@ -3396,7 +3535,7 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
     }
     if (int r = zero_async(J->cells_cnt.p, sizeof(int) * (size_t)F * (cap + 1), s)) return r;
     if (tiles) {
-        if (int r = zero_async(J->near.p, (size_t)F * cap, s)) return r;
+        if (int r = zero_async(J->near.p, near_bytes(F, cap), s)) return r;
         if (int r = zero_async(J->mcnt.p, sizeof(int) * (size_t)F * P.NP * (kcap + 1), s)) return r;
     }
 
@@ -3442,6 +3581,7 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
     hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)F * B.nitems)), dim3(BLOCK), 0, s, P, B, B.nitems);
     if (tiles) {
         hipLaunchKernelGGL(k_near_mark, dim3(grid_for((int64_t)F * P.nu)), dim3(BLOCK), 0, s, P, B);
+        hipLaunchKernelGGL(k_dfield, dim3(grid_for((int64_t)F * cap * 8)), dim3(BLOCK), 0, s, P, B);
         int64_t n = (int64_t)F * P.Mv;
         if (f32) hipLaunchKernelGGL(k_prep_solvent_tiles<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);
         else hipLaunchKernelGGL(k_prep_solvent_tiles<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vd);
@@ -3548,7 +3688,7 @@ int alloc_batch(cm_job* J, int F, int64_t cap, int64_t kcap, int dtype_size, boo
         if ((r = ensure(J, J->vcell, 4 * F * Mv))) return r;
         if ((r = ensure(J, J->vcen, 16 * F * Mv))) return r;
     } else {
-        if ((r = ensure(J, J->near, round16((size_t)F * cap)))) return r;
+        if ((r = ensure(J, J->near, near_bytes(F, cap)))) return r;
         if ((r = ensure(J, J->mkey, (size_t)4 * F * NP * Mv))) return r;
         if ((r = ensure(J, J->mcnt, round16((size_t)4 * F * NP * (kcap + 1))))) return r;
         if ((r = ensure(J, J->mstart, (size_t)4 * F * NP * (kcap + 1)))) return r;
@@ -4730,6 +4870,7 @@ void multi_launch(hipStream_t s, const MultiTab* tab, int tcap, int cus)
     hipLaunchKernelGGL(km_scan_cells, dim3(MG_1024), dim3(1024), 0, s, tab);
     hipLaunchKernelGGL(km_scatter, dim3(MG_256), dim3(BLOCK), 0, s, tab);
     hipLaunchKernelGGL(km_near_mark, dim3(MG_256), dim3(BLOCK), 0, s, tab);
+    hipLaunchKernelGGL(km_dfield, dim3(MG_256), dim3(BLOCK), 0, s, tab);
     hipLaunchKernelGGL(km_prep_solvent_tiles<T>, dim3(MG_256), dim3(BLOCK), 0, s, tab);
     hipLaunchKernelGGL(km_scan_rows<false>, dim3(MG_1024), dim3(1024), 0, s, tab);
     hipLaunchKernelGGL(km_mol_scatter<false>, dim3(MG_256), dim3(BLOCK), 0, s, tab);
@@ -4793,6 +4934,7 @@ void stage_blocks(const KP& P, int F, const int64_t z16[3], int nbx[MS_N], int c
     set(MS_SCAN_CELLS, F, 1);
     set(MS_SCATTER, grid_for((int64_t)F * P.nu), 1);
     set(MS_NEAR, grid_for((int64_t)F * P.nu), 1);
+    set(MS_DFIELD, grid_for((int64_t)F * P.cap * 8), 1);
     set(MS_PREP, grid_for((int64_t)F * P.Mv), 1);
     set(MS_SCAN_ROWS0, F, 1);
     set(MS_MOL_SCATTER0, mvb, std::min(F, 65535));
@@ -4855,13 +4997,14 @@ int multi_run_group(MultiRunner& R, std::vector<MJob>& grp, int dtype)
                 S.P.cap = (int)m.cap;
                 S.P.kcap = (int)m.kcap;
                 S.B = batch_bufs(J, gd + goff);
+                S.B.dfield = (unsigned*)((char*)J->near.p + near_flags_bytes(F, m.cap));
                 S.xu = (const char*)fr.d_x_solute + (size_t)f0 * J->Nu * 3 * ds;
                 S.xv = (const char*)fr.d_x_solvent + (size_t)f0 * J->Nv * 3 * ds;
                 S.zp[0] = (uint4*)J->cells_cnt.p;
                 S.zp[1] = (uint4*)J->near.p;
                 S.zp[2] = (uint4*)J->mcnt.p;
                 S.z16[0] = (int64_t)(round16(sizeof(int) * (size_t)F * (m.cap + 1)) / 16);
-                S.z16[1] = (int64_t)(round16((size_t)F * m.cap) / 16);
+                S.z16[1] = (int64_t)(near_bytes(F, m.cap) / 16);
                 S.z16[2] = (int64_t)(round16((size_t)4 * F * S.P.NP * (m.kcap + 1)) / 16);
                 stage_blocks(S.P, F, S.z16, S.nbx, cnt);
                 for (int st = 0; st < MS_N; st++) T->base[st][i + 1] = T->base[st][i] + cnt[st];
