@@ -1677,37 +1677,62 @@ __device__ __forceinline__ void cell_range(const Geo& g, float lx, float ly, flo
 }
 
 
-// cells near the solute: every cell within rnear of a solute atom (index box)
+// cells near the solute: every cell within rnear of a solute atom (index
+// box).  A block takes 256 consecutive atoms of one frame (neighbours along
+// the chain, whose cell ranges overlap): the marks go into a bitmap of the
+// frame's cells in LDS first, and the block writes each marked cell once.
+constexpr int NEAR_WORDS = 8192;   // LDS bitmap words (frames of up to 262,144 cells; larger: direct stores)
+
 __device__ __forceinline__ void b_near_mark(KP P, Bufs B, int bx_, int nbx_, int by_, int nby_)
 {
-    int64_t total = (int64_t)P.F * P.nu;
-    for (int64_t it = bx_ * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)nbx_ * blockDim.x) {
-        int f = (int)(it / P.nu);
+    __shared__ unsigned bits[NEAR_WORDS];
+    for (int f = by_; f < P.F; f += nby_) {
         const Geo& g = B.geo[f];
-        float4 s = B.tmp4[it];
-        int ilo[3], ihi[3];
-        cell_range(g, s.x, s.y, s.z, s.x, s.y, s.z, g.rnear, ilo, ihi);
+        const int ncell = g.nc[0] * g.nc[1] * g.nc[2];
+        const bool lds = ncell <= NEAR_WORDS * 32;
+        const int nw = (ncell + 31) >> 5;
+        if (lds)
+            for (int w = threadIdx.x; w < nw; w += blockDim.x) bits[w] = 0u;
+        __syncthreads();
         unsigned char* nr = B.near + (int64_t)f * P.cap;
         // boxes of a diagonal cell: only the cells whose box is within rnear
         // (in the image of index i); otherwise the whole index box
         const float rn2 = g.rnear * g.rnear;
         const float hx = g.H32[0] / g.nc[0], hy = g.H32[4] / g.nc[1], hz = g.H32[8] / g.nc[2];
         auto gap = [](float v, float lo, float hi) { return fmaxf(fmaxf(lo - v, v - hi), 0.f); };
-        for (int iz = ilo[2]; iz <= ihi[2]; iz++) {
-            int mz; int cz = wrap_index(iz, g.nc[2], mz);
-            const float dz = g.ortho ? gap(s.z, iz * hz, (iz + 1) * hz) : 0.f;
-            for (int iy = ilo[1]; iy <= ihi[1]; iy++) {
-                int my; int cy = wrap_index(iy, g.nc[1], my);
-                const float dy = g.ortho ? gap(s.y, iy * hy, (iy + 1) * hy) : 0.f;
-                if (dz * dz + dy * dy > rn2) continue;
-                for (int ix = ilo[0]; ix <= ihi[0]; ix++) {
-                    int mx; int cx = wrap_index(ix, g.nc[0], mx);
-                    const float dx = g.ortho ? gap(s.x, ix * hx, (ix + 1) * hx) : 0.f;
-                    if (dz * dz + dy * dy + dx * dx > rn2) continue;
-                    nr[(cz * g.nc[1] + cy) * g.nc[0] + cx] = 1;
+        for (int64_t at = bx_ * (int64_t)blockDim.x + threadIdx.x; at < P.nu; at += (int64_t)nbx_ * blockDim.x) {
+            const float4 s = B.tmp4[(int64_t)f * P.nu + at];
+            int ilo[3], ihi[3];
+            cell_range(g, s.x, s.y, s.z, s.x, s.y, s.z, g.rnear, ilo, ihi);
+            for (int iz = ilo[2]; iz <= ihi[2]; iz++) {
+                int mz; int cz = wrap_index(iz, g.nc[2], mz);
+                const float dz = g.ortho ? gap(s.z, iz * hz, (iz + 1) * hz) : 0.f;
+                for (int iy = ilo[1]; iy <= ihi[1]; iy++) {
+                    int my; int cy = wrap_index(iy, g.nc[1], my);
+                    const float dy = g.ortho ? gap(s.y, iy * hy, (iy + 1) * hy) : 0.f;
+                    if (dz * dz + dy * dy > rn2) continue;
+                    for (int ix = ilo[0]; ix <= ihi[0]; ix++) {
+                        int mx; int cx = wrap_index(ix, g.nc[0], mx);
+                        const float dx = g.ortho ? gap(s.x, ix * hx, (ix + 1) * hx) : 0.f;
+                        if (dz * dz + dy * dy + dx * dx > rn2) continue;
+                        const int cell = (cz * g.nc[1] + cy) * g.nc[0] + cx;
+                        if (lds) atomicOr(&bits[cell >> 5], 1u << (cell & 31));
+                        else nr[cell] = 1;
+                    }
                 }
             }
         }
+        __syncthreads();
+        if (lds)
+            for (int w = threadIdx.x; w < nw; w += blockDim.x) {
+                unsigned m = bits[w];
+                while (m) {
+                    const int bit = __ffs(m) - 1;
+                    m &= m - 1u;
+                    nr[(w << 5) + bit] = 1;
+                }
+            }
+        __syncthreads();
     }
 }
 
@@ -1881,8 +1906,8 @@ __global__ void k_fill_real(KP P, Bufs B, const T* xv)
 }
 
 // exclusive scan of the cell counts of passes [p0, p0+np) of every frame (one
-// block per (frame, pass)); zeroes the counts and writes the number of near
-// molecules of the pass
+// block per (frame, pass)); writes the number of near molecules of the pass
+// (the counts stay: the scatter counts them back down to zero)
 __device__ __forceinline__ void b_scan_rows(KP P, Bufs B, int p0, int np, int bx_, int nbx_, int by_, int nby_)
 {
     __shared__ int wsum[16];
@@ -1911,7 +1936,7 @@ __device__ __forceinline__ void b_scan_rows(KP P, Bufs B, int p0, int np, int bx
         int e = carry + woff + incl - tsum;
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            if (i0 + u < n) { st[i0 + u] = e; c[i0 + u] = 0; }
+            if (i0 + u < n) st[i0 + u] = e;
             e += v[u];
         }
         __syncthreads();
@@ -1942,7 +1967,9 @@ __device__ __forceinline__ void b_mol_scatter(KP P, Bufs B, int p0, int np, int 
         int key = B.mkey[fp * P.Mv + v];
         if (key < 0) continue;
         int64_t cb = fp * (P.kcap + 1);
-        int pos = B.mstart[cb + key] + atomicAdd(&B.mcnt[cb + key], 1);
+        // the key's count (left by the scan) counted down: each molecule
+        // takes one slot of the key's range
+        int pos = B.mstart[cb + key] + atomicSub(&B.mcnt[cb + key], 1) - 1;
         B.msorted[fp * P.Mv + pos] = v;
     }
 }
@@ -2788,7 +2815,7 @@ __global__ void km_scatter(const MultiTab* __restrict__ MT)
 
 __global__ void km_near_mark(const MultiTab* __restrict__ MT)
 {
-    multi_blocks<MS_NEAR, false>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
+    multi_blocks<MS_NEAR, true>(MT, [&](const SysDev& S, int bx, int nbx, int by, int nby) {
         b_near_mark(S.P, S.B, bx, nbx, by, nby);
     });
 }
@@ -3580,7 +3607,7 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
     hipLaunchKernelGGL(k_scan_cells, dim3(F), dim3(1024), 0, s, P, B);
     hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)F * B.nitems)), dim3(BLOCK), 0, s, P, B, B.nitems);
     if (tiles) {
-        hipLaunchKernelGGL(k_near_mark, dim3(grid_for((int64_t)F * P.nu)), dim3(BLOCK), 0, s, P, B);
+        hipLaunchKernelGGL(k_near_mark, dim3((P.nu + BLOCK - 1) / BLOCK, std::min(F, 65535)), dim3(BLOCK), 0, s, P, B);
         hipLaunchKernelGGL(k_dfield, dim3(grid_for((int64_t)F * cap * 8)), dim3(BLOCK), 0, s, P, B);
         int64_t n = (int64_t)F * P.Mv;
         if (f32) hipLaunchKernelGGL(k_prep_solvent_tiles<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);
@@ -4933,7 +4960,7 @@ void stage_blocks(const KP& P, int F, const int64_t z16[3], int nbx[MS_N], int c
     set(MS_WRAP, grid_for((int64_t)F * P.nu), 1);
     set(MS_SCAN_CELLS, F, 1);
     set(MS_SCATTER, grid_for((int64_t)F * P.nu), 1);
-    set(MS_NEAR, grid_for((int64_t)F * P.nu), 1);
+    set(MS_NEAR, (P.nu + BLOCK - 1) / BLOCK, std::min(F, 65535));
     set(MS_DFIELD, grid_for((int64_t)F * P.cap * 8), 1);
     set(MS_PREP, grid_for((int64_t)F * P.Mv), 1);
     set(MS_SCAN_ROWS0, F, 1);
