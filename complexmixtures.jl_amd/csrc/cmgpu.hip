@@ -159,8 +159,8 @@ struct Bufs {
     int* drawn_idx;          // [F][K]
     // tiles family: near cells and molecules sorted by cell per (frame, pass)
     unsigned char* near;     // [F][cap]
-    unsigned* dfield;        // [F][cap] 0xffffffff - bits of the distance from a near cell's centre to the
-                             // nearest solute atom image of its own and face-neighbour cells (0: none)
+    unsigned* dfield;        // [2][F][cap] 0xffffffff - bits of the distances from a near cell's centre and
+                             // lower corner to the nearest solute atom image nearby (0: none), b_dfield
     int* mkey;               // [F][NP][Mv] Morton key of the centre's cell, -1 when far
     int* mcnt;               // [F][NP][kcap+1]
     int* mstart;             // [F][NP][kcap+1]
@@ -1742,35 +1742,46 @@ __global__ void k_near_mark(KP P, Bufs B)
 }
 
 // Every near cell: the distance from its centre q to the nearest image of a
-// solute atom in the cell itself or one of its six face neighbours, encoded
-// as 0xffffffff - bits (0, the cleared value, when they hold none).  It is
-// the distance to one particular atom image, so for any point p
-// min_a |p - a| <= |p - q| + that distance: each molecule of the cell gets
-// from it an upper bound on its atoms' nearest solute distance
-// (molecule_bound).  Eight lanes per cell, lane j < 7 scanning one cell.
+// solute atom in the cell itself or one of its six face neighbours, and the
+// distance from its lower corner to the nearest image in the eight cells
+// around that corner; each encoded as 0xffffffff - bits (0, the cleared
+// value, when the cells hold no atom).  Each is the distance to one
+// particular atom image, so for any point p min_a |p - a| <= |p - q| + that
+// distance: each molecule of the cell gets from them an upper bound on its
+// atoms' nearest solute distance (molecule_bound).  Sixteen lanes per cell:
+// lane j < 7 scans one cell for the centre, lane 8 + t one for the corner.
 __device__ __forceinline__ void b_dfield(KP P, Bufs B, int bx_, int nbx_, int by_, int nby_)
 {
     const int64_t total = (int64_t)P.F * P.cap;
-    const int j = threadIdx.x & 7;
-    const int64_t step = ((int64_t)nbx_ * blockDim.x) >> 3;
-    // the eight lanes of a group share it (blockDim is a multiple of 8)
-    for (int64_t it = (bx_ * (int64_t)blockDim.x + threadIdx.x) >> 3; it < total; it += step) {
+    const int j = threadIdx.x & 15;
+    const int64_t step = ((int64_t)nbx_ * blockDim.x) >> 4;
+    // the sixteen lanes of a group share it (blockDim is a multiple of 16)
+    for (int64_t it = (bx_ * (int64_t)blockDim.x + threadIdx.x) >> 4; it < total; it += step) {
         const bool nearc = B.near[it] != 0;
         float best = INFINITY;
-        if (nearc && j < 7) {
+        if (nearc && j != 7) {
             const int f = (int)(it / P.cap);
             const int cell = (int)(it - (int64_t)f * P.cap);
             const Geo& g = B.geo[f];
             const int cx = cell % g.nc[0], cy = (cell / g.nc[0]) % g.nc[1], cz = cell / (g.nc[0] * g.nc[1]);
-            const int ax = j == 1 ? 1 : (j == 2 ? -1 : 0), ay = j == 3 ? 1 : (j == 4 ? -1 : 0),
-                      az = j == 5 ? 1 : (j == 6 ? -1 : 0);
+            int ax, ay, az;
+            float fx, fy, fz;
+            if (j < 7) {
+                ax = j == 1 ? 1 : (j == 2 ? -1 : 0);
+                ay = j == 3 ? 1 : (j == 4 ? -1 : 0);
+                az = j == 5 ? 1 : (j == 6 ? -1 : 0);
+                fx = (cx + 0.5f) / g.nc[0]; fy = (cy + 0.5f) / g.nc[1]; fz = (cz + 0.5f) / g.nc[2];
+            } else {
+                const int t = j - 8;
+                ax = -(t & 1); ay = -((t >> 1) & 1); az = -((t >> 2) & 1);
+                fx = (float)cx / g.nc[0]; fy = (float)cy / g.nc[1]; fz = (float)cz / g.nc[2];
+            }
             int mx, my, mz;
             const int ix = wrap_index(cx + ax, g.nc[0], mx), iy = wrap_index(cy + ay, g.nc[1], my),
                       iz = wrap_index(cz + az, g.nc[2], mz);
             float tx, ty, tz;
             shift_of(g, mx, my, mz, tx, ty, tz);
-            const float fx = (cx + 0.5f) / g.nc[0], fy = (cy + 0.5f) / g.nc[1], fz = (cz + 0.5f) / g.nc[2];
-            // q minus the image shift, against the wrapped atoms
+            // the reference point minus the image shift, against the wrapped atoms
             const float rx = g.H32[0] * fx + g.H32[3] * fy + g.H32[6] * fz - tx;
             const float ry = g.H32[1] * fx + g.H32[4] * fy + g.H32[7] * fz - ty;
             const float rz = g.H32[2] * fx + g.H32[5] * fy + g.H32[8] * fz - tz;
@@ -1795,10 +1806,12 @@ __device__ __forceinline__ void b_dfield(KP P, Bufs B, int bx_, int nbx_, int by
                 best = fminf(best, ex * ex + ey * ey + ez * ez);
             }
         }
+        // minimum over each eight-lane half (centre: lanes 0-6, corner: 8-15)
         best = fminf(best, __shfl_xor(best, 1, 64));
         best = fminf(best, __shfl_xor(best, 2, 64));
         best = fminf(best, __shfl_xor(best, 4, 64));
-        if (nearc && j == 0 && best < INFINITY) B.dfield[it] = 0xffffffffu - __float_as_uint(sqrtf(best));
+        if (nearc && (j & 7) == 0 && best < INFINITY)
+            B.dfield[(j ? total : 0) + it] = 0xffffffffu - __float_as_uint(sqrtf(best));
     }
 }
 
@@ -1809,25 +1822,56 @@ __global__ void k_dfield(KP P, Bufs B)
 
 // Upper bound (FP32, rounded outward by a wide margin) on the distance from
 // any atom of a molecule whose centre has fractional coordinates fr (in the
-// near cell `cell`) to its nearest solute atom: dfield of the cell, plus the
-// distance from the centre to the cell centre, plus the molecule radius.
-// +inf when the cell found no solute atom.
-// FP32 throughout: every rounding is far below the 1e-5 relative + 0.01 A
-// margin.
-// e: the cell's dfield entry, read by the caller before its stores (they may
-// alias it as far as the compiler knows, which would hold the load back)
-__device__ __forceinline__ float molecule_bound(const Geo& g, unsigned e, int cell, const double fr[3])
+// near cell `cell`) to its nearest solute atom: the smallest, over the cell's
+// centre and its eight corners, of the reference point's atom distance plus
+// the distance from the centre to the point, plus the molecule radius; +inf
+// when none has a distance.  FP32 throughout: every rounding is far below the
+// 1e-5 relative + 0.01 A margin.
+struct BoundLoads {
+    unsigned e[9];   // the centre's entry, then the corners (a + 2 b + 4 c)
+};
+
+// the entries, read by the callers before their stores (which may alias them
+// as far as the compiler knows, and would hold the loads back)
+__device__ __forceinline__ BoundLoads bound_loads(const KP& P, const Bufs& B, const Geo& g, int f, int cell)
 {
-    if (e == 0u) return INFINITY;
-    const float D = __uint_as_float(0xffffffffu - e);
+    BoundLoads L;
+    const unsigned* d = B.dfield + (int64_t)f * P.cap;
+    const unsigned* dc = d + (int64_t)P.F * P.cap;
+    L.e[0] = d[cell];
     const int c0 = cell % g.nc[0], c1 = (cell / g.nc[0]) % g.nc[1], c2 = cell / (g.nc[0] * g.nc[1]);
-    const float d0 = (float)fr[0] - (c0 + 0.5f) / g.nc[0], d1 = (float)fr[1] - (c1 + 0.5f) / g.nc[1],
-                d2 = (float)fr[2] - (c2 + 0.5f) / g.nc[2];
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        int i0 = c0 + (t & 1), i1 = c1 + ((t >> 1) & 1), i2 = c2 + ((t >> 2) & 1);
+        if (i0 == g.nc[0]) i0 = 0;
+        if (i1 == g.nc[1]) i1 = 0;
+        if (i2 == g.nc[2]) i2 = 0;
+        L.e[1 + t] = dc[(i2 * g.nc[1] + i1) * g.nc[0] + i0];
+    }
+    return L;
+}
+
+__device__ __forceinline__ float molecule_bound(const Geo& g, const BoundLoads& L, int cell, const double fr[3])
+{
+    const int c0 = cell % g.nc[0], c1 = (cell / g.nc[0]) % g.nc[1], c2 = cell / (g.nc[0] * g.nc[1]);
     const float* H = g.H32;
-    const float ex = H[0] * d0 + H[3] * d1 + H[6] * d2;
-    const float ey = H[1] * d0 + H[4] * d1 + H[7] * d2;
-    const float ez = H[2] * d0 + H[5] * d1 + H[8] * d2;
-    return (D + sqrtf(ex * ex + ey * ey + ez * ez) + __uint_as_float(g.rv_bits)) * (1.0f + 1e-5f) + 1e-2f;
+    float best = INFINITY;
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+        if (L.e[t] == 0u) continue;
+        const float D = __uint_as_float(0xffffffffu - L.e[t]);
+        // the reference point's fractional coordinates
+        const float q0 = t == 0 ? (c0 + 0.5f) / g.nc[0] : (float)(c0 + ((t - 1) & 1)) / g.nc[0];
+        const float q1 = t == 0 ? (c1 + 0.5f) / g.nc[1] : (float)(c1 + (((t - 1) >> 1) & 1)) / g.nc[1];
+        const float q2 = t == 0 ? (c2 + 0.5f) / g.nc[2] : (float)(c2 + (((t - 1) >> 2) & 1)) / g.nc[2];
+        const float d0 = (float)fr[0] - q0, d1 = (float)fr[1] - q1, d2 = (float)fr[2] - q2;
+        const float ex = H[0] * d0 + H[3] * d1 + H[6] * d2;
+        const float ey = H[1] * d0 + H[4] * d1 + H[7] * d2;
+        const float ez = H[2] * d0 + H[5] * d1 + H[8] * d2;
+        best = fminf(best, D + sqrtf(ex * ex + ey * ey + ez * ez));
+    }
+    if (!(best < INFINITY)) return INFINITY;
+    return (best + __uint_as_float(g.rv_bits)) * (1.0f + 1e-5f) + 1e-2f;
 }
 
 // real pass: centre of every solvent molecule (made whole around its first
@@ -1888,7 +1932,7 @@ __device__ __forceinline__ void b_fill_real(KP P, Bufs B, const T* xv, int bx_, 
         // the first atom's w carries the molecule's nearest-solute bound
         int cell;
         cell_of_frac(g, fr[0], fr[1], fr[2], cell);
-        const float ub = molecule_bound(g, B.dfield[(int64_t)f * P.cap + cell], cell, fr);
+        const float ub = molecule_bound(g, bound_loads(P, B, g, f, cell), cell, fr);
         float4* dst = B.vw + (int64_t)f * P.Nv + (int64_t)pos * P.nv;
         for (int a = 0; a < P.nv; a++) {
             double xa, ya, za, ox, oy, oz;
@@ -2074,9 +2118,9 @@ __device__ __forceinline__ void b_place_near(KP P, Bufs B, const TV* xv, int bx_
         // the first atom's w carries the molecule's nearest-solute bound
         int cell;
         cell_of_frac(g, uc[0], uc[1], uc[2], cell);
-        const unsigned de = B.dfield[(int64_t)f * P.cap + cell];
+        const float ub = molecule_bound(g, bound_loads(P, B, g, f, cell), cell, uc);
         cm::random_move(src, dst, P.nv, P.iref, g.cell, uc, ua);
-        dst.w[0].w = molecule_bound(g, de, cell, uc);
+        dst.w[0].w = ub;
     }
 }
 
@@ -3310,7 +3354,7 @@ int64_t per_frame_bytes(const cm_job* J, int dtype_size, bool host_input, int64_
         b += J->Nv * 16 + Mv * 20;
         b += (int64_t)J->K * (J->Nv * (24 + 16) + Mv * 16);
     } else {
-        b += cap * 5 + NP * ((kcap + 1) * 8 + Mv * 8 + 4) + 8;
+        b += cap * 9 + NP * ((kcap + 1) * 8 + Mv * 8 + 4) + 8;
         b += J->Nv * 16 + (int64_t)J->K * J->Nv * (24 + 16);
     }
     b += (int64_t)J->kb * Mv * 5 + J->kb * 4;
@@ -3467,10 +3511,11 @@ int batch_geo(const cm_job* J, int F, const int64_t* frame_index, const double* 
 }
 
 // The kernels' view of a job's buffers (geometry array at geo)
-// The near flags [F][cap] (bytes) and, after them, the encoded distance
-// bounds [F][cap] of the tiles family: one block, cleared together per batch.
+// The near flags [F][cap] (bytes) and, after them, the encoded distances of
+// the cell centres [F][cap] and lower corners [F][cap] of the tiles family:
+// one block, cleared together per batch.
 size_t near_flags_bytes(int64_t F, int64_t cap) { return ((size_t)F * cap + 15) & ~(size_t)15; }
-size_t near_bytes(int64_t F, int64_t cap) { return near_flags_bytes(F, cap) + (((size_t)4 * F * cap + 15) & ~(size_t)15); }
+size_t near_bytes(int64_t F, int64_t cap) { return near_flags_bytes(F, cap) + (((size_t)8 * F * cap + 15) & ~(size_t)15); }
 
 Bufs batch_bufs(const cm_job* J, Geo* geo)
 {
@@ -3608,7 +3653,7 @@ int run_batch(cm_job* J, int F, const int64_t* frame_index, const void* xu_dev, 
     hipLaunchKernelGGL(k_scatter, dim3(grid_for((int64_t)F * B.nitems)), dim3(BLOCK), 0, s, P, B, B.nitems);
     if (tiles) {
         hipLaunchKernelGGL(k_near_mark, dim3((P.nu + BLOCK - 1) / BLOCK, std::min(F, 65535)), dim3(BLOCK), 0, s, P, B);
-        hipLaunchKernelGGL(k_dfield, dim3(grid_for((int64_t)F * cap * 8)), dim3(BLOCK), 0, s, P, B);
+        hipLaunchKernelGGL(k_dfield, dim3(grid_for((int64_t)F * cap * 16)), dim3(BLOCK), 0, s, P, B);
         int64_t n = (int64_t)F * P.Mv;
         if (f32) hipLaunchKernelGGL(k_prep_solvent_tiles<float>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vf);
         else hipLaunchKernelGGL(k_prep_solvent_tiles<double>, dim3(grid_for(n)), dim3(BLOCK), 0, s, P, B, vd);
@@ -4961,7 +5006,7 @@ void stage_blocks(const KP& P, int F, const int64_t z16[3], int nbx[MS_N], int c
     set(MS_SCAN_CELLS, F, 1);
     set(MS_SCATTER, grid_for((int64_t)F * P.nu), 1);
     set(MS_NEAR, (P.nu + BLOCK - 1) / BLOCK, std::min(F, 65535));
-    set(MS_DFIELD, grid_for((int64_t)F * P.cap * 8), 1);
+    set(MS_DFIELD, grid_for((int64_t)F * P.cap * 16), 1);
     set(MS_PREP, grid_for((int64_t)F * P.Mv), 1);
     set(MS_SCAN_ROWS0, F, 1);
     set(MS_MOL_SCATTER0, mvb, std::min(F, 65535));
