@@ -418,10 +418,10 @@ def main():
     if args.cell_fraction or args.columns:
         tuning = dict(cell_fraction=args.cell_fraction or 0.5, columns=args.columns)
     else:
-        # the autotuner's choice on the first 8 frames (untimed)
+        # the autotuner's choice on the first 40 frames (one batch; untimed)
         import complexmixtures_amd.serve as SV
         from complexmixtures_amd.mddf import job_kwargs
-        tuning, _ = SV.autotune_job(job_kwargs(R, opts, tm, False), du, dv, cells, frames=8)
+        tuning, _ = SV.autotune_job(job_kwargs(R, opts, tm, False), du, dv, cells, frames=40)
 
     def one_job(profile=False, nframes=nf):
         job = _job_for(R, opts, tm, False, tuning=tuning)

@@ -1535,16 +1535,23 @@ __device__ __forceinline__ void b_bulk_compact(KP P, Bufs B, int bx_, int nbx_, 
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int base = 0; base < P.Mv; base += blockDim.x) {
-        int v = base + threadIdx.x;
-        int flag = (v < P.Mv) ? fl[v] : 0;
-        ull mask = __ballot(flag);
-        int pre = __popcll(mask & ((1ull << lane) - 1ull));
-        if (lane == 0) wsum[wid] = __popcll(mask);
+    // four consecutive flags per thread, the counts scanned over the
+    // wavefront (DPP) and across the block's wavefronts
+    for (int base = 0; base < P.Mv; base += 4 * blockDim.x) {
+        const int v0 = base + 4 * threadIdx.x;
+        int fl4[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) fl4[u] = v0 + u < P.Mv ? fl[v0 + u] : 0;
+        const int cnt = fl4[0] + fl4[1] + fl4[2] + fl4[3];
+        const int incl = wave_incl_scan(cnt);
+        if (lane == 63) wsum[wid] = incl;
         __syncthreads();
         int woff = 0, tot = 0;
         for (int w = 0; w < nw; w++) { if (w < wid) woff += wsum[w]; tot += wsum[w]; }
-        if (flag) out[carry + woff + pre] = v;
+        int o = carry + woff + incl - cnt;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (fl4[u]) out[o++] = v0 + u;
         __syncthreads();
         if (threadIdx.x == 0) carry += tot;
         __syncthreads();
@@ -2728,20 +2735,30 @@ __global__ void k_flush(ull* cnt, double* acc, ull* hits, int64_t n, int64_t nhi
     }
 }
 
-// Read-back of a job whose frames all had one weight: the hits as 32-bit
-// words (over[0] set when one exceeds limit, at most 2^32 - 1), the md/rdf
-// hits added to the cumulative ones.  The counts stay in cnt for the FP64
-// path.
-__global__ void k_pack32(const ull* cnt, uint32_t* out, ull* hits, int64_t n, int64_t nhist, ull limit, ull* over)
+// Read-back of a job whose frames all had one weight: the md/rdf hits are
+// added to the cumulative ones (read back as 64-bit words); each group count
+// goes out as a 16-bit word, and the counts that do not fit (0xffff and up)
+// are listed as (entry, count) in ov, at most ovcap of them (over[1] counts
+// them all).  over[0] is set when a count exceeds limit (at most 2^32 - 1).
+// The counts stay in cnt for the FP64 path.
+__global__ void k_pack16(const ull* cnt, uint16_t* out, ull* hits, int64_t n, int64_t nhist, ull limit, ull* over,
+                         ulonglong2* ov, int64_t ovcap)
 {
     bool big = false;
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
         const ull c = cnt[k];
-        out[k] = (uint32_t)c;
         big |= c > limit;
-        if (k < nhist) hits[k] += c;
+        if (k < nhist) {
+            hits[k] += c;
+            continue;
+        }
+        out[k - nhist] = (uint16_t)(c < 0xffffull ? c : 0xffffull);
+        if (c >= 0xffffull) {
+            const ull slot = atomicAdd(&over[1], 1ull);
+            if (slot < (ull)ovcap) ov[slot] = make_ulonglong2((ull)k, c);
+        }
     }
-    if (big) atomicOr(over, 1ull);
+    if (big) atomicOr(&over[0], 1ull);
 }
 
 // ---------------------------------------------------------------------------
@@ -4105,15 +4122,42 @@ std::atomic<uint64_t>& pack_limit()
     return v;
 }
 
-int finish_copy(cm_job* J, cm_raw* out, const char* h, bool packed, double w, size_t piece, const hipEvent_t* ev)
+// Layout of a read-back in page-locked memory, for n accumulator entries of
+// which the first nh = 4 nbins are the md/rdf histograms: the wire data
+// (FP64 accumulators, or the 16-bit group counts of a packed read-back) in
+// [0, 8 n), the cumulative md/rdf hits (64-bit), three flag words (geometry
+// error bits, pack limit exceeded, number of overflowing counts), then the
+// overflow list (entry, count) of a packed read-back.
+struct RbLayout {
+    int64_t n, nh, ng, ovcap;
+    size_t off_hits, off_flags, off_ov, need, dev_ov;
+    RbLayout(int64_t n_, int64_t nbins)
+    {
+        n = n_;
+        nh = 4 * nbins;
+        ng = n - nh;
+        off_hits = 8 * (size_t)n;
+        off_flags = off_hits + 8 * (size_t)nh;
+        off_ov = off_flags + 32;
+        // the overflow list sits in the accumulator block after the 16-bit words
+        dev_ov = ((size_t)2 * ng + 15) & ~(size_t)15;
+        ovcap = std::min<int64_t>(4096, (int64_t)((8 * (size_t)n - dev_ov) / 16));
+        need = off_ov + 16 * (size_t)ovcap;
+    }
+};
+
+// Writes a read-back (layout L in h) into the caller's arrays: FP64 copies,
+// or for a packed read-back the products (double)count * w (w/2 in the
+// self-MDDF group range [half_lo, half_hi)), the values k_flush forms; worker
+// threads take tasks of 256K entries, and with ev each first waits for the
+// wire piece holding its task.
+int copy_out(const cm_job_desc& d, const RbLayout& L, const char* h, bool packed, double w, int64_t half_lo,
+             int64_t half_hi, size_t piece, const hipEvent_t* ev, cm_raw* out)
 {
-    const cm_job_desc& d = J->d;
     const int64_t nb = d.nbins, ngu = d.n_groups_u, ngv = d.n_groups_v;
-    const int64_t n = J->acc_len;
-    const size_t es = packed ? 4 : 8;
-    int64_t half_lo, half_hi;
-    half_range(J, &half_lo, &half_hi);
+    const int64_t n = L.n, nh = L.nh;
     const double wh = w / 2;
+    auto mult = [&](int64_t k) { return (k >= half_lo && k < half_hi) ? wh : w; };
     // the caller's arrays as entry ranges of the accumulator block, in order
     struct Seg { int64_t o, n; double* dst; };
     std::vector<Seg> segs;
@@ -4127,6 +4171,8 @@ int finish_copy(cm_job* J, cm_raw* out, const char* h, bool packed, double w, si
     add(out->solvent_group_count, ngv * nb);
     add(out->solute_group_count_random, ngu * nb);
     add(out->solvent_group_count_random, ngv * nb);
+    const ull* hits = (const ull*)(h + L.off_hits);
+    const uint16_t* w16 = (const uint16_t*)h;
     constexpr int64_t TASK = (int64_t)1 << 18;
     const int64_t ntask = (n + TASK - 1) / TASK;
     hipError_t werr = hipSuccess;
@@ -4136,11 +4182,14 @@ int finish_copy(cm_job* J, cm_raw* out, const char* h, bool packed, double w, si
             const int64_t a = t * TASK, b = std::min(n, a + TASK);
             if (ev) {
                 // pieces land in stream order: the piece holding the task's last byte
-                const hipError_t e = hipEventSynchronize(ev[(es * (size_t)b - 1) / piece]);
-                if (e != hipSuccess) {
-                    std::lock_guard<std::mutex> lk(werr_mu);
-                    werr = e;
-                    return;
+                const size_t last = packed ? (b > nh ? 2 * (size_t)(b - 1 - nh) + 1 : 0) : 8 * (size_t)b - 1;
+                if (!packed || b > nh) {
+                    const hipError_t e = hipEventSynchronize(ev[last / piece]);
+                    if (e != hipSuccess) {
+                        std::lock_guard<std::mutex> lk(werr_mu);
+                        werr = e;
+                        return;
+                    }
                 }
             }
             for (const Seg& sg : segs) {
@@ -4151,8 +4200,16 @@ int finish_copy(cm_job* J, cm_raw* out, const char* h, bool packed, double w, si
                     memcpy(dst, h + 8 * lo, 8 * (size_t)(hi - lo));
                     continue;
                 }
-                const uint32_t* src = (const uint32_t*)h;
-                for (int64_t k = lo; k < hi; k++) dst[k - lo] = (double)src[k] * ((k >= half_lo && k < half_hi) ? wh : w);
+                // one multiplier per array (the self-MDDF half range is whole
+                // arrays); signed conversions, a single instruction each
+                const double m = mult(lo);
+                if (hi <= nh) {
+                    for (int64_t k = lo; k < hi; k++) dst[k - lo] = (double)(int64_t)hits[k] * m;
+                } else {
+                    const uint16_t* src = w16 + (lo - nh);
+                    const int64_t cnt = hi - lo;
+                    for (int64_t k = 0; k < cnt; k++) dst[k] = (double)(int32_t)src[k] * m;
+                }
             }
         }
     };
@@ -4163,62 +4220,87 @@ int finish_copy(cm_job* J, cm_raw* out, const char* h, bool packed, double w, si
     for (auto& x : th) x.join();
     if (werr != hipSuccess) return fail(CM_EHIP, std::string("read-back: ") + hipGetErrorString(werr));
     if (ev) HIPCHK(hipEventSynchronize(ev[RB_PIECES - 1]));
-    out->volume_total = J->volume_total;
-    const ull* hits = (const ull*)(h + 8 * (size_t)n);
+    if (packed) {
+        // the counts that did not fit 16 bits
+        uint64_t nov;
+        memcpy(&nov, h + L.off_flags + 16, 8);
+        const ulonglong2* ov = (const ulonglong2*)(h + L.off_ov);
+        for (uint64_t q = 0; q < nov; q++) {
+            const int64_t k = (int64_t)ov[q].x;
+            for (const Seg& sg : segs)
+                if (k >= sg.o && k < sg.o + sg.n) sg.dst[k - sg.o] = (double)ov[q].y * mult(k);
+        }
+    }
     auto cpu = [&](uint64_t* dst, const ull* src) { if (dst) memcpy(dst, src, 8 * nb); };
     cpu(out->hits_md, hits);
     cpu(out->hits_md_random, hits + nb);
     cpu(out->hits_rdf, hits + 2 * nb);
     cpu(out->hits_rdf_random, hits + 3 * nb);
-    J->finished = true;
     return CM_OK;
 }
 
+// The packed read-back of a one-weight job: k_pack16 into the accumulator
+// block (its FP64 contents are not needed: the job has no flushed run)
+void launch_pack16(cm_job* J, const RbLayout& L, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack16, dim3(grid_for(L.n)), dim3(BLOCK), 0, s, (const ull*)J->cnt.p, (uint16_t*)J->acc.p,
+                       (ull*)J->hits.p, L.n, L.nh, (ull)pack_limit().load(), (ull*)J->stats.p + 3,
+                       (ulonglong2*)((char*)J->acc.p + L.dev_ov), L.ovcap);
+}
+
+// a packed read-back that cannot be decoded: a count above the pack limit,
+// or more overflowing counts than the list holds
+bool pack_failed(const RbLayout& L, const char* h)
+{
+    uint64_t fl[3];
+    memcpy(fl, h + L.off_flags, 24);
+    return fl[1] != 0 || fl[2] > (uint64_t)L.ovcap;
+}
+
 // Read-back of cm_job_finish, pipelined: the accumulators come back in
-// RB_PIECES pieces, each with an event, and worker threads copy 1M-entry tasks
-// into the caller's arrays as soon as the piece holding a task has landed, so
-// the copies into (often freshly allocated, page-faulting) caller memory
-// overlap the transfer.  The geometry error bits come back first.
+// RB_PIECES pieces, each with an event, and worker threads copy 256K-entry
+// tasks into the caller's arrays as soon as the piece holding a task has
+// landed, so the copies into (often freshly allocated, page-faulting) caller
+// memory overlap the transfer.  The geometry error bits come back first.
 //
 // When every frame of the job had one weight w (no weight change flushed
 // yet), each FP64 result is (double)hits * w (w/2 in the self-MDDF group
-// range) with nothing added before it, so the hits cross PCIe as 32-bit words
-// (half the bytes of the accumulators) and the host forms the same products,
-// rounded the same way.  A count that needs more than 32 bits sends the job
+// range) with nothing added before it, so the group counts cross PCIe as
+// 16-bit words (a quarter of the bytes of the accumulators; the few larger
+// counts in a list beside them, the md/rdf histograms from the 64-bit hits)
+// and the host forms the same products, rounded the same way.  A count above
+// the pack limit, or more large counts than the list holds, sends the job
 // down the FP64 path.
 int job_finish_pipelined(cm_job* J, cm_raw* out)
 {
     const cm_job_desc& d = J->d;
-    const int64_t nb = d.nbins, ngu = d.n_groups_u, ngv = d.n_groups_v;
-    const int64_t n = J->acc_len;
+    const int64_t nb = d.nbins;
+    const RbLayout L(J->acc_len, nb);
     hipStream_t s = J->stream;
     const bool packed = J->pending && J->flushes == 0;
     const double w = J->pending_w;
     if (packed) {
-        hipLaunchKernelGGL(k_pack32, dim3(grid_for(n)), dim3(BLOCK), 0, s, (const ull*)J->cnt.p, (uint32_t*)J->acc.p,
-                           (ull*)J->hits.p, n, 4 * nb, (ull)pack_limit().load(), (ull*)J->stats.p + 3);
+        launch_pack16(J, L, s);
         HIPCHK(hipGetLastError());
     } else if (J->pending) {
         if (int r = flush(J, J->pending_w)) return r;
     }
     J->pending = false;
-    const size_t acc_bytes = 8 * (size_t)n;
-    const size_t off_hits = acc_bytes, off_err = acc_bytes + 8 * 4 * (size_t)nb;
-    const size_t need = off_err + 16;
-    if (J->hres.bytes < need) {
+    if (J->hres.bytes < L.need) {
         pin_put(J->hres);
-        int r = pin_take(J->hres, need);
+        int r = pin_take(J->hres, L.need);
         if (r) return r;
     }
     for (auto& e : J->rb_ev)
         if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     char* h = (char*)J->hres.p;
-    // error bits and the packing overflow flag, then the md/rdf hits
-    HIPCHK(hipMemcpyAsync(h + off_err, (char*)J->stats.p + 16, 16, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(h + off_hits, J->hits.p, 8 * 4 * nb, hipMemcpyDeviceToHost, s));
+    // error bits and the packing flags, the md/rdf hits, the overflow list
+    HIPCHK(hipMemcpyAsync(h + L.off_flags, (char*)J->stats.p + 16, 24, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(h + L.off_hits, J->hits.p, 8 * (size_t)L.nh, hipMemcpyDeviceToHost, s));
+    if (packed)
+        HIPCHK(hipMemcpyAsync(h + L.off_ov, (char*)J->acc.p + L.dev_ov, 16 * (size_t)L.ovcap, hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(J->rb_ev[RB_PIECES], s));
-    const size_t es = packed ? 4 : 8;   // bytes per entry on the wire
-    const size_t wire = es * (size_t)n;
+    const size_t wire = packed ? 2 * (size_t)L.ng : 8 * (size_t)L.n;
     const size_t piece = ((wire + RB_PIECES - 1) / RB_PIECES + 4095) & ~(size_t)4095;
     for (int k = 0; k < RB_PIECES; k++) {
         const size_t o = std::min(wire, k * piece), m = std::min(wire, o + piece) - o;
@@ -4226,55 +4308,63 @@ int job_finish_pipelined(cm_job* J, cm_raw* out)
         HIPCHK(hipEventRecord(J->rb_ev[k], s));
     }
     HIPCHK(hipEventSynchronize(J->rb_ev[RB_PIECES]));
-    uint64_t flags[2];
-    memcpy(flags, h + off_err, 16);
-    if (int r = geometry_error(flags[0])) return r;
-    if (packed && flags[1]) {
-        // a count beyond 32 bits: the FP64 accumulators after all (the hits
-        // were already added to the cumulative md/rdf ones)
+    uint64_t gerr;
+    memcpy(&gerr, h + L.off_flags, 8);
+    if (int r = geometry_error(gerr)) return r;
+    int64_t half_lo, half_hi;
+    half_range(J, &half_lo, &half_hi);
+    int r;
+    if (packed && pack_failed(L, h)) {
+        // the FP64 accumulators after all (the hits were already added to
+        // the cumulative md/rdf ones)
         HIPCHK(hipStreamSynchronize(s));
-        if (int r = flush(J, w, nullptr, true, false)) return r;
-        HIPCHK(hipMemcpyAsync(h, J->acc.p, acc_bytes, hipMemcpyDeviceToHost, s));
+        if ((r = flush(J, w, nullptr, true, false))) return r;
+        HIPCHK(hipMemcpyAsync(h, J->acc.p, 8 * (size_t)L.n, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        return finish_copy(J, out, h, false, w, piece, nullptr);
+        r = copy_out(d, L, h, false, w, half_lo, half_hi, piece, nullptr, out);
+    } else {
+        r = copy_out(d, L, h, packed, w, half_lo, half_hi, piece, J->rb_ev, out);
     }
-    return finish_copy(J, out, h, packed, w, piece, J->rb_ev);
+    if (r) return r;
+    out->volume_total = J->volume_total;
+    J->finished = true;
+    return CM_OK;
 }
 
-// Reads a finished request's results back into J->hres: the accumulators
-// (or, when its frames shared one weight, the 32-bit hits: *packed set, as in
-// cm_job_finish), the md/rdf hits, then the geometry error bits.
+// Reads a finished request's results back into J->hres (layout RbLayout):
+// the accumulators, or, when its frames shared one weight, the packed counts
+// (*packed set, as in cm_job_finish), the md/rdf hits and the flags.
 int job_collect(cm_job* J, bool* packed)
 {
-    const int64_t n = J->acc_len, nb = J->d.nbins;
+    const RbLayout L(J->acc_len, J->d.nbins);
     *packed = J->pending && J->flushes == 0;
     if (*packed) {
-        hipLaunchKernelGGL(k_pack32, dim3(grid_for(n)), dim3(BLOCK), 0, J->stream, (const ull*)J->cnt.p,
-                           (uint32_t*)J->acc.p, (ull*)J->hits.p, n, 4 * nb, (ull)pack_limit().load(),
-                           (ull*)J->stats.p + 3);
+        launch_pack16(J, L, J->stream);
         HIPCHK(hipGetLastError());
     } else if (J->pending) {
         if (int r = flush(J, J->pending_w)) return r;
     }
     J->pending = false;
-    const size_t off_err = 8 * (size_t)n + 8 * 4 * (size_t)nb;
-    const size_t need = off_err + 16;
-    if (J->hres.bytes < need) {
+    if (J->hres.bytes < L.need) {
         pin_put(J->hres);
-        int r = pin_take(J->hres, need);
+        int r = pin_take(J->hres, L.need);
         if (r) return r;
     }
-    HIPCHK(hipMemcpyAsync(J->hres.p, J->acc.p, (*packed ? 4 : 8) * (size_t)n, hipMemcpyDeviceToHost, J->stream));
-    HIPCHK(hipMemcpyAsync((char*)J->hres.p + 8 * n, J->hits.p, 8 * 4 * nb, hipMemcpyDeviceToHost, J->stream));
-    HIPCHK(hipMemcpyAsync((char*)J->hres.p + off_err, (char*)J->stats.p + 16, 16, hipMemcpyDeviceToHost, J->stream));
+    char* h = (char*)J->hres.p;
+    HIPCHK(hipMemcpyAsync(h, J->acc.p, *packed ? 2 * (size_t)L.ng : 8 * (size_t)L.n, hipMemcpyDeviceToHost, J->stream));
+    HIPCHK(hipMemcpyAsync(h + L.off_hits, J->hits.p, 8 * (size_t)L.nh, hipMemcpyDeviceToHost, J->stream));
+    HIPCHK(hipMemcpyAsync(h + L.off_flags, (char*)J->stats.p + 16, 24, hipMemcpyDeviceToHost, J->stream));
+    if (*packed)
+        HIPCHK(hipMemcpyAsync(h + L.off_ov, (char*)J->acc.p + L.dev_ov, 16 * (size_t)L.ovcap, hipMemcpyDeviceToHost,
+                              J->stream));
     HIPCHK(hipStreamSynchronize(J->stream));
-    uint64_t flags[2];
-    memcpy(flags, (const char*)J->hres.p + off_err, 16);
-    if (int r = geometry_error(flags[0])) return r;
-    if (*packed && flags[1]) {
-        // a count beyond 32 bits: FP64 accumulators (md/rdf hits already added)
+    uint64_t gerr;
+    memcpy(&gerr, h + L.off_flags, 8);
+    if (int r = geometry_error(gerr)) return r;
+    if (*packed && pack_failed(L, h)) {
+        // FP64 accumulators (md/rdf hits already added)
         if (int r = flush(J, J->pending_w, nullptr, true, false)) return r;
-        HIPCHK(hipMemcpyAsync(J->hres.p, J->acc.p, 8 * (size_t)n, hipMemcpyDeviceToHost, J->stream));
+        HIPCHK(hipMemcpyAsync(h, J->acc.p, 8 * (size_t)L.n, hipMemcpyDeviceToHost, J->stream));
         HIPCHK(hipStreamSynchronize(J->stream));
         *packed = false;
     }
@@ -4282,82 +4372,15 @@ int job_collect(cm_job* J, bool* packed)
     return CM_OK;
 }
 
-// raw accumulators of a collected job into the caller's cm_raw
-// memcpy of large arrays split over a few threads (one thread copies pinned
-// to pageable memory at a few GB/s; the group counts of a 5,000-atom solute
-// are 40 MB)
-void copy_par(void* dst, const void* src, size_t bytes)
+// raw accumulators of a collected request into the caller's cm_raw
+int fill_raw(const cm_job_desc& d, int64_t acc_len, const PinBuf& res, double volume, bool packed, double w,
+             cm_raw* out)
 {
-    constexpr size_t PIECE = (size_t)4 << 20;
-    const int nt = (int)std::min<size_t>(8, bytes / PIECE);
-    if (nt < 2) {
-        memcpy(dst, src, bytes);
-        return;
-    }
-    const size_t step = (bytes / nt + 63) & ~(size_t)63;
-    std::vector<std::thread> th;
-    for (int t = 1; t < nt; t++) {
-        const size_t o = t * step;
-        if (o >= bytes) break;
-        th.emplace_back([=] { memcpy((char*)dst + o, (const char*)src + o, std::min(step, bytes - o)); });
-    }
-    memcpy(dst, src, std::min(step, bytes));
-    for (auto& x : th) x.join();
-}
-
-// dst[k] = (double)src[o + k] * w (w/2 for entries o + k in [half_lo,
-// half_hi)), split over a few threads like copy_par
-void conv_par(double* dst, const uint32_t* src, int64_t o, int64_t n, int64_t half_lo, int64_t half_hi, double w)
-{
-    const double wh = w / 2;
-    auto run = [=](int64_t a, int64_t b) {
-        for (int64_t k = a; k < b; k++) dst[k] = (double)src[o + k] * ((o + k >= half_lo && o + k < half_hi) ? wh : w);
-    };
-    constexpr int64_t PIECE = (int64_t)1 << 19;
-    const int nt = (int)std::min<int64_t>(8, n / PIECE);
-    if (nt < 2) {
-        run(0, n);
-        return;
-    }
-    const int64_t step = (n + nt - 1) / nt;
-    std::vector<std::thread> th;
-    for (int t = 1; t < nt; t++) {
-        const int64_t a = t * step;
-        if (a >= n) break;
-        th.emplace_back(run, a, std::min(n, a + step));
-    }
-    run(0, std::min(n, step));
-    for (auto& x : th) x.join();
-}
-
-// raw accumulators of a collected request into the caller's cm_raw: FP64
-// copies, or the products (double)hits * w of a packed read-back
-void fill_raw(const cm_job_desc& d, int64_t acc_len, const PinBuf& res, double volume, bool packed, double w,
-              cm_raw* out)
-{
-    const int64_t nb = d.nbins, ngu = d.n_groups_u, ngv = d.n_groups_v;
-    const int64_t half_lo = d.autocorrelation ? 4 * nb : 0, half_hi = d.autocorrelation ? acc_len : 0;
-    const ull* hits = (const ull*)((const char*)res.p + 8 * acc_len);
-    auto cp = [&](double* dst, int64_t o, int64_t n) {
-        if (!dst) return;
-        if (packed) conv_par(dst, (const uint32_t*)res.p, o, n, half_lo, half_hi, w);
-        else copy_par(dst, (const double*)res.p + o, 8 * (size_t)n);
-    };
-    cp(out->md_count, 0, nb);
-    cp(out->md_count_random, nb, nb);
-    cp(out->rdf_count, 2 * nb, nb);
-    cp(out->rdf_count_random, 3 * nb, nb);
-    const int64_t g = 4 * nb;
-    cp(out->solute_group_count, g, ngu * nb);
-    cp(out->solvent_group_count, g + ngu * nb, ngv * nb);
-    cp(out->solute_group_count_random, g + (ngu + ngv) * nb, ngu * nb);
-    cp(out->solvent_group_count_random, g + (2 * ngu + ngv) * nb, ngv * nb);
+    const RbLayout L(acc_len, d.nbins);
+    const int64_t half_lo = d.autocorrelation ? L.nh : 0, half_hi = d.autocorrelation ? acc_len : 0;
+    if (int r = copy_out(d, L, (const char*)res.p, packed, w, half_lo, half_hi, 1, nullptr, out)) return r;
     out->volume_total = volume;
-    auto cpu = [&](uint64_t* dst, const ull* src) { if (dst) memcpy(dst, src, 8 * nb); };
-    cpu(out->hits_md, hits);
-    cpu(out->hits_md_random, hits + nb);
-    cpu(out->hits_rdf, hits + 2 * nb);
-    cpu(out->hits_rdf_random, hits + 3 * nb);
+    return CM_OK;
 }
 
 // Times of the batches profiled since the last call: waits for the job's
@@ -4830,9 +4853,9 @@ int cm_server_wait(cm_server* S, uint64_t ticket, cm_raw* out, cm_request_times*
         S->reqs.erase(ticket);
     }
     if (times) *times = q->t;
-    const int st = q->status;
+    int st = q->status;
     if (st == CM_OK) {
-        if (out) fill_raw(q->d, q->acc_len, q->res, q->volume, q->packed, q->wt, out);
+        if (out) st = fill_raw(q->d, q->acc_len, q->res, q->volume, q->packed, q->wt, out);
     } else {
         g_err = q->err;
     }

@@ -148,9 +148,10 @@ int cm_job_submit_device(cm_job *job, int32_t nframes, const int64_t *frame_inde
 
 /* Waits for the job's work and writes the raw accumulators (the chunk sum!
  * of src/mddf.jl:336).  The job may not be submitted to afterwards.  When
- * every frame had the same weight, the hit counts cross PCIe as 32-bit words
- * and the FP64 values are formed on the host (the same products, bit for
- * bit); otherwise the FP64 accumulators are read back. */
+ * every frame had the same weight, the group counts cross PCIe as 16-bit
+ * words (larger counts listed beside them, the md/rdf histograms as 64-bit
+ * hits) and the FP64 values are formed on the host (the same products, bit
+ * for bit); otherwise the FP64 accumulators are read back. */
 int cm_job_finish(cm_job *job, cm_raw *out);
 
 /* Frames of one job for cm_jobs_submit_device: the arguments of
@@ -242,9 +243,10 @@ const char *cm_last_error(void);
  * returns them to the system.  No counterpart in the reference. */
 int cm_release_cached_memory(void);
 
-/* Largest hit count cm_job_finish sends as a 32-bit word (default and maximum
- * 2^32 - 1); a job with a larger count is read back as FP64 accumulators.  A
- * lower limit forces that path, for tests.  No counterpart in the reference. */
+/* Largest hit count cm_job_finish and cm_server_wait read back in packed form
+ * (default and maximum 2^32 - 1); a job with a larger count is read back as
+ * FP64 accumulators.  A lower limit forces that path, for tests.  No
+ * counterpart in the reference. */
 int cm_set_readback_pack_limit(uint64_t limit);
 
 /* Device memory for callers that keep trajectories resident in HBM. */

@@ -632,3 +632,31 @@ def test_packed_and_fp64_readback_agree(pkg, oracle, self_mddf):
         if w == 0.75:
             assert_group_split(packed, self_mddf)
         assert packed["md_count"].sum() > 0
+
+
+def test_packed_readback_overflow_list(pkg, oracle):
+    """Group counts of 65,535 and more go out beside the 16-bit words of the
+    packed read-back: a point solute with 5 A bins and 600 ideal-gas passes
+    puts about 70,000 hits in one solute-group bin.  The packed read-back
+    equals the FP64 one and the oracle."""
+    from complexmixtures_amd import _lib
+    S = syn()
+    rng = np.random.default_rng(61)
+    box, n, nf = 30.0, 900, 60
+    x = S.place_molecules(rng, S.water_template(), rng.uniform(0, box, (n, 3)))
+    xv = np.stack([x + rng.normal(scale=0.3, size=x.shape) for _ in range(nf)])
+    xu = np.full((nf, 1, 3), box / 2)
+    su = pkg.AtomSelection([1], nmols=1)
+    sv = pkg.AtomSelection(list(range(2, 3 * n + 2)), natomspermol=3)
+    traj = pkg.MemoryTrajectory(xu, xv, np.repeat(np.diag([box] * 3)[None], nf, axis=0), su, sv)
+    opts = pkg.Options(binstep=5.0, n_random_samples=10, seed=62, silent=True)
+    packed = gpu_raw(pkg, traj, opts)
+    assert packed["solute_group_count_random"].max() >= 65535
+    _lib.set_readback_pack_limit(0)
+    try:
+        wide = gpu_raw(pkg, traj, opts)
+    finally:
+        _lib.set_readback_pack_limit()
+    assert_raw_equal(packed, wide)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    assert_raw_equal(a, packed)
