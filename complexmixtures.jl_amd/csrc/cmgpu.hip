@@ -1862,20 +1862,27 @@ __device__ __forceinline__ float molecule_bound(const Geo& g, const BoundLoads& 
 {
     const int c0 = cell % g.nc[0], c1 = (cell / g.nc[0]) % g.nc[1], c2 = cell / (g.nc[0] * g.nc[1]);
     const float* H = g.H32;
+    const float i0 = 1.0f / g.nc[0], i1 = 1.0f / g.nc[1], i2 = 1.0f / g.nc[2];
+    // centre -> cell centre, and the half-cell steps along the three cell
+    // vectors: centre -> corner (a, b, c) = e + (2a - 1) h0 + (2b - 1) h1 + (2c - 1) h2
+    const float d0 = (float)fr[0] - (c0 + 0.5f) * i0, d1 = (float)fr[1] - (c1 + 0.5f) * i1,
+                d2 = (float)fr[2] - (c2 + 0.5f) * i2;
+    const float ex = H[0] * d0 + H[3] * d1 + H[6] * d2;
+    const float ey = H[1] * d0 + H[4] * d1 + H[7] * d2;
+    const float ez = H[2] * d0 + H[5] * d1 + H[8] * d2;
+    const float h0x = 0.5f * i0 * H[0], h0y = 0.5f * i0 * H[1], h0z = 0.5f * i0 * H[2];
+    const float h1x = 0.5f * i1 * H[3], h1y = 0.5f * i1 * H[4], h1z = 0.5f * i1 * H[5];
+    const float h2x = 0.5f * i2 * H[6], h2y = 0.5f * i2 * H[7], h2z = 0.5f * i2 * H[8];
     float best = INFINITY;
+    if (L.e[0] != 0u) best = __uint_as_float(0xffffffffu - L.e[0]) + sqrtf(ex * ex + ey * ey + ez * ez);
 #pragma unroll
-    for (int t = 0; t < 9; t++) {
-        if (L.e[t] == 0u) continue;
-        const float D = __uint_as_float(0xffffffffu - L.e[t]);
-        // the reference point's fractional coordinates
-        const float q0 = t == 0 ? (c0 + 0.5f) / g.nc[0] : (float)(c0 + ((t - 1) & 1)) / g.nc[0];
-        const float q1 = t == 0 ? (c1 + 0.5f) / g.nc[1] : (float)(c1 + (((t - 1) >> 1) & 1)) / g.nc[1];
-        const float q2 = t == 0 ? (c2 + 0.5f) / g.nc[2] : (float)(c2 + (((t - 1) >> 2) & 1)) / g.nc[2];
-        const float d0 = (float)fr[0] - q0, d1 = (float)fr[1] - q1, d2 = (float)fr[2] - q2;
-        const float ex = H[0] * d0 + H[3] * d1 + H[6] * d2;
-        const float ey = H[1] * d0 + H[4] * d1 + H[7] * d2;
-        const float ez = H[2] * d0 + H[5] * d1 + H[8] * d2;
-        best = fminf(best, D + sqrtf(ex * ex + ey * ey + ez * ez));
+    for (int t = 0; t < 8; t++) {
+        if (L.e[1 + t] == 0u) continue;
+        const float sa = (t & 1) ? 1.0f : -1.0f, sb = (t & 2) ? 1.0f : -1.0f, sc = (t & 4) ? 1.0f : -1.0f;
+        const float qx = ex - sa * h0x - sb * h1x - sc * h2x;
+        const float qy = ey - sa * h0y - sb * h1y - sc * h2y;
+        const float qz = ez - sa * h0z - sb * h1z - sc * h2z;
+        best = fminf(best, __uint_as_float(0xffffffffu - L.e[1 + t]) + sqrtf(qx * qx + qy * qy + qz * qz));
     }
     if (!(best < INFINITY)) return INFINITY;
     return (best + __uint_as_float(g.rv_bits)) * (1.0f + 1e-5f) + 1e-2f;
