@@ -660,3 +660,24 @@ def test_packed_readback_overflow_list(pkg, oracle):
     assert_raw_equal(packed, wide)
     a, _ = oracle_raw(pkg, oracle, traj, opts)
     assert_raw_equal(a, packed)
+
+
+def test_large_cell_grid_direct_near_marks(pkg, oracle):
+    """A 330 A box with 5 A cells has 287,496 cells, more than the LDS bitmap
+    of the near-cell marks holds: the marks go straight to memory.  Counts
+    against the oracle, ideal-gas passes included."""
+    S = syn()
+    rng = np.random.default_rng(63)
+    box, n, nf = 330.0, 3000, 2
+    xu0 = box / 2 + rng.normal(scale=4.0, size=(60, 3))
+    x = S.place_molecules(rng, S.water_template(), rng.uniform(0, box, (n, 3)))
+    xu = np.stack([xu0 + rng.normal(scale=0.2, size=xu0.shape) for _ in range(nf)])
+    xv = np.stack([x + rng.normal(scale=0.2, size=x.shape) for _ in range(nf)])
+    su = pkg.AtomSelection(list(range(1, 61)), nmols=1)
+    sv = pkg.AtomSelection(list(range(61, 61 + 3 * n)), natomspermol=3)
+    traj = pkg.MemoryTrajectory(xu, xv, np.repeat(np.diag([box] * 3)[None], nf, axis=0), su, sv)
+    opts = pkg.Options(n_random_samples=40, seed=64, silent=True)
+    a, _ = oracle_raw(pkg, oracle, traj, opts)
+    b = gpu_raw(pkg, traj, opts, job_kw=dict(tuning=dict(cell_fraction=0.5, columns=4)))
+    assert_raw_equal(a, b)
+    assert b["md_count_random"].sum() > 0
