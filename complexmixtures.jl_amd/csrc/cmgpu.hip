@@ -1580,6 +1580,48 @@ struct MolDst {   // placed atoms: FP64 (exact resolve) and FP32 (evaluation)
     }
 };
 
+// Device memory addressed as the global address space.  A pointer read from
+// memory (the job table of the multi-job stages) is otherwise a generic one,
+// and its loads and stores compile to flat instructions, which wait on the
+// scalar-load counter too and take 64-bit VGPR addresses; through these they
+// are global ones, and scalar loads where the address is uniform.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <class T>
+__device__ __forceinline__ T ldg(const T* p)
+{
+    return *(gptr<const T>)p;
+}
+
+// A frame's geometry by value, read through the constant address space (it
+// is not written while the reading kernel runs): uniform addresses give
+// scalar loads.  Not a template: the memcpy builtin is not available to the
+// device side of function templates.
+__device__ __forceinline__ Geo ldc_geo(const Geo* p)
+{
+    Geo g;
+    __builtin_memcpy(&g, (__attribute__((address_space(4))) const Geo*)p, sizeof(Geo));
+    return g;
+}
+
+template <class TV>
+struct MolSrcG {   // MolSrc through a global pointer
+    gptr<const TV> p;
+    __device__ double operator()(int a, int k) const { return (double)p[3 * a + k]; }
+};
+
+struct MolDstG {   // MolDst through global pointers
+    gptr<double> p;
+    gptr<f4v> w;
+    __device__ void operator()(int a, double x, double y, double z) const
+    {
+        p[3 * a] = x; p[3 * a + 1] = y; p[3 * a + 2] = z;
+        w[a] = f4v{(float)x, (float)y, (float)z, 0.f};
+    }
+};
+
 // randomize_solvent!, src/mddf.jl:65-88, for every slot of every random pass
 template <class TV>
 __global__ void k_place(KP P, Bufs B, const TV* xv)
@@ -1843,8 +1885,8 @@ struct BoundLoads {
 __device__ __forceinline__ BoundLoads bound_loads(const KP& P, const Bufs& B, const Geo& g, int f, int cell)
 {
     BoundLoads L;
-    const unsigned* d = B.dfield + (int64_t)f * P.cap;
-    const unsigned* dc = d + (int64_t)P.F * P.cap;
+    const gptr<const unsigned> d = (gptr<const unsigned>)(B.dfield + (int64_t)f * P.cap);
+    const gptr<const unsigned> dc = d + (int64_t)P.F * P.cap;
     L.e[0] = d[cell];
     const int c0 = cell % g.nc[0], c1 = (cell / g.nc[0]) % g.nc[1], c2 = cell / (g.nc[0] * g.nc[1]);
 #pragma unroll
@@ -2116,16 +2158,17 @@ __device__ __forceinline__ void b_place_near(KP P, Bufs B, const TV* xv, int bx_
         const int f = fk / P.K;
         const int k = fk - f * P.K;
         int64_t fp = (int64_t)f * P.NP + 1 + k;
-        if (pos >= B.nnear[fp]) continue;
-        const int v = B.msorted[fp * P.Mv + pos];
-        const Geo& g = B.geo[f];
+        if (pos >= ldg(B.nnear + fp)) continue;
+        const int v = ldg(B.msorted + fp * P.Mv + pos);
+        const Geo g = ldc_geo(B.geo + f);
         cm::u32x4 r = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT}, P.key0, P.key1);
         cm::u32x4 r2 = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT + 1u}, P.key0, P.key1);
         int64_t lk = (int64_t)f * P.kb;
-        int nb = B.nbulk[lk];
-        int64_t jmol = nb > 0 ? (int64_t)B.blist[lk * P.Mv + cm::draw_index(r.x, nb)] : cm::draw_index(r.x, P.Mv);
-        MolSrc<TV> src{xv + (int64_t)f * P.Nv * 3 + jmol * P.nv * 3};
-        MolDst dst{B.P + (fk * P.Nv + (int64_t)pos * P.nv) * 3, B.vw_r + fk * P.Nv + (int64_t)pos * P.nv};
+        int nb = ldg(B.nbulk + lk);
+        int64_t jmol = nb > 0 ? (int64_t)ldg(B.blist + lk * P.Mv + cm::draw_index(r.x, nb)) : cm::draw_index(r.x, P.Mv);
+        MolSrcG<TV> src{(gptr<const TV>)(xv + (int64_t)f * P.Nv * 3 + jmol * P.nv * 3)};
+        MolDstG dst{(gptr<double>)(B.P + (fk * P.Nv + (int64_t)pos * P.nv) * 3),
+                    (gptr<f4v>)(B.vw_r + fk * P.Nv + (int64_t)pos * P.nv)};
         double uc[3] = {cm::u01(r.y), cm::u01(r.z), cm::u01(r.w)};
         double ua[3] = {cm::u01(r2.x), cm::u01(r2.y), cm::u01(r2.z)};
         // the placed centre is H uc (a rigid copy of a molecule of the frame):
@@ -2134,7 +2177,7 @@ __device__ __forceinline__ void b_place_near(KP P, Bufs B, const TV* xv, int bx_
         cell_of_frac(g, uc[0], uc[1], uc[2], cell);
         const float ub = molecule_bound(g, bound_loads(P, B, g, f, cell), cell, uc);
         cm::random_move(src, dst, P.nv, P.iref, g.cell, uc, ua);
-        dst.w[0].w = ub;
+        ((gptr<float>)dst.w)[3] = ub;
     }
 }
 
