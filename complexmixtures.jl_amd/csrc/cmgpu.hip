@@ -166,13 +166,39 @@ struct Bufs {
     int* mstart;             // [F][NP][kcap+1]
     int* msorted;            // [F][NP][Mv]
     int* nnear;              // [F][NP]
-    int* chunk_base;         // [F*npass + 1] chunk offsets of one launch
+    int* chunk_base;         // [F*npass + 2] chunk offsets of one launch, total, evaluation chunk counter
     // accumulators
     ull* cnt_md; ull* cnt_rdf; ull* cnt_md_r; ull* cnt_rdf_r;
     ull* cnt_gu; ull* cnt_gv; ull* cnt_gu_r; ull* cnt_gv_r;
     ull* stats;              // [0] exhaustive resolves, [2] geometry error bits of the job's frames
     VRec* vrec;              // verification: [F][NP][Mv] records of every pass (null when off)
 };
+
+// Device memory addressed as the global address space.  A pointer read from
+// memory (the job table of the multi-job stages) is otherwise a generic one,
+// and its loads and stores compile to flat instructions, which wait on the
+// scalar-load counter too and take 64-bit VGPR addresses; through these they
+// are global ones, and scalar loads where the address is uniform.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <class T>
+__device__ __forceinline__ T ldg(const T* p)
+{
+    return *(gptr<const T>)p;
+}
+
+// A frame's geometry by value, read through the constant address space (it
+// is not written while the reading kernel runs): uniform addresses give
+// scalar loads.  Not a template: the memcpy builtin is not available to the
+// device side of function templates.
+__device__ __forceinline__ Geo ldc_geo(const Geo* p)
+{
+    Geo g;
+    __builtin_memcpy(&g, (__attribute__((address_space(4))) const Geo*)p, sizeof(Geo));
+    return g;
+}
 
 struct Rec {
     int within;
@@ -407,7 +433,7 @@ __device__ __forceinline__ void b_wrap_atoms(KP P, Bufs B, const T* xu, int bx_,
     for (int64_t it = bx_ * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)nbx_ * blockDim.x) {
         int f = (int)(it / P.nu);
         int i = (int)(it - (int64_t)f * P.nu);
-        const Geo& g = B.geo[f];
+        const Geo g = ldc_geo(B.geo + f);
         double x, y, z;
         load3(xu + (int64_t)f * P.Nu * 3, i, x, y, z);
         double fr[3], n[3];
@@ -1580,32 +1606,6 @@ struct MolDst {   // placed atoms: FP64 (exact resolve) and FP32 (evaluation)
     }
 };
 
-// Device memory addressed as the global address space.  A pointer read from
-// memory (the job table of the multi-job stages) is otherwise a generic one,
-// and its loads and stores compile to flat instructions, which wait on the
-// scalar-load counter too and take 64-bit VGPR addresses; through these they
-// are global ones, and scalar loads where the address is uniform.
-template <class T>
-using gptr = __attribute__((address_space(1))) T*;
-typedef float f4v __attribute__((ext_vector_type(4)));
-
-template <class T>
-__device__ __forceinline__ T ldg(const T* p)
-{
-    return *(gptr<const T>)p;
-}
-
-// A frame's geometry by value, read through the constant address space (it
-// is not written while the reading kernel runs): uniform addresses give
-// scalar loads.  Not a template: the memcpy builtin is not available to the
-// device side of function templates.
-__device__ __forceinline__ Geo ldc_geo(const Geo* p)
-{
-    Geo g;
-    __builtin_memcpy(&g, (__attribute__((address_space(4))) const Geo*)p, sizeof(Geo));
-    return g;
-}
-
 template <class TV>
 struct MolSrcG {   // MolSrc through a global pointer
     gptr<const TV> p;
@@ -1667,6 +1667,16 @@ constexpr int TCAP_DEFAULT = 640;   // staged solute atoms per round (10 KiB of 
 constexpr int RB_PIECES = 8;        // read-back pieces of cm_job_finish
 constexpr int TCAP_MOL = 512;       // the same with the squared norms (20 B each: 10 KiB)
 constexpr int CBCAP = 1024;         // chunk-table entries kept in LDS
+// The last DYN_PCT per cent of an evaluation launch's chunks are handed out
+// on request, in launches of at least DYN_MIN chunks per workgroup, in about
+// DYN_GRABS requests per workgroup (consecutive chunks per request).  On the
+// 200-frame globule job and 64 uneven systems (random-pass launch time):
+// 35% in requests of one chunk -9%, of two (8 requests) -17%, of four -12%;
+// 25% -11%, 45% (two per request) -17%, 55% and more in single chunks slower
+// than a fixed stride alone.
+constexpr int DYN_PCT = 35;
+constexpr int DYN_MIN = 2;
+constexpr int DYN_GRABS = 8;
 constexpr int SU = 4;               // staging: atoms per lane per step
 constexpr int SEG = 8;              // staging: atoms of a row per lane (segment)
 
@@ -1736,7 +1746,7 @@ __device__ __forceinline__ void b_near_mark(KP P, Bufs B, int bx_, int nbx_, int
 {
     __shared__ unsigned bits[NEAR_WORDS];
     for (int f = by_; f < P.F; f += nby_) {
-        const Geo& g = B.geo[f];
+        const Geo g = ldc_geo(B.geo + f);
         const int ncell = g.nc[0] * g.nc[1] * g.nc[2];
         const bool lds = ncell <= NEAR_WORDS * 32;
         const int nw = (ncell + 31) >> 5;
@@ -1811,7 +1821,7 @@ __device__ __forceinline__ void b_dfield(KP P, Bufs B, int bx_, int nbx_, int by
         if (nearc && j != 7) {
             const int f = (int)(it / P.cap);
             const int cell = (int)(it - (int64_t)f * P.cap);
-            const Geo& g = B.geo[f];
+            const Geo g = ldc_geo(B.geo + f);
             const int cx = cell % g.nc[0], cy = (cell / g.nc[0]) % g.nc[1], cz = cell / (g.nc[0] * g.nc[1]);
             int ax, ay, az;
             float fx, fy, fz;
@@ -1939,7 +1949,7 @@ __device__ __forceinline__ void b_prep_solvent_tiles(KP P, Bufs B, const T* xv, 
     for (int64_t it = bx_ * (int64_t)blockDim.x + threadIdx.x; it < total; it += (int64_t)nbx_ * blockDim.x) {
         int f = (int)(it / P.Mv);
         int v = (int)(it - (int64_t)f * P.Mv);
-        const Geo& g = B.geo[f];
+        const Geo g = ldc_geo(B.geo + f);
         double c[3];
         mol_centre(g, xv + (int64_t)f * P.Nv * 3, (int64_t)v * P.nv, P.nv, c);
         double fr[3], n[3];
@@ -1972,7 +1982,7 @@ __device__ __forceinline__ void b_fill_real(KP P, Bufs B, const T* xv, int bx_, 
         const int64_t fp = (int64_t)f * P.NP;
         if (pos >= B.nnear[fp]) continue;
         const int v = B.msorted[fp * P.Mv + pos];
-        const Geo& g = B.geo[f];
+        const Geo g = ldc_geo(B.geo + f);
         const T* x = xv + (int64_t)f * P.Nv * 3;
         int64_t a0 = (int64_t)v * P.nv;
         double c[3];
@@ -2110,7 +2120,10 @@ __device__ __forceinline__ void b_chunk_table(KP P, Bufs B, int p0, int np, int 
         if (threadIdx.x == 0) carry += tot;
         __syncthreads();
     }
-    if (threadIdx.x == 0) B.chunk_base[n] = carry;
+    if (threadIdx.x == 0) {
+        B.chunk_base[n] = carry;
+        B.chunk_base[n + 1] = 0;   // the evaluation's chunk counter
+    }
 }
 
 __global__ void __launch_bounds__(1024) k_chunk_table(KP P, Bufs B, int p0, int np)
@@ -2129,7 +2142,7 @@ __device__ __forceinline__ void b_place_keys(KP P, Bufs B, int bx_, int nbx_, in
     for (int fk = by_; fk < P.F * P.K; fk += nby_) {
         const int f = fk / P.K;
         const int k = fk - f * P.K;
-        const Geo& g = B.geo[f];
+        const Geo g = ldc_geo(B.geo + f);
         int64_t fp = (int64_t)f * P.NP + 1 + k;
         cm::u32x4 r = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT}, P.key0, P.key1);
         int cell;
@@ -2248,12 +2261,20 @@ __device__ __forceinline__ void col_reduce(Best& b)
 // NVM pairs) and combines them in the lane.
 template <class TU, class TV, bool RANDOM, int C, int NVM, bool VERIFY>
 __device__ __forceinline__ void eval_tiles_chunks(const KP& P, const Bufs& B, const TU* xu, const TV* xv, int ch0, int ch1,
-                                                  int chs, float4* stg, float* stw, const int* cbs, bool cb_lds)
+                                                  int chs, float4* stg, float* stw, const int* cbs, bool cb_lds,
+                                                  int* ctr)
 {
-    // chunks ch0, ch0 + chs, ... below ch1 of one job; LDS: the staging
-    // buffer stg (P.tcap float4: atom, index; relative to the chunk centre
-    // when NVM > 0), with NVM > 0 the squared norms stw of the staged atoms,
-    // and (cb_lds) the job's chunk table cbs
+    // chunks of one job below ch1: ch0, ch0 + chs, ... below tdyn, then
+    // (with a counter ctr, zero at the start of the launch) the chunks from
+    // tdyn on in the order the workgroups ask for them, so the ones that
+    // finish early take more of the last chunks.  The chunks' costs vary
+    // with their staged atoms, and a fixed stride alone leaves a tail of
+    // workgroups still busy; requests go to one device-wide counter, so
+    // only the last DYN_PCT per cent are handed out that way, a few
+    // consecutive chunks per request.  LDS: the
+    // staging buffer stg (P.tcap float4: atom, index; relative to the chunk
+    // centre when NVM > 0), with NVM > 0 the squared norms stw of the staged
+    // atoms, and (cb_lds) the job's chunk table cbs
     const int tcap = P.tcap;
     // staging ranks per lane per step (fewer with a molecule per lane, whose
     // evaluation holds more registers)
@@ -2267,7 +2288,20 @@ __device__ __forceinline__ void eval_tiles_chunks(const KP& P, const Bufs& B, co
     ull* gv = RANDOM ? B.cnt_gv_r : B.cnt_gv;
     ull* gmd = RANDOM ? B.cnt_md_r : B.cnt_md;
     ull* grdf = RANDOM ? B.cnt_rdf_r : B.cnt_rdf;
-    for (int ch = ch0; ch < ch1; ch += chs) {
+    const int ndyn = (int)((int64_t)ch1 * DYN_PCT / 100);
+    const int tdyn = ctr && ch1 >= DYN_MIN * chs ? ch1 - ndyn : ch1;
+    const int dgrab = max(1, ndyn / (chs * DYN_GRABS));   // chunks per request
+    int dnext = 0, dend = 0;
+    auto next_chunk = [&](int c) {
+        if (c < tdyn || !ctr) return c;
+        if (dnext < dend) return dnext++;
+        int g = 0;
+        if (lane == 0) g = atomicAdd(ctr, dgrab);
+        dnext = tdyn + __builtin_amdgcn_readlane(g, 0);
+        dend = dnext + dgrab;
+        return dnext++;
+    };
+    for (int ch = next_chunk(ch0); ch < ch1; ch = next_chunk(ch + chs)) {
         int lo_ = 0, hi_ = nfp;
         while (hi_ - lo_ > 1) {
             int mid = (lo_ + hi_) >> 1;
@@ -2742,7 +2776,7 @@ __global__ void __launch_bounds__(64, 3) k_eval_tiles(KP P, Bufs B, const TU* xu
         for (int k = threadIdx.x; k <= nfp; k += 64) cbs[k] = B.chunk_base[k];
     __syncthreads();
     eval_tiles_chunks<TU, TV, RANDOM, C, NVM, VERIFY>(P, B, xu, xv, blockIdx.x, B.chunk_base[nfp], gridDim.x, stg, stw,
-                                                     cbs, cb_lds);
+                                                     cbs, cb_lds, B.chunk_base + nfp + 1);
 }
 
 // Fills n16 16-byte words and then ntail single bytes with zeros (a kernel,
@@ -3020,7 +3054,7 @@ __global__ void km_geo_flags(const MultiTab* __restrict__ MT)
 // (read from their chunk tables at the start of the launch; jobs without
 // random passes have none in the random launch).
 template <class TU, class TV, bool RANDOM, int C, int NVM>
-__global__ void __launch_bounds__(64, 3) km_eval_tiles(const MultiTab* __restrict__ MT)
+__global__ void __launch_bounds__(64, 3) km_eval_tiles(const MultiTab* __restrict__ MT, int* ctr)
 {
     extern __shared__ float4 dyn[];
     __shared__ int cb[MS_MAX + 1];
@@ -3037,8 +3071,24 @@ __global__ void __launch_bounds__(64, 3) km_eval_tiles(const MultiTab* __restric
     }
     __syncthreads();
     const int total = __builtin_amdgcn_readfirstlane(cb[n]);
+    // a fixed stride, then the last DYN_PCT per cent of the chunks on request
+    // from the counter ctr (zero at the start of the launch), as in the
+    // single-job evaluation; either way a block's chunks increase
+    const int ndyn = (int)((int64_t)total * DYN_PCT / 100);
+    const int tdyn = total >= DYN_MIN * (int)gridDim.x ? total - ndyn : total;
+    const int dgrab = max(1, ndyn / ((int)gridDim.x * DYN_GRABS));
+    int dnext = 0, dend = 0;
+    auto next_chunk = [&](int g) {
+        if (g < tdyn) return g;
+        if (dnext < dend) return dnext++;
+        int v = 0;
+        if (threadIdx.x == 0) v = atomicAdd(ctr, dgrab);
+        dnext = tdyn + __builtin_amdgcn_readlane(v, 0);
+        dend = dnext + dgrab;
+        return dnext++;
+    };
     int lo = 0;
-    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    for (int g = next_chunk(blockIdx.x); g < total; g = next_chunk(g + gridDim.x)) {
         while (cb[lo + 1] <= g) lo++;
         // uniform (the same in every lane): the job's parameters are scalar loads
         lo = __builtin_amdgcn_readfirstlane(lo);
@@ -3047,7 +3097,7 @@ __global__ void __launch_bounds__(64, 3) km_eval_tiles(const MultiTab* __restric
         float4* stg = dyn;
         float* stw = (float*)(dyn + S.P.tcap);
         eval_tiles_chunks<TU, TV, RANDOM, C, NVM, false>(S.P, S.B, (const TU*)S.xu, (const TV*)S.xv, ch, ch + 1, 1, stg,
-                                                         stw, nullptr, false);
+                                                         stw, nullptr, false, nullptr);
     }
 }
 
@@ -3833,7 +3883,7 @@ int alloc_batch(cm_job* J, int F, int64_t cap, int64_t kcap, int dtype_size, boo
         if ((r = ensure(J, J->mstart, (size_t)4 * F * NP * (kcap + 1)))) return r;
         if ((r = ensure(J, J->msorted, (size_t)4 * F * NP * Mv))) return r;
         if ((r = ensure(J, J->nnear, (size_t)4 * F * NP))) return r;
-        if ((r = ensure(J, J->chunk_base, (size_t)4 * (F * std::max(J->K, 1) + 1)))) return r;
+        if ((r = ensure(J, J->chunk_base, (size_t)4 * (F * std::max(J->K, 1) + 2)))) return r;
     }
     if (J->K > 0) {
         if ((r = ensure(J, J->P, (size_t)24 * F * J->K * J->Nv))) return r;
@@ -4968,7 +5018,10 @@ MultiRunner* multi_runners()
     return r;
 }
 
-constexpr size_t MT_GEO_OFF = (sizeof(MultiTab) + 255) & ~(size_t)255;
+// upload layout: the table, the two evaluation launches' chunk counters
+// (zero in every upload), the frames' Geo records
+constexpr size_t MT_CTR_OFF = (sizeof(MultiTab) + 255) & ~(size_t)255;
+constexpr size_t MT_GEO_OFF = MT_CTR_OFF + 256;
 
 int multi_init(MultiRunner& R)
 {
@@ -5022,14 +5075,15 @@ void multi_launch(hipStream_t s, const MultiTab* tab, int tcap, int cus)
     hipLaunchKernelGGL(km_fill_real<T>, dim3(MG_256), dim3(BLOCK), 0, s, tab);
     hipLaunchKernelGGL(km_chunk_table<false>, dim3(MG_1024), dim3(1024), 0, s, tab);
     hipLaunchKernelGGL(km_bulk_init, dim3(MG_256), dim3(BLOCK), 0, s, tab);
-    hipLaunchKernelGGL(ev_real, dim3(g_real), dim3(64), shm, s, tab);
+    int* ctr = (int*)((char*)tab + MT_CTR_OFF);
+    hipLaunchKernelGGL(ev_real, dim3(g_real), dim3(64), shm, s, tab, ctr);
     hipLaunchKernelGGL(km_bulk_compact, dim3(MG_1024), dim3(1024), 0, s, tab);
     hipLaunchKernelGGL(km_place_keys, dim3(MG_256), dim3(BLOCK), 0, s, tab);
     hipLaunchKernelGGL(km_scan_rows<true>, dim3(MG_1024), dim3(1024), 0, s, tab);
     hipLaunchKernelGGL(km_mol_scatter<true>, dim3(MG_256), dim3(BLOCK), 0, s, tab);
     hipLaunchKernelGGL(km_place_near<T>, dim3(MG_256), dim3(BLOCK), 0, s, tab);
     hipLaunchKernelGGL(km_chunk_table<true>, dim3(MG_1024), dim3(1024), 0, s, tab);
-    hipLaunchKernelGGL(ev_rand, dim3(g_rand), dim3(64), shm, s, tab);
+    hipLaunchKernelGGL(ev_rand, dim3(g_rand), dim3(64), shm, s, tab, ctr + 1);
     hipLaunchKernelGGL(km_geo_flags, dim3(MG_64), dim3(64), 0, s, tab);
 }
 
@@ -5116,6 +5170,7 @@ int multi_run_group(MultiRunner& R, std::vector<MJob>& grp, int dtype)
             Geo* gd = (Geo*)((char*)R.tab + MT_GEO_OFF);
             T->nsys = n;
             T->pad = 0;
+            memset((char*)R.htab[R.cur] + MT_CTR_OFF, 0, MT_GEO_OFF - MT_CTR_OFF);
             int64_t goff = 0;
             int cnt[MS_N];
             for (int st = 0; st < MS_N; st++) T->base[st][0] = 0;
