@@ -308,10 +308,15 @@ def test_exact_ties(pkg, oracle):
 
 @pytest.mark.parametrize("tuning", [dict(cell_fraction=0.5, columns=1), dict(cell_fraction=0.3, columns=2),
                                     dict(cell_fraction=1.0, columns=4), dict(cell_fraction=0.25, columns=8),
-                                    dict(cell_fraction=0.2, columns=1, eval_workgroups=7)])
+                                    dict(cell_fraction=0.2, columns=1, eval_workgroups=7),
+                                    dict(cell_fraction=0.5, columns=4, eval_workgroups=1),
+                                    dict(cell_fraction=0.35, columns=2, eval_workgroups=13)])
 def test_tuning_does_not_change_results(pkg, oracle, tuning):
     """Cell size, columns and grid of the single-solute kernels change the work
-    decomposition, never the records."""
+    decomposition, never the records.  Small grids give each workgroup many
+    chunks, so the chunks handed out on request at the end of a launch (several
+    per request) are most of the work: every chunk must still be evaluated
+    exactly once."""
     s = syn().config1(nframes=2, n_solute=300, n_water=1200, box=30.0, seed=41)
     traj = mem(pkg, s)
     opts = pkg.Options(dbulk=9.0, n_random_samples=4, seed=6, silent=True)
