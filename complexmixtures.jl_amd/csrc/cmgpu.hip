@@ -2271,10 +2271,10 @@ __device__ __forceinline__ void eval_tiles_chunks(const KP& P, const Bufs& B, co
     // with their staged atoms, and a fixed stride alone leaves a tail of
     // workgroups still busy; requests go to one device-wide counter, so
     // only the last DYN_PCT per cent are handed out that way, a few
-    // consecutive chunks per request.  LDS: the
-    // staging buffer stg (P.tcap float4: atom, index; relative to the chunk
-    // centre when NVM > 0), with NVM > 0 the squared norms stw of the staged
-    // atoms, and (cb_lds) the job's chunk table cbs
+    // consecutive chunks per request.
+    // LDS: the staging buffer stg (P.tcap float4: atom, index; relative to
+    // the chunk centre when NVM > 0), with NVM > 0 the squared norms stw of
+    // the staged atoms, and (cb_lds) the job's chunk table cbs
     const int tcap = P.tcap;
     // staging ranks per lane per step (fewer with a molecule per lane, whose
     // evaluation holds more registers)
@@ -2293,7 +2293,7 @@ __device__ __forceinline__ void eval_tiles_chunks(const KP& P, const Bufs& B, co
     const int dgrab = max(1, ndyn / (chs * DYN_GRABS));   // chunks per request
     int dnext = 0, dend = 0;
     auto next_chunk = [&](int c) {
-        if (c < tdyn || !ctr) return c;
+        if (c < tdyn || tdyn == ch1) return c;
         if (dnext < dend) return dnext++;
         int g = 0;
         if (lane == 0) g = atomicAdd(ctr, dgrab);
@@ -3079,7 +3079,7 @@ __global__ void __launch_bounds__(64, 3) km_eval_tiles(const MultiTab* __restric
     const int dgrab = max(1, ndyn / ((int)gridDim.x * DYN_GRABS));
     int dnext = 0, dend = 0;
     auto next_chunk = [&](int g) {
-        if (g < tdyn) return g;
+        if (g < tdyn || tdyn == total) return g;
         if (dnext < dend) return dnext++;
         int v = 0;
         if (threadIdx.x == 0) v = atomicAdd(ctr, dgrab);
