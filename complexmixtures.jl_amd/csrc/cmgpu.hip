@@ -1590,36 +1590,27 @@ __global__ void __launch_bounds__(1024) k_bulk_compact(KP P, Bufs B)
     b_bulk_compact(P, B, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
 }
 
+// The atoms of one source molecule (xyz interleaved) and the placed atoms
+// (FP64 for the exact resolve, FP32 for the evaluation), through global
+// pointers (also where the pointers were read from a job table).
 template <class TV>
-struct MolSrc {   // atoms of one source molecule, xyz interleaved
-    const TV* p;
-    __device__ double operator()(int a, int k) const { return (double)p[3 * a + k]; }
-};
-
-struct MolDst {   // placed atoms: FP64 (exact resolve) and FP32 (evaluation)
-    double* p;
-    float4* w;
-    __device__ void operator()(int a, double x, double y, double z) const
-    {
-        p[3 * a] = x; p[3 * a + 1] = y; p[3 * a + 2] = z;
-        w[a] = mk4((float)x, (float)y, (float)z, 0.f);
-    }
-};
-
-template <class TV>
-struct MolSrcG {   // MolSrc through a global pointer
+struct MolSrc {
     gptr<const TV> p;
+    __device__ MolSrc(const TV* q) : p((gptr<const TV>)q) {}
     __device__ double operator()(int a, int k) const { return (double)p[3 * a + k]; }
 };
 
-struct MolDstG {   // MolDst through global pointers
+struct MolDst {
     gptr<double> p;
     gptr<f4v> w;
+    __device__ MolDst(double* q, float4* v) : p((gptr<double>)q), w((gptr<f4v>)v) {}
     __device__ void operator()(int a, double x, double y, double z) const
     {
         p[3 * a] = x; p[3 * a + 1] = y; p[3 * a + 2] = z;
         w[a] = f4v{(float)x, (float)y, (float)z, 0.f};
     }
+    // the first atom's fourth FP32 component
+    __device__ void set_w0(float x) const { ((gptr<float>)w)[3] = x; }
 };
 
 // randomize_solvent!, src/mddf.jl:65-88, for every slot of every random pass
@@ -1653,8 +1644,8 @@ __global__ void k_place(KP P, Bufs B, const TV* xv)
         int64_t lk = (int64_t)f * P.kb + (P.kb == 1 ? 0 : k);
         int nb = B.nbulk[lk];
         int64_t jmol = nb > 0 ? (int64_t)B.blist[lk * P.Mv + cm::draw_index(r.x, nb)] : cm::draw_index(r.x, P.Mv);
-        MolSrc<TV> src{xv + (int64_t)f * P.Nv * 3 + jmol * P.nv * 3};
-        MolDst dst{B.P + (fk * P.Nv + (int64_t)v * P.nv) * 3, B.vw_r + fk * P.Nv + (int64_t)v * P.nv};
+        MolSrc<TV> src(xv + (int64_t)f * P.Nv * 3 + jmol * P.nv * 3);
+        MolDst dst(B.P + (fk * P.Nv + (int64_t)v * P.nv) * 3, B.vw_r + fk * P.Nv + (int64_t)v * P.nv);
         double ua[3] = {cm::u01(r2.x), cm::u01(r2.y), cm::u01(r2.z)};
         cm::random_move(src, dst, P.nv, P.iref, g.cell, uc, ua);
     }
@@ -2179,9 +2170,8 @@ __device__ __forceinline__ void b_place_near(KP P, Bufs B, const TV* xv, int bx_
         int64_t lk = (int64_t)f * P.kb;
         int nb = ldg(B.nbulk + lk);
         int64_t jmol = nb > 0 ? (int64_t)ldg(B.blist + lk * P.Mv + cm::draw_index(r.x, nb)) : cm::draw_index(r.x, P.Mv);
-        MolSrcG<TV> src{(gptr<const TV>)(xv + (int64_t)f * P.Nv * 3 + jmol * P.nv * 3)};
-        MolDstG dst{(gptr<double>)(B.P + (fk * P.Nv + (int64_t)pos * P.nv) * 3),
-                    (gptr<f4v>)(B.vw_r + fk * P.Nv + (int64_t)pos * P.nv)};
+        MolSrc<TV> src(xv + (int64_t)f * P.Nv * 3 + jmol * P.nv * 3);
+        MolDst dst(B.P + (fk * P.Nv + (int64_t)pos * P.nv) * 3, B.vw_r + fk * P.Nv + (int64_t)pos * P.nv);
         double uc[3] = {cm::u01(r.y), cm::u01(r.z), cm::u01(r.w)};
         double ua[3] = {cm::u01(r2.x), cm::u01(r2.y), cm::u01(r2.z)};
         // the placed centre is H uc (a rigid copy of a molecule of the frame):
@@ -2190,7 +2180,7 @@ __device__ __forceinline__ void b_place_near(KP P, Bufs B, const TV* xv, int bx_
         cell_of_frac(g, uc[0], uc[1], uc[2], cell);
         const float ub = molecule_bound(g, bound_loads(P, B, g, f, cell), cell, uc);
         cm::random_move(src, dst, P.nv, P.iref, g.cell, uc, ua);
-        ((gptr<float>)dst.w)[3] = ub;
+        dst.set_w0(ub);
     }
 }
 
