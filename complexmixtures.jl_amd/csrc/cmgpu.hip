@@ -2035,9 +2035,11 @@ __device__ __forceinline__ void b_scan_rows(KP P, Bufs B, int p0, int np, int bx
         int woff = 0, tot = 0;
         for (int w = 0; w < nw; w++) { if (w < wid) woff += wsum[w]; tot += wsum[w]; }
         int e = carry + woff + incl - tsum;
+        // starts of empty keys are never read (the scatter reads the keys it
+        // holds molecules of): most keys are empty, so only the others are written
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            if (i0 + u < n) st[i0 + u] = e;
+            if (i0 + u < n && v[u] != 0) st[i0 + u] = e;
             e += v[u];
         }
         __syncthreads();
