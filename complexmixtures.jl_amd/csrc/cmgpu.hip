@@ -162,6 +162,7 @@ struct Bufs {
     unsigned* dfield;        // [2][F][cap] 0xffffffff - bits of the distances from a near cell's centre and
                              // lower corner to the nearest solute atom image nearby (0: none), b_dfield
     int* mkey;               // [F][NP][Mv] Morton key of the centre's cell, -1 when far
+    int* mrank;              // [F][NP][Mv] rank of a near molecule among those of its key (count order)
     int* mcnt;               // [F][NP][kcap+1]
     int* mstart;             // [F][NP][kcap+1]
     int* msorted;            // [F][NP][Mv]
@@ -1951,7 +1952,7 @@ __device__ __forceinline__ void b_prep_solvent_tiles(KP P, Bufs B, const T* xv, 
         if (!B.near[(int64_t)f * P.cap + cell]) { B.mkey[kb] = -1; continue; }
         const int key = morton_key(g, cell);
         B.mkey[kb] = key;
-        atomicAdd(&B.mcnt[(int64_t)f * P.NP * (P.kcap + 1) + key], 1);
+        B.mrank[kb] = atomicAdd(&B.mcnt[(int64_t)f * P.NP * (P.kcap + 1) + key], 1);
     }
 }
 
@@ -2069,10 +2070,9 @@ __device__ __forceinline__ void b_mol_scatter(KP P, Bufs B, int p0, int np, int 
         int64_t fp = (int64_t)f * P.NP + p;
         int key = B.mkey[fp * P.Mv + v];
         if (key < 0) continue;
-        int64_t cb = fp * (P.kcap + 1);
-        // the key's count (left by the scan) counted down: each molecule
-        // takes one slot of the key's range
-        int pos = B.mstart[cb + key] + atomicSub(&B.mcnt[cb + key], 1) - 1;
+        // the key's range starts where the scan put it; the molecule's rank
+        // within the key was taken when the key was counted
+        const int pos = B.mstart[fp * (P.kcap + 1) + key] + B.mrank[fp * P.Mv + v];
         B.msorted[fp * P.Mv + pos] = v;
     }
 }
@@ -2143,7 +2143,7 @@ __device__ __forceinline__ void b_place_keys(KP P, Bufs B, int bx_, int nbx_, in
         if (!B.near[(int64_t)f * P.cap + cell]) { B.mkey[fp * P.Mv + v] = -1; continue; }
         const int key = morton_key(g, cell);
         B.mkey[fp * P.Mv + v] = key;
-        atomicAdd(&B.mcnt[fp * (P.kcap + 1) + key], 1);
+        B.mrank[fp * P.Mv + v] = atomicAdd(&B.mcnt[fp * (P.kcap + 1) + key], 1);
     }
 }
 
@@ -3377,7 +3377,7 @@ struct cm_job {
     // batch buffers
     DevBuf geo, cells_cnt, cells_start, tmp4, tmpcell, sorted, solw, ucen, vw, vcell, vcen;
     DevBuf P, vw_r, vcen_r, bflag, blist, nbulk, uk, drawn_ptr, drawn_idx;
-    DevBuf near, mkey, mcnt, mstart, msorted, nnear, chunk_base;
+    DevBuf near, mkey, mrank, mcnt, mstart, msorted, nnear, chunk_base;
     DevBuf xin_u, xin_v;
     double cell_frac = 0.5;   // tiles family: fine cell side as a fraction of the cutoff
     int tile_c = 1;           // tiles family: columns per atom row (1, 2, 4 or 8)
@@ -3463,7 +3463,7 @@ int64_t per_frame_bytes(const cm_job* J, int dtype_size, bool host_input, int64_
         b += J->Nv * 16 + Mv * 20;
         b += (int64_t)J->K * (J->Nv * (24 + 16) + Mv * 16);
     } else {
-        b += cap * 9 + NP * ((kcap + 1) * 8 + Mv * 8 + 4) + 8;
+        b += cap * 9 + NP * ((kcap + 1) * 8 + Mv * 12 + 4) + 8;
         b += J->Nv * 16 + (int64_t)J->K * J->Nv * (24 + 16);
     }
     b += (int64_t)J->kb * Mv * 5 + J->kb * 4;
@@ -3653,6 +3653,7 @@ Bufs batch_bufs(const cm_job* J, Geo* geo)
     B.drawn_idx = (int*)J->drawn_idx.p;
     B.near = (unsigned char*)J->near.p;
     B.mkey = (int*)J->mkey.p;
+    B.mrank = (int*)J->mrank.p;
     B.mcnt = (int*)J->mcnt.p;
     B.mstart = (int*)J->mstart.p;
     B.msorted = (int*)J->msorted.p;
@@ -3871,6 +3872,7 @@ int alloc_batch(cm_job* J, int F, int64_t cap, int64_t kcap, int dtype_size, boo
     } else {
         if ((r = ensure(J, J->near, near_bytes(F, cap)))) return r;
         if ((r = ensure(J, J->mkey, (size_t)4 * F * NP * Mv))) return r;
+        if ((r = ensure(J, J->mrank, (size_t)4 * F * NP * Mv))) return r;
         if ((r = ensure(J, J->mcnt, round16((size_t)4 * F * NP * (kcap + 1))))) return r;
         if ((r = ensure(J, J->mstart, (size_t)4 * F * NP * (kcap + 1)))) return r;
         if ((r = ensure(J, J->msorted, (size_t)4 * F * NP * Mv))) return r;
@@ -4649,7 +4651,7 @@ void cm_job_destroy(cm_job* J)
     DevBuf* bufs[] = {&J->cnt, &J->acc, &J->hits, &J->stats, &J->gmap, &J->geo, &J->cells_cnt, &J->cells_start,
                       &J->tmp4, &J->tmpcell, &J->sorted, &J->solw, &J->ucen, &J->vw, &J->vcell, &J->vcen, &J->P,
                       &J->vw_r, &J->vcen_r, &J->bflag, &J->blist, &J->nbulk, &J->uk, &J->drawn_ptr,
-                      &J->drawn_idx, &J->xin_u, &J->xin_v, &J->near, &J->mkey, &J->mcnt, &J->mstart,
+                      &J->drawn_idx, &J->xin_u, &J->xin_v, &J->near, &J->mkey, &J->mrank, &J->mcnt, &J->mstart,
                       &J->msorted, &J->nnear, &J->chunk_base, &J->vrec};
     for (DevBuf* b : bufs) dev_put(*b);
     pin_put(J->hres);
@@ -5450,7 +5452,7 @@ int cm_mddf_noalloc(const cm_job_desc* desc, int32_t nframes, const int64_t* fra
     DevBuf* bufs[] = {&J->cnt, &J->acc, &J->hits, &J->stats, &J->gmap, &J->geo, &J->cells_cnt, &J->cells_start,
                       &J->tmp4, &J->tmpcell, &J->sorted, &J->solw, &J->ucen, &J->vw, &J->vcell, &J->vcen, &J->P,
                       &J->vw_r, &J->vcen_r, &J->bflag, &J->blist, &J->nbulk, &J->uk, &J->drawn_ptr,
-                      &J->drawn_idx, &J->xin_u, &J->xin_v, &J->near, &J->mkey, &J->mcnt, &J->mstart,
+                      &J->drawn_idx, &J->xin_u, &J->xin_v, &J->near, &J->mkey, &J->mrank, &J->mcnt, &J->mstart,
                       &J->msorted, &J->nnear, &J->chunk_base};
     for (DevBuf* b : bufs) *b = DevBuf{};
     J->arena = (char*)D.ptr;
