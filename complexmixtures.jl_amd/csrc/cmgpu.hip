@@ -239,11 +239,31 @@ __device__ __forceinline__ void load3(const T* p, int64_t atom, double& x, doubl
 
 __device__ __forceinline__ float4 mk4(float x, float y, float z, float w) { return make_float4(x, y, z, w); }
 
+__device__ __forceinline__ void cell_of_frac(const Geo& g, double f0, double f1, double f2, int& cell, int& c0, int& c1,
+                                             int& c2)
+{
+    c0 = (int)(f0 * g.nc[0]); if (c0 >= g.nc[0]) c0 = g.nc[0] - 1; if (c0 < 0) c0 = 0;
+    c1 = (int)(f1 * g.nc[1]); if (c1 >= g.nc[1]) c1 = g.nc[1] - 1; if (c1 < 0) c1 = 0;
+    c2 = (int)(f2 * g.nc[2]); if (c2 >= g.nc[2]) c2 = g.nc[2] - 1; if (c2 < 0) c2 = 0;
+    cell = (c2 * g.nc[1] + c1) * g.nc[0] + c0;
+}
+
 __device__ __forceinline__ void cell_of_frac(const Geo& g, double f0, double f1, double f2, int& cell)
 {
-    int c0 = (int)(f0 * g.nc[0]); if (c0 >= g.nc[0]) c0 = g.nc[0] - 1; if (c0 < 0) c0 = 0;
-    int c1 = (int)(f1 * g.nc[1]); if (c1 >= g.nc[1]) c1 = g.nc[1] - 1; if (c1 < 0) c1 = 0;
-    int c2 = (int)(f2 * g.nc[2]); if (c2 >= g.nc[2]) c2 = g.nc[2] - 1; if (c2 < 0) c2 = 0;
+    int c0, c1, c2;
+    cell_of_frac(g, f0, f1, f2, cell, c0, c1, c2);
+}
+
+// cell_of_frac of the fractional coordinates u01(r0), u01(r1), u01(r2), in
+// integers: r 2^-32 is exact in FP64 and so is its product with nc (at most
+// 32 + 21 significant bits), so (int)(u01(r) nc) = floor(r nc / 2^32) =
+// (r nc) >> 32, always below nc
+__device__ __forceinline__ void cell_of_draws(const Geo& g, uint32_t r0, uint32_t r1, uint32_t r2, int& cell, int& c0,
+                                              int& c1, int& c2)
+{
+    c0 = (int)(((uint64_t)r0 * (uint64_t)g.nc[0]) >> 32);
+    c1 = (int)(((uint64_t)r1 * (uint64_t)g.nc[1]) >> 32);
+    c2 = (int)(((uint64_t)r2 * (uint64_t)g.nc[2]) >> 32);
     cell = (c2 * g.nc[1] + c1) * g.nc[0] + c0;
 }
 
@@ -1681,9 +1701,21 @@ __device__ __forceinline__ int wrap_index(int i, int n, int& m)
 
 // Morton code of a fine cell: chunks of consecutive molecules in this order
 // stay compact in space
-__device__ __forceinline__ int morton_key(const Geo& g, int cell)
+// bits 0..9 of x moved to bits 0, 3, 6, ..., 27
+__device__ __forceinline__ unsigned spread3(unsigned x)
 {
-    int c0 = cell % g.nc[0], c1 = (cell / g.nc[0]) % g.nc[1], c2 = cell / (g.nc[0] * g.nc[1]);
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// Morton code of the fine cell (c0, c1, c2), each below 2^kbits
+__device__ __forceinline__ int morton_key3(const Geo& g, int c0, int c1, int c2)
+{
+    if (g.kbits <= 10) return (int)(spread3(c0) | (spread3(c1) << 1) | (spread3(c2) << 2));
     unsigned k = 0;
     for (int b = 0; b < g.kbits; b++) {
         k |= (((unsigned)c0 >> b) & 1u) << (3 * b);
@@ -1946,11 +1978,11 @@ __device__ __forceinline__ void b_prep_solvent_tiles(KP P, Bufs B, const T* xv, 
         mol_centre(g, xv + (int64_t)f * P.Nv * 3, (int64_t)v * P.nv, P.nv, c);
         double fr[3], n[3];
         wrap_point(g, c[0], c[1], c[2], fr, n);
-        int cell;
-        cell_of_frac(g, fr[0], fr[1], fr[2], cell);
+        int cell, c0, c1, c2;
+        cell_of_frac(g, fr[0], fr[1], fr[2], cell, c0, c1, c2);
         int64_t kb = (int64_t)f * P.NP * P.Mv + v;
         if (!B.near[(int64_t)f * P.cap + cell]) { B.mkey[kb] = -1; continue; }
-        const int key = morton_key(g, cell);
+        const int key = morton_key3(g, c0, c1, c2);
         B.mkey[kb] = key;
         B.mrank[kb] = atomicAdd(&B.mcnt[(int64_t)f * P.NP * (P.kcap + 1) + key], 1);
     }
@@ -2138,10 +2170,10 @@ __device__ __forceinline__ void b_place_keys(KP P, Bufs B, int bx_, int nbx_, in
         const Geo g = ldc_geo(B.geo + f);
         int64_t fp = (int64_t)f * P.NP + 1 + k;
         cm::u32x4 r = cm::philox4x32_10({(uint32_t)v, (uint32_t)k, g.frame, cm::TAG_SLOT}, P.key0, P.key1);
-        int cell;
-        cell_of_frac(g, cm::u01(r.y), cm::u01(r.z), cm::u01(r.w), cell);
+        int cell, c0, c1, c2;
+        cell_of_draws(g, r.y, r.z, r.w, cell, c0, c1, c2);
         if (!B.near[(int64_t)f * P.cap + cell]) { B.mkey[fp * P.Mv + v] = -1; continue; }
-        const int key = morton_key(g, cell);
+        const int key = morton_key3(g, c0, c1, c2);
         B.mkey[fp * P.Mv + v] = key;
         B.mrank[fp * P.Mv + v] = atomicAdd(&B.mcnt[fp * (P.kcap + 1) + key], 1);
     }
