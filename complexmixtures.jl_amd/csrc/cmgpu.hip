@@ -1783,6 +1783,20 @@ __device__ __forceinline__ void b_near_mark(KP P, Bufs B, int bx_, int nbx_, int
         const float rn2 = g.rnear * g.rnear;
         const float hx = g.H32[0] / g.nc[0], hy = g.H32[4] / g.nc[1], hz = g.H32[8] / g.nc[2];
         auto gap = [](float v, float lo, float hi) { return fmaxf(fmaxf(lo - v, v - hi), 0.f); };
+        // cells a..b (0 <= a <= b < nc0) of the row starting at cell rb: a
+        // bit range, one LDS atomic per word
+        auto mark_run = [&](int rb, int a, int b) {
+            const int c0 = rb + a, c1 = rb + b;
+            if (lds) {
+                for (int w = c0 >> 5; w <= (c1 >> 5); w++) {
+                    const int lo = w == (c0 >> 5) ? (c0 & 31) : 0, hi = w == (c1 >> 5) ? (c1 & 31) : 31;
+                    const unsigned m = (hi == 31 ? 0xffffffffu : (2u << hi) - 1u) & ~((1u << lo) - 1u);
+                    atomicOr(&bits[w], m);
+                }
+            } else {
+                for (int c = c0; c <= c1; c++) nr[c] = 1;
+            }
+        };
         for (int64_t at = bx_ * (int64_t)blockDim.x + threadIdx.x; at < P.nu; at += (int64_t)nbx_ * blockDim.x) {
             const float4 s = B.tmp4[(int64_t)f * P.nu + at];
             int ilo[3], ihi[3];
@@ -1793,14 +1807,30 @@ __device__ __forceinline__ void b_near_mark(KP P, Bufs B, int bx_, int nbx_, int
                 for (int iy = ilo[1]; iy <= ihi[1]; iy++) {
                     int my; int cy = wrap_index(iy, g.nc[1], my);
                     const float dy = g.ortho ? gap(s.y, iy * hy, (iy + 1) * hy) : 0.f;
-                    if (dz * dz + dy * dy > rn2) continue;
-                    for (int ix = ilo[0]; ix <= ihi[0]; ix++) {
-                        int mx; int cx = wrap_index(ix, g.nc[0], mx);
-                        const float dx = g.ortho ? gap(s.x, ix * hx, (ix + 1) * hx) : 0.f;
-                        if (dz * dz + dy * dy + dx * dx > rn2) continue;
-                        const int cell = (cz * g.nc[1] + cy) * g.nc[0] + cx;
-                        if (lds) atomicOr(&bits[cell >> 5], 1u << (cell & 31));
-                        else nr[cell] = 1;
+                    const float r2 = rn2 - dz * dz - dy * dy;
+                    if (r2 < 0.f) continue;
+                    // the x cells of the row within rnear (diagonal cells: their
+                    // box gap <= sqrt(r2), widened over FP32 rounding, which
+                    // only adds cells), as at most two runs of wrapped cells
+                    int x0 = ilo[0], x1 = ihi[0];
+                    if (g.ortho) {
+                        const float rx = sqrtf(r2) * (1.0f + 1e-6f) + 1e-4f;
+                        x0 = max(x0, (int)floorf((s.x - rx) / hx));
+                        x1 = min(x1, (int)floorf((s.x + rx) / hx));
+                    }
+                    if (x0 > x1) continue;
+                    const int rb = (cz * g.nc[1] + cy) * g.nc[0];
+                    if (x1 - x0 + 1 >= g.nc[0]) {
+                        mark_run(rb, 0, g.nc[0] - 1);
+                    } else {
+                        int m0, m1;
+                        const int a = wrap_index(x0, g.nc[0], m0), b = wrap_index(x1, g.nc[0], m1);
+                        if (a <= b) {
+                            mark_run(rb, a, b);
+                        } else {
+                            mark_run(rb, a, g.nc[0] - 1);
+                            mark_run(rb, 0, b);
+                        }
                     }
                 }
             }
