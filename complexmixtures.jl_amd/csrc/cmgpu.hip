@@ -1946,13 +1946,14 @@ struct BoundLoads {
 
 // the entries, read by the callers before their stores (which may alias them
 // as far as the compiler knows, and would hold the loads back)
-__device__ __forceinline__ BoundLoads bound_loads(const KP& P, const Bufs& B, const Geo& g, int f, int cell)
+__device__ __forceinline__ BoundLoads bound_loads(const KP& P, const Bufs& B, const Geo& g, int f, int cell, int c0,
+                                                  int c1, int c2)
 {
+    // cell = (c2 nc1 + c1) nc0 + c0
     BoundLoads L;
     const gptr<const unsigned> d = (gptr<const unsigned>)(B.dfield + (int64_t)f * P.cap);
     const gptr<const unsigned> dc = d + (int64_t)P.F * P.cap;
     L.e[0] = d[cell];
-    const int c0 = cell % g.nc[0], c1 = (cell / g.nc[0]) % g.nc[1], c2 = cell / (g.nc[0] * g.nc[1]);
 #pragma unroll
     for (int t = 0; t < 8; t++) {
         int i0 = c0 + (t & 1), i1 = c1 + ((t >> 1) & 1), i2 = c2 + ((t >> 2) & 1);
@@ -1964,11 +1965,14 @@ __device__ __forceinline__ BoundLoads bound_loads(const KP& P, const Bufs& B, co
     return L;
 }
 
-__device__ __forceinline__ float molecule_bound(const Geo& g, const BoundLoads& L, int cell, const double fr[3])
+__device__ __forceinline__ float molecule_bound(const Geo& g, const BoundLoads& L, int c0, int c1, int c2,
+                                                const double fr[3])
 {
-    const int c0 = cell % g.nc[0], c1 = (cell / g.nc[0]) % g.nc[1], c2 = cell / (g.nc[0] * g.nc[1]);
     const float* H = g.H32;
-    const float i0 = 1.0f / g.nc[0], i1 = 1.0f / g.nc[1], i2 = 1.0f / g.nc[2];
+    // (hardware reciprocals, 1 ulp: a relative 2^-23 in the offsets below is
+    // far inside the bound's margin)
+    const float i0 = __builtin_amdgcn_rcpf((float)g.nc[0]), i1 = __builtin_amdgcn_rcpf((float)g.nc[1]),
+                i2 = __builtin_amdgcn_rcpf((float)g.nc[2]);
     // centre -> cell centre, and the half-cell steps along the three cell
     // vectors: centre -> corner (a, b, c) = e + (2a - 1) h0 + (2b - 1) h1 + (2c - 1) h2
     const float d0 = (float)fr[0] - (c0 + 0.5f) * i0, d1 = (float)fr[1] - (c1 + 0.5f) * i1,
@@ -2050,9 +2054,9 @@ __device__ __forceinline__ void b_fill_real(KP P, Bufs B, const T* xv, int bx_, 
         double x0, y0, z0;
         load3(x, a0, x0, y0, z0);
         // the first atom's w carries the molecule's nearest-solute bound
-        int cell;
-        cell_of_frac(g, fr[0], fr[1], fr[2], cell);
-        const float ub = molecule_bound(g, bound_loads(P, B, g, f, cell), cell, fr);
+        int cell, c0, c1, c2;
+        cell_of_frac(g, fr[0], fr[1], fr[2], cell, c0, c1, c2);
+        const float ub = molecule_bound(g, bound_loads(P, B, g, f, cell, c0, c1, c2), c0, c1, c2, fr);
         float4* dst = B.vw + (int64_t)f * P.Nv + (int64_t)pos * P.nv;
         for (int a = 0; a < P.nv; a++) {
             double xa, ya, za, ox, oy, oz;
@@ -2240,9 +2244,9 @@ __device__ __forceinline__ void b_place_near(KP P, Bufs B, const TV* xv, int bx_
         double ua[3] = {cm::u01(r2.x), cm::u01(r2.y), cm::u01(r2.z)};
         // the placed centre is H uc (a rigid copy of a molecule of the frame):
         // the first atom's w carries the molecule's nearest-solute bound
-        int cell;
-        cell_of_frac(g, uc[0], uc[1], uc[2], cell);
-        const float ub = molecule_bound(g, bound_loads(P, B, g, f, cell), cell, uc);
+        int cell, c0, c1, c2;
+        cell_of_frac(g, uc[0], uc[1], uc[2], cell, c0, c1, c2);
+        const float ub = molecule_bound(g, bound_loads(P, B, g, f, cell, c0, c1, c2), c0, c1, c2, uc);
         cm::random_move(src, dst, P.nv, P.iref, g.cell, uc, ua);
         dst.set_w0(ub);
     }
